@@ -1,0 +1,177 @@
+#!/usr/bin/env python
+"""Benchmark: multigrid-PCG throughput on one MI355X.
+
+One step = one full solve of the synthetic ellipsoid-union elasticity problem
+to relative residual 1e-6 with the mixed-precision solver (FP32 V-cycle inside
+an FP64 flexible PCG), inputs resident in HBM, set-up excluded.  The FP64
+V-cycle solve of the same problem runs once beside it, and the dominant
+kernel (the level-1 FP64 matrix-free matvec q = A p) is timed with HIP events
+on the solver's stream during the timed steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    root = REPO / "gmgs-3d_amd"
+    spec = importlib.util.spec_from_file_location(
+        "gmgs3d_amd", root / "__init__.py", submodule_search_locations=[str(root)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gmgs3d_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    --pmc summary (line 'traffic_bytes_per_launch: N'), or None."""
+    f = REPO / "profiles" / f"{name}_traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(problem, its_fp64, iters):
+    """Oracle FP64 PCG (reference algorithm) on the host: set-up untimed, then
+    ``iters`` timed iterations; time to solution = per-iteration time x the
+    FP64 iteration count."""
+    sys.path.insert(0, str(REPO))
+    from oracle import gmgs_oracle as oracle
+    m = oracle.build(problem.vox, problem.bbox, num_levels=problem.num_levels, omega=problem.omega)
+    oracle.apply_bc(m, problem.loads, problem.fixed)
+    oracle.factor_coarsest(m)
+    t0 = time.perf_counter()
+    oracle.cg_solver(m, m.F, 1e-30, iters - 1)      # exactly `iters` iterations
+    dt = (time.perf_counter() - t0) / iters
+    ndofs = m.levels[0].numDOFs
+    return ndofs / (dt * its_fp64) / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU slab decomposition is "
+              "designed (see DESIGN.md) but not implemented. Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_package()
+    wl = pkg.workloads
+    t0 = time.perf_counter()
+    prob = wl.ellipsoids_c2(args.seed) if args.config == "c2" else wl.ellipsoids_c3(args.seed)
+    log(f"[bench] generated {prob.name} in {time.perf_counter() - t0:.1f}s")
+    common = dict(num_levels=prob.num_levels, jacobi_weight=prob.omega, tol=1e-6,
+                  max_it=prob.max_it, flags=pkg.FLAG_NO_INDEX_TABLES)
+
+    t0 = time.perf_counter()
+    s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, **common)
+    s.set_bc(prob.fixed, prob.loads)
+    setup_s = time.perf_counter() - t0
+    nn = s.num_nodes(1)
+    ndofs = 3 * nn
+    log(f"[bench] mixed solver set up in {setup_s:.2f}s: {ndofs} DOFs, {s.num_levels} levels")
+
+    for _ in range(args.warmup):
+        s.solve()
+    s.kernel_timing(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        U, info, _ = s.solve()
+        log(f"[bench] step {k}: its {info.its} relres {info.relres:.3e} "
+            f"true {info.true_relres:.3e} {info.solve_seconds * 1e3:.1f} ms")
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) / args.steps * 1e3
+    kt = s.kernel_times()
+    value = ndofs / (ms_per_step * 1e-3) / 1e6
+
+    # FP64 V-cycle (reference algorithm, Fletcher-Reeves) on the same problem
+    s64 = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_FP64, **common)
+    s64.set_bc(prob.fixed, prob.loads)
+    s64.solve()
+    t0 = time.perf_counter()
+    U64, info64, _ = s64.solve()
+    ms64 = (time.perf_counter() - t0) * 1e3
+    err = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
+    log(f"[bench] fp64: its {info64.its} {ms64:.1f} ms; mixed vs fp64 rel L2 {err:.2e}")
+
+    # roofline of the level-1 FP64 matvec: x read once, y written once, one
+    # 32-bit flag word per node
+    mv_ms, mv_n = kt["matvec64"]
+    alg_bytes = 2 * 8 * ndofs + 4 * nn
+    achieved = alg_bytes / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else None
+    traffic = read_traffic(f"matvec64_{args.config}")
+
+    cpu = None
+    if not args.no_cpu_baseline and args.config == "c2":
+        log(f"[bench] cpu baseline: oracle set-up + {args.cpu_iters} timed iterations")
+        v, dt = cpu_baseline(prob, info64.its, args.cpu_iters)
+        cpu = {"value": round(v, 5), "unit": "MDOF/s", "cores": 1, "kind": "port",
+               "sample": (f"numpy FP64 oracle (reference algorithm) on the same {prob.name} "
+                          f"problem: {args.cpu_iters} PCG iterations timed ({dt:.2f} s each), "
+                          f"scaled to the {info64.its} iterations the FP64 solve needs; "
+                          f"set-up excluded; numpy single-threaded")}
+    line = {
+        "metric": "MDOF/s to rel. residual 1e-6, mixed-prec vs FP64 V-cycle; matvec HBM GB/s",
+        "value": round(value, 3),
+        "unit": "MDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle / f64 PCG",
+        "data": "synthetic (seeded ellipsoid-union voxel solid)",
+        "config": {"workload": prob.name, "dofs": ndofs, "levels": s.num_levels,
+                   "rtol": 1e-6, "its_mixed": info.its, "its_fp64": info64.its,
+                   "fp64_ms": round(ms64, 3),
+                   "fp64_mdofs": round(ndofs / (ms64 * 1e-3) / 1e6, 3),
+                   "speedup_vs_fp64": round(ms64 / ms_per_step, 3),
+                   "mixed_vs_fp64_rel_l2": err,
+                   "setup_s": round(setup_s, 3)},
+        "roofline": {"kernel": "k_apply<f64,AX> (level-1 matvec q = A p)", "bound": "hbm",
+                     "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_ms": round(mv_ms, 5), "launches": mv_n},
+        "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    s.close()
+    s64.close()
+
+
+if __name__ == "__main__":
+    main()

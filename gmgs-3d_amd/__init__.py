@@ -1,2 +1,10 @@
-"""MI355X-native geometric-multigrid PCG for voxel hex8 elasticity."""
+"""MI355X-native geometric-multigrid PCG for voxel hex8 elasticity.
+
+The compute path is libgmgs.so (hand-written HIP for gfx950, C ABI in
+include/gmgs.h); this package is its Python host side.
+"""
 from . import workloads  # noqa: F401
+from .gmgs import (GMGS3D, GMGSError, NotConverged, Solver, default_params,  # noqa: F401
+                   load_library, library_path, exported_symbols,
+                   GMGS_PREC_FP64, GMGS_PREC_MIXED, FLAG_EXACT_GALERKIN, FLAG_NO_INDEX_TABLES,
+                   FLAG_FR_BETA, FLAG_PR_BETA)

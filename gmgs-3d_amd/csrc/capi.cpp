@@ -1,0 +1,1312 @@
+// Context, set-up orchestration, PCG driver and the C ABI (include/gmgs.h).
+#include "../../include/gmgs.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "dense.h"
+#include "fe_host.h"
+#include "mesh.h"
+#include "ops.h"
+#include "setup.h"
+
+using namespace gmgs;
+
+namespace {
+
+struct Status : std::runtime_error {
+  int code;
+  Status(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct Level {
+  LevelMesh mesh;
+  DevBuf tiles;
+  long long ntiles = 0;
+  int state = 0;                 // storingState (SetupSolver.m:10,41,70,...)
+  DevBuf eop;                    // int32 per padded cell (levels >= 2)
+  DevBuf Ks64, Ks32;             // stored element matrices
+  long long nKs = 0;
+  DevBuf Ke64, Ke32, S64, S32;   // universal matrix and interior stencil of this level
+  DevBuf invd64, invd32;
+  DevBuf uniq;                   // int32 compact ids of unique elements (state 2)
+  long long nuniq = 0;
+  DevBuf r, rp, x;               // V-cycle work vectors (precision of the cycle)
+  double scale = 1.0;            // Ke_l = Ke0 * scale
+};
+
+}  // namespace
+
+struct gmgs_ctx {
+  gmgs_params prm{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int L = 0;
+  int nely = 0, nelx = 0, nelz = 0;       // input voxel dims
+  double bbox[6] = {0, 0, 0, 0, 0, 0};     // adjusted [min xyz, max xyz]
+  double eleSize[3] = {1, 1, 1};
+  double Ke0[576];
+  double M[48 * 24];
+  std::vector<Level> lev;
+  bool mixed = false;
+  bool flexible = false;
+  bool exact = false;
+  bool tables = true;
+  // SIMP
+  bool simp = false;
+  DevBuf emod64, emod32;
+  DevBuf stressM;
+  // boundary conditions
+  bool bc = false;
+  DevBuf b;                 // dense FP64 load vector (level 1)
+  std::vector<double> F;    // compact F_
+  std::vector<int32_t> bc_fixed;
+  std::vector<double> bc_loads;
+  int64_t bc_m = 0;
+  DevBuf free_code, dofmap, A, W, Kinv, gv_b, gv_part;
+  long long nfree = 0, nfp = 0;
+  // PCG state
+  DevBuf x, r, ra, z[2], p, q, tmp, tmp2, part, sc, hist_rel, hist_t;
+  long long nparts = 0;
+  bool have_solution = false;
+  std::string err;
+  double clock_khz = 100000.0;
+  // kernel timing (HIP events around level-1 launches on this stream)
+  bool ktime = false;
+  struct KPair { hipEvent_t a, b; int kernel, iter; };
+  std::vector<KPair> kpairs;
+  std::vector<hipEvent_t> kpool;
+  size_t kpool_used = 0;
+  double ktot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long long kcnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int cur_iter = 0;
+};
+
+namespace {
+
+Level& L1(gmgs_ctx* c) { return c->lev[0]; }
+
+hipEvent_t kt_event(gmgs_ctx* c) {
+  if (c->kpool_used == c->kpool.size()) {
+    hipEvent_t e;
+    GMGS_HIP(hipEventCreate(&e));
+    c->kpool.push_back(e);
+  }
+  return c->kpool[c->kpool_used++];
+}
+
+// RAII bracket: records an event pair around one kernel when timing is on
+struct KTime {
+  gmgs_ctx* c;
+  int kernel;
+  hipEvent_t a = nullptr;
+  KTime(gmgs_ctx* cc, int k) : c(cc), kernel(k) {
+    if (c->ktime) {
+      a = kt_event(c);
+      GMGS_HIP(hipEventRecord(a, c->stream));
+    }
+  }
+  ~KTime() {
+    if (a) {
+      hipEvent_t b = kt_event(c);
+      (void)hipEventRecord(b, c->stream);
+      c->kpairs.push_back({a, b, kernel, c->cur_iter});
+    }
+  }
+};
+
+// fold the recorded pairs of one solve into the totals (iterations < its)
+void kt_collect(gmgs_ctx* c, int its) {
+  for (const auto& p : c->kpairs) {
+    if (p.iter >= its) continue;
+    float ms = 0.f;
+    GMGS_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    c->ktot[p.kernel] += ms;
+    c->kcnt[p.kernel] += 1;
+  }
+  c->kpairs.clear();
+  c->kpool_used = 0;
+}
+
+template <typename T> const T* Kptr(const Level& l);
+template <> const double* Kptr<double>(const Level& l) { return l.Ke64.as<double>(); }
+template <> const float* Kptr<float>(const Level& l) { return l.Ke32.as<float>(); }
+
+template <typename T>
+LevelOp<T> make_op(gmgs_ctx* c, int li) {
+  Level& l = c->lev[li];
+  LevelOp<T> op;
+  op.g = l.mesh.g;
+  op.flags = l.mesh.flags.as<uint32_t>();
+  op.eop = li == 0 ? nullptr : l.eop.as<int>();
+  if constexpr (std::is_same<T, double>::value) {
+    op.Ks = l.Ks64.empty() ? nullptr : l.Ks64.as<double>();
+    op.Ke = l.Ke64.as<double>();
+    op.stencil = l.S64.as<double>();
+    op.invd = l.invd64.empty() ? nullptr : l.invd64.as<double>();
+    op.emod = (li == 0 && c->simp) ? c->emod64.as<double>() : nullptr;
+  } else {
+    op.Ks = l.Ks32.empty() ? nullptr : l.Ks32.as<float>();
+    op.Ke = l.Ke32.as<float>();
+    op.stencil = l.S32.as<float>();
+    op.invd = l.invd32.empty() ? nullptr : l.invd32.as<float>();
+    op.emod = (li == 0 && c->simp) ? c->emod32.as<float>() : nullptr;
+  }
+  return op;
+}
+
+template <typename T>
+void upload(DevBuf& d, const T* h, size_t n, hipStream_t s) {
+  d.alloc(sizeof(T) * n);
+  GMGS_HIP(hipMemcpyAsync(d.as<void>(), h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+}
+
+template <typename T>
+std::vector<T> download(const DevBuf& d, size_t n, hipStream_t s) {
+  std::vector<T> h(n);
+  if (n) GMGS_HIP(hipMemcpyAsync(h.data(), d.as<void>(), sizeof(T) * n, hipMemcpyDeviceToHost, s));
+  GMGS_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+void to_f32(const DevBuf& a, DevBuf& b, long long n, hipStream_t s) {
+  b.alloc(sizeof(float) * (n > 0 ? n : 1));
+  launch_convert<double, float>(a.as<double>(), b.as<float>(), n, s);
+}
+
+// MATLAB a:d:b evaluated from both ends
+double colon_value(double a, double d, double b, long long n, long long k) {
+  return (2 * k >= n) ? b - (double)(n - k) * d : a + (double)k * d;
+}
+
+// ------------------------------------------------------------------ set-up
+void build_mesh(gmgs_ctx* c, const int32_t* vox) {
+  hipStream_t s = c->stream;
+  const gmgs_params& p = c->prm;
+  const long long nelx = c->nelx, nely = c->nely, nelz = c->nelz;
+  int Lm1;   // levels - 1
+  if (p.num_levels > 0) {
+    Lm1 = p.num_levels - 1;
+  } else {
+    double num = (double)nelx * nely * nelz;
+    if (num < p.coarsest_resolution_control)
+      throw Status(GMGS_ERR_RESOLUTION, "There is no Sufficient Resolution to Build Mesh Hierarchy!");
+    Lm1 = 0;
+    while (num >= p.coarsest_resolution_control) {
+      ++Lm1;
+      const double f = std::pow(2.0, Lm1);
+      num = nelx / f * nely / f * nelz / f;
+    }
+  }
+  c->L = Lm1 + 1;
+  if (c->L < 2) throw Status(GMGS_ERR_LEVELS, "There is no sufficient resolution to invoke GMGS!");
+  const long long sp = 1LL << Lm1;
+  const long long ax = (nelx + sp - 1) / sp * sp, ay = (nely + sp - 1) / sp * sp,
+                  az = (nelz + sp - 1) / sp * sp;
+  // bounding box stretch (DiscretizeDesignDomain.m:57-58) and element size (:65)
+  const long long res[3] = {ax, ay, az};
+  const long long nel[3] = {nelx, nely, nelz};
+  for (int d = 0; d < 3; ++d) {
+    const double lo = c->bbox[d], hi = c->bbox[3 + d];
+    c->bbox[3 + d] = lo + (hi - lo) * (double)res[d] / (double)nel[d];
+    c->eleSize[d] = (c->bbox[3 + d] - lo) / (double)res[d];
+  }
+  DevBuf dvox;
+  upload(dvox, vox, (size_t)(nelx * nely * nelz), s);
+  c->lev.resize(c->L);
+  c->lev[0].mesh.g = make_grid((int)ay, (int)ax, (int)az);
+  build_level1(c->lev[0].mesh, dvox.as<int>(), (int)nely, (int)nelx, (int)nelz, c->tables, s);
+  if (c->lev[0].mesh.ne == 0) throw Status(GMGS_ERR_RESOLUTION, "the voxel model has no solid voxel");
+  for (int l = 1; l < c->L; ++l) {
+    const Grid& gf = c->lev[l - 1].mesh.g;
+    c->lev[l].mesh.g = make_grid(gf.ey / 2, gf.ex / 2, gf.ez / 2);
+    build_coarse(c->lev[l].mesh, c->lev[l - 1].mesh, c->lev[0].mesh.g, c->tables, s);
+  }
+  for (int l = 0; l < c->L; ++l) {
+    Level& lv = c->lev[l];
+    lv.ntiles = build_tiles(lv.mesh.g, lv.mesh.flags.as<uint32_t>(), lv.tiles, s);
+  }
+}
+
+void probe_coords(gmgs_ctx* c, double probe[8][3]) {
+  // nodeCoords_ of the first solid element's corners (SetupStiffnessMatrix.m:11-13)
+  const Grid& g = L1(c).mesh.g;
+  int e1 = 0;
+  GMGS_HIP(hipMemcpyAsync(&e1, L1(c).mesh.eleMapBack.as<int>(), sizeof(int), hipMemcpyDeviceToHost,
+                          c->stream));
+  GMGS_HIP(hipStreamSynchronize(c->stream));
+  const long long f = e1 - 1;
+  const int y = (int)(f % g.ey), x = (int)((f / g.ey) % g.ex), z = (int)(f / ((long long)g.ey * g.ex));
+  const double* bb = c->bbox;
+  const double dx = (bb[3] - bb[0]) / g.ex, dy = (bb[1] - bb[4]) / g.ey, dz = (bb[5] - bb[2]) / g.ez;
+  for (int m = 0; m < 8; ++m) {
+    const int ny_ = y + hex_ro(m), nx_ = x + hex_xo(m), nz_ = z + hex_zo(m);
+    probe[m][0] = colon_value(bb[0], dx, bb[3], g.ex, nx_);
+    probe[m][1] = colon_value(bb[4], dy, bb[1], g.ey, ny_);   // ySeed descends (DiscretizeDesignDomain.m:96)
+    probe[m][2] = colon_value(bb[2], dz, bb[5], g.ez, nz_);
+  }
+}
+
+void upload_level_constants(gmgs_ctx* c, int li) {
+  Level& l = c->lev[li];
+  hipStream_t s = c->stream;
+  // Ke_l = Ke0 * eleSize_l(1)/eleSize_1(1) (SetupSolver.m:37)
+  l.scale = std::ldexp(1.0, li);
+  std::vector<double> K(576), S(243);
+  for (int i = 0; i < 576; ++i) K[i] = c->Ke0[i] * l.scale;
+  double St[27][3][3];
+  interior_stencil(K.data(), St);
+  std::memcpy(S.data(), St, sizeof(double) * 243);
+  upload(l.Ke64, K.data(), 576, s);
+  upload(l.S64, S.data(), 243, s);
+  if (c->mixed) {
+    std::vector<float> Kf(576), Sf(243);
+    for (int i = 0; i < 576; ++i) Kf[i] = (float)K[i];
+    for (int i = 0; i < 243; ++i) Sf[i] = (float)S[i];
+    upload(l.Ke32, Kf.data(), 576, s);
+    upload(l.S32, Sf.data(), 243, s);
+  }
+}
+
+// coarse element matrices and storing states (SetupSolver.m:34-196)
+void build_operators(gmgs_ctx* c) {
+  hipStream_t s = c->stream;
+  const int L = c->L;
+  const int exact = c->exact ? 1 : 0;
+  for (int li = 1; li < L; ++li) {
+    Level& l = c->lev[li];
+    Level& f = c->lev[li - 1];
+    const Grid& g = l.mesh.g;
+    const long long ne = l.mesh.ne;
+    const int lvl = li + 1;
+    l.eop.alloc(sizeof(int) * g.N);
+    launch_fill<int>(l.eop.as<int>(), -1, g.N, s);
+    std::vector<int> val;
+    DevBuf dval;
+    dval.alloc(sizeof(int) * (ne > 0 ? ne : 1));
+    const double* emodf = (li == 1 && c->simp) ? c->emod64.as<double>() : nullptr;
+    if (c->simp) {
+      // per-element Galerkin of scaled children on every coarse level
+      l.state = 1;
+    } else if ((lvl == 2 && L > 2) || (lvl == 3 && L > 3)) {
+      l.state = 2;
+    } else {
+      l.state = 1;
+    }
+    if (lvl == 2 && !c->simp) {
+      // children are level-1 universal elements: Ks depends on the child mask only
+      DevBuf mask;
+      mask.alloc(sizeof(int) * (ne > 0 ? ne : 1));
+      child_masks(l.mesh.elem_pidx.as<long long>(), ne, g, f.mesh.g, f.mesh.esolid.as<uint8_t>(),
+                  mask.as<int>(), s);
+      l.Ks64.alloc(sizeof(double) * 256 * 576);
+      GMGS_HIP(hipMemsetAsync(l.Ks64.as<double>(), 0, sizeof(double) * 256 * 576, s));
+      galerkin_mask_table(f.Ke64.as<double>(), exact, l.Ks64.as<double>(), s);
+      l.nKs = 256;
+      std::vector<int> m = download<int>(mask, ne, s);
+      val.resize(ne);
+      std::vector<int> uq;
+      for (long long k = 0; k < ne; ++k) {
+        if (l.state == 2 && m[k] == 0xFF) val[k] = 0;
+        else val[k] = 1 + m[k];
+        if (l.state == 2 && m[k] != 0xFF) uq.push_back((int)(k + 1));
+      }
+      l.nuniq = (long long)uq.size();
+      if (!uq.empty()) upload(l.uniq, uq.data(), uq.size(), s);
+    } else if (l.state == 2) {
+      // level 3 with deeper levels: B3 test, unique ones stored
+      DevBuf flag;
+      flag.alloc(sizeof(int) * (ne > 0 ? ne : 1));
+      unique_flags_l3(l.mesh.elem_pidx.as<long long>(), ne, g, f.mesh.g, f.eop.as<int>(), exact,
+                      flag.as<int>(), s);
+      std::vector<int> fl = download<int>(flag, ne, s);
+      std::vector<long long> epidx = download<long long>(l.mesh.elem_pidx, ne, s);
+      std::vector<long long> cells;
+      std::vector<int> uq;
+      val.resize(ne);
+      for (long long k = 0; k < ne; ++k) {
+        if (fl[k]) {
+          val[k] = 1 + (int)cells.size();
+          cells.push_back(epidx[k]);
+          uq.push_back((int)(k + 1));
+        } else {
+          val[k] = 0;
+        }
+      }
+      l.nKs = (long long)cells.size();
+      l.nuniq = l.nKs;
+      l.Ks64.alloc(sizeof(double) * 576 * (l.nKs > 0 ? l.nKs : 1));
+      if (l.nKs) {
+        DevBuf dcells;
+        upload(dcells, cells.data(), cells.size(), s);
+        galerkin_cells(dcells.as<long long>(), l.nKs, g, f.mesh.g, f.mesh.esolid.as<uint8_t>(),
+                       f.eop.as<int>(), f.Ke64.as<double>(),
+                       f.Ks64.empty() ? nullptr : f.Ks64.as<double>(), nullptr, exact,
+                       l.Ks64.as<double>(), s);
+        upload(l.uniq, uq.data(), uq.size(), s);
+        GMGS_HIP(hipStreamSynchronize(s));
+      }
+    } else {
+      // storing state 1: every element stored
+      l.nKs = ne;
+      l.Ks64.alloc(sizeof(double) * 576 * (ne > 0 ? ne : 1));
+      galerkin_cells(l.mesh.elem_pidx.as<long long>(), ne, g, f.mesh.g, f.mesh.esolid.as<uint8_t>(),
+                     li == 1 ? nullptr : f.eop.as<int>(), f.Ke64.as<double>(),
+                     f.Ks64.empty() ? nullptr : f.Ks64.as<double>(), emodf, exact,
+                     l.Ks64.as<double>(), s);
+      val.resize(ne);
+      for (long long k = 0; k < ne; ++k) val[k] = (int)(k + 1);
+    }
+    if (ne) {
+      upload(dval, val.data(), (size_t)ne, s);
+      scatter_eop(l.mesh.elem_pidx.as<long long>(), ne, dval.as<int>(), l.eop.as<int>(), s);
+    }
+    GMGS_HIP(hipStreamSynchronize(s));
+  }
+  // fast-path marks, inverse diagonals (levels 1..L-1), FP32 copies
+  for (int li = 0; li < L; ++li) {
+    Level& l = c->lev[li];
+    mark_fast(l.mesh.g, l.tiles.as<TileRec>(), l.ntiles, l.mesh.flags.as<uint32_t>(),
+              li == 0 ? nullptr : l.eop.as<int>(), !(li == 0 && c->simp), c->stream);
+    if (li < L - 1) {
+      l.invd64.alloc_zero(sizeof(double) * 3 * l.mesh.g.N, s);
+      inverse_diagonal(l.mesh.g, l.tiles.as<TileRec>(), l.ntiles, l.mesh.flags.as<uint32_t>(),
+                       li == 0 ? nullptr : l.eop.as<int>(), l.Ke64.as<double>(),
+                       l.Ks64.empty() ? nullptr : l.Ks64.as<double>(),
+                       (li == 0 && c->simp) ? c->emod64.as<double>() : nullptr, l.invd64.as<double>(), s);
+    }
+    if (c->mixed) {
+      if (!l.Ks64.empty()) to_f32(l.Ks64, l.Ks32, (long long)(l.Ks64.bytes() / sizeof(double)), s);
+      if (!l.invd64.empty()) to_f32(l.invd64, l.invd32, 3 * l.mesh.g.N, s);
+    }
+  }
+  GMGS_HIP(hipStreamSynchronize(s));
+}
+
+void alloc_vectors(gmgs_ctx* c) {
+  hipStream_t s = c->stream;
+  const size_t es = c->mixed ? sizeof(float) : sizeof(double);
+  for (int li = 0; li < c->L; ++li) {
+    Level& l = c->lev[li];
+    const long long n3 = 3 * l.mesh.g.N;
+    if (li == 0) {
+      l.rp.alloc_zero(es * n3, s);
+    } else {
+      l.r.alloc_zero(es * n3, s);
+      l.x.alloc_zero(es * n3, s);
+      if (li < c->L - 1) l.rp.alloc_zero(es * n3, s);
+    }
+  }
+  const long long n3 = 3 * L1(c).mesh.g.N;
+  c->x.alloc_zero(sizeof(double) * n3, s);
+  c->r.alloc_zero(sizeof(double) * n3, s);
+  c->ra.alloc_zero(sizeof(double) * n3, s);
+  c->p.alloc_zero(sizeof(double) * n3, s);
+  c->q.alloc_zero(sizeof(double) * n3, s);
+  c->tmp.alloc_zero(sizeof(double) * n3, s);
+  c->tmp2.alloc_zero(sizeof(double) * n3, s);
+  c->z[0].alloc_zero(es * n3, s);
+  c->z[1].alloc_zero(es * n3, s);
+  c->b.alloc_zero(sizeof(double) * n3, s);
+  c->nparts = apply_blocks(L1(c).ntiles);
+  c->part.alloc_zero(sizeof(double) * 2 * (c->nparts + 1), s);
+  c->sc.alloc_zero(sizeof(PcgScalars), s);
+  const long long hcap = (long long)c->prm.max_it + 2;
+  c->hist_rel.alloc_zero(sizeof(double) * hcap, s);
+  c->hist_t.alloc_zero(sizeof(unsigned long long) * hcap, s);
+  GMGS_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ V-cycle
+template <typename T>
+T* vec(DevBuf& b) { return b.as<T>(); }
+
+// z = V(rin) (kernel/Vcycle.m:1-36); rin is the level-1 FP64 residual
+template <typename T>
+void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
+  hipStream_t s = c->stream;
+  const int L = c->L;
+  const T omega = (T)c->prm.jacobi_weight;
+  // down: level 1
+  {
+    Level& l = c->lev[0];
+    LevelOp<T> op = make_op<T>(c, 0);
+    {
+      KTime t(c, 2);
+      launch_apply<T, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, rin, omega,
+                              vec<T>(l.rp), nullptr, nullptr, done, s);
+    }
+    Level& n2 = c->lev[1];
+    KTime t(c, 3);
+    launch_restrict<T>(n2.mesh.g, n2.mesh.flags.as<uint32_t>(), n2.tiles.as<TileRec>(), n2.ntiles,
+                       l.mesh.g, vec<T>(l.rp), vec<T>(n2.r), done, s);
+  }
+  for (int li = 1; li < L - 1; ++li) {
+    Level& l = c->lev[li];
+    LevelOp<T> op = make_op<T>(c, li);
+    launch_apply<T, T>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, vec<T>(l.r), omega,
+                       vec<T>(l.rp), nullptr, nullptr, done, s);
+    Level& nx = c->lev[li + 1];
+    launch_restrict<T>(nx.mesh.g, nx.mesh.flags.as<uint32_t>(), nx.tiles.as<TileRec>(), nx.ntiles,
+                       l.mesh.g, vec<T>(l.rp), vec<T>(nx.r), done, s);
+  }
+  // coarsest
+  {
+    Level& lc = c->lev[L - 1];
+    coarsest_apply<T>(c->Kinv.as<double>(), c->nfp, c->nfree, c->free_code.as<long long>(),
+                      vec<T>(lc.r), vec<T>(lc.x), lc.mesh.g.N, c->gv_b.as<double>(),
+                      c->gv_part.as<double>(), done, s);
+  }
+  // up
+  for (int li = L - 2; li >= 1; --li) {
+    Level& l = c->lev[li];
+    Level& cr = c->lev[li + 1];
+    const T* invd = make_op<T>(c, li).invd;
+    launch_prolong_smooth<T, T, T>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(),
+                                   l.ntiles, cr.mesh.g, vec<T>(cr.x), invd, vec<T>(l.r),
+                                   vec<T>(l.rp), omega, vec<T>(l.x), false, (const T*)nullptr,
+                                   nullptr, done, s);
+  }
+  {
+    Level& l = c->lev[0];
+    Level& cr = c->lev[1];
+    const T* invd = make_op<T>(c, 0).invd;
+    KTime t(c, 4);
+    launch_prolong_smooth<T, double, T>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(),
+                                        l.ntiles, cr.mesh.g, vec<T>(cr.x), invd, rin, vec<T>(l.rp),
+                                        omega, z, true, zprev, partial, done, s);
+  }
+}
+
+// ------------------------------------------------------------------ PCG
+template <typename T>
+void pcg_iteration(gmgs_ctx* c, int k) {
+  hipStream_t s = c->stream;
+  Level& l = L1(c);
+  const Grid& g = l.mesh.g;
+  const uint32_t* fl = l.mesh.flags.as<uint32_t>();
+  const TileRec* tl = l.tiles.as<TileRec>();
+  PcgScalars* sc = c->sc.as<PcgScalars>();
+  const int* done = &sc->done;
+  double* part = c->part.as<double>();
+  const int np = (int)c->nparts;
+  LevelOp<double> op = make_op<double>(c, 0);
+  c->cur_iter = k;
+  KTime whole(c, 5);
+  // Jacobi step on (x, r) (CG_solver.m:18-19)
+  {
+    KTime t(c, 1);
+    launch_apply<double, double>(op, tl, l.ntiles, AP_SMOOTH, c->r.as<double>(), c->prm.jacobi_weight,
+                                 c->ra.as<double>(), c->x.as<double>(), nullptr, done, s);
+  }
+  // z = V(r) with r'z and r'z_prev partials (CG_solver.m:21)
+  T* z = c->z[k & 1].as<T>();
+  const T* zprev = c->flexible ? c->z[(k + 1) & 1].as<T>() : nullptr;
+  vcycle<T>(c, c->ra.as<double>(), z, zprev, part, done);
+  launch_finalize(1, part, np, sc, nullptr, nullptr, done, s);
+  // p = z + beta p (CG_solver.m:23-28)
+  launch_update_p<T>(g, fl, tl, l.ntiles, z, c->p.as<double>(), sc, done, s);
+  // q = A p, p'q (CG_solver.m:29-31)
+  {
+    KTime t(c, 0);
+    launch_apply<double, double>(op, tl, l.ntiles, AP_AX, c->p.as<double>(), 0.0, c->q.as<double>(),
+                                 nullptr, part, done, s);
+  }
+  launch_finalize(2, part, np, sc, nullptr, nullptr, done, s);
+  // x += alpha p; r = r - alpha q; ||r|| (CG_solver.m:32-39)
+  launch_update_xr(g, fl, tl, l.ntiles, c->x.as<double>(), c->p.as<double>(), c->ra.as<double>(),
+                   c->q.as<double>(), c->r.as<double>(), sc, part, done, s);
+  launch_finalize(3, part, np, sc, c->hist_rel.as<double>(), c->hist_t.as<unsigned long long>(),
+                  done, s);
+}
+
+void pcg_init(gmgs_ctx* c, double tol) {
+  hipStream_t s = c->stream;
+  Level& l = L1(c);
+  const long long n3 = 3 * l.mesh.g.N;
+  launch_fill<double>(c->x.as<double>(), 0.0, n3, s);
+  launch_fill<double>(c->p.as<double>(), 0.0, n3, s);
+  if (c->mixed) {
+    launch_fill<float>(c->z[0].as<float>(), 0.0f, n3, s);
+    launch_fill<float>(c->z[1].as<float>(), 0.0f, n3, s);
+  } else {
+    launch_fill<double>(c->z[0].as<double>(), 0.0, n3, s);
+    launch_fill<double>(c->z[1].as<double>(), 0.0, n3, s);
+  }
+  // r1 = b - AtX(0) = b (CG_solver.m:12)
+  GMGS_HIP(hipMemcpyAsync(c->r.as<double>(), c->b.as<double>(), sizeof(double) * n3,
+                          hipMemcpyDeviceToDevice, s));
+  PcgScalars h{};
+  h.tol = tol;
+  h.max_it = c->prm.max_it;
+  h.flexible = c->flexible ? 1 : 0;
+  GMGS_HIP(hipMemcpyAsync(c->sc.as<void>(), &h, sizeof(h), hipMemcpyHostToDevice, s));
+  launch_dot(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->b.as<double>(),
+             c->b.as<double>(), c->part.as<double>(), s);
+  launch_finalize(0, c->part.as<double>(), (int)c->nparts, c->sc.as<PcgScalars>(), nullptr, nullptr,
+                  nullptr, s);
+}
+
+struct LoopResult {
+  int its = 0, converged = 0;
+  double relres = 0, seconds = 0;
+};
+
+LoopResult pcg_run(gmgs_ctx* c) {
+  hipStream_t s = c->stream;
+  PcgScalars h{};
+  hipEvent_t e0, e1;
+  GMGS_HIP(hipEventCreate(&e0));
+  GMGS_HIP(hipEventCreate(&e1));
+  GMGS_HIP(hipEventRecord(e0, s));
+  const int maxk = c->prm.max_it + 1;
+  int k = 0;
+  int chunk = 1;
+  while (k < maxk) {
+    const int n = std::min(chunk, maxk - k);
+    for (int i = 0; i < n; ++i, ++k) {
+      if (c->mixed) pcg_iteration<float>(c, k);
+      else pcg_iteration<double>(c, k);
+    }
+    GMGS_HIP(hipMemcpyAsync(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost, s));
+    GMGS_HIP(hipStreamSynchronize(s));
+    if (h.done) break;
+    chunk = std::min(chunk * 2, 8);
+  }
+  GMGS_HIP(hipEventRecord(e1, s));
+  GMGS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  GMGS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  GMGS_HIP(hipEventDestroy(e0));
+  GMGS_HIP(hipEventDestroy(e1));
+  LoopResult r;
+  r.its = h.its;
+  r.converged = h.converged;
+  r.relres = h.relres;
+  r.seconds = ms * 1e-3;
+  if (!(h.relres == h.relres)) throw Status(GMGS_ERR_BREAKDOWN, "NaN in the PCG recursion");
+  return r;
+}
+
+double dot_host(gmgs_ctx* c, const double* a, const double* b) {
+  Level& l = L1(c);
+  launch_dot(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, a, b,
+             c->part.as<double>(), c->stream);
+  std::vector<double> h = download<double>(c->part, c->nparts, c->stream);
+  double sum = 0.0;
+  for (double v : h) sum += v;
+  return sum;
+}
+
+void set_bc_impl(gmgs_ctx* c, const int32_t* fixed, int64_t k, const double* loads, int64_t m) {
+  hipStream_t s = c->stream;
+  Level& l = L1(c);
+  const long long nn = l.mesh.nn;
+  if (m <= 0 || !loads) throw Status(GMGS_ERR_NO_LOADS, "No Loads! Unable to Perform Static FEA!");
+  if (k <= 0 || !fixed) throw Status(GMGS_ERR_NO_CONSTRAINTS, "No Constraints! Unable to Perform Static FEA!");
+  // F_(loadingCond_(:,1),:) = loadingCond_(:,2:4): the last duplicate row wins
+  c->F.assign(3 * nn, 0.0);
+  for (int64_t i = 0; i < m; ++i) {
+    const double id = loads[i];
+    if (!(id >= 1 && id <= (double)nn && std::floor(id) == id))
+      throw Status(GMGS_ERR_NODE_ID, "load row " + std::to_string(i + 1) + ": node id out of range");
+    for (int cc = 0; cc < 3; ++cc) {
+      const double v = loads[i + m * (1 + cc)];
+      if (!std::isfinite(v)) throw Status(GMGS_ERR_NONFINITE, "non-finite load value");
+      c->F[3 * ((long long)id - 1) + cc] = v;
+    }
+  }
+  for (int64_t i = 0; i < k; ++i)
+    if (fixed[i] < 1 || fixed[i] > nn)
+      throw Status(GMGS_ERR_NODE_ID, "fixed node " + std::to_string(i + 1) + ": id out of range");
+  DevBuf dF;
+  upload(dF, c->F.data(), c->F.size(), s);
+  const long long n3 = 3 * l.mesh.g.N;
+  launch_fill<double>(c->b.as<double>(), 0.0, n3, s);
+  launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dF.as<double>(), c->b.as<double>(),
+                         l.mesh.g.N, s);
+  // fixed flags on every level (ApplyBoundaryCondition.m:14-20, SetupSolver.m:246-254)
+  DevBuf dfix;
+  upload(dfix, fixed, (size_t)k, s);
+  for (int li = 0; li < c->L; ++li)
+    clear_fixed(c->lev[li].mesh.g, c->lev[li].mesh.flags.as<uint32_t>(), s);
+  set_fixed_nodes(l.mesh.node_pidx.as<long long>(), dfix.as<int>(), k, l.mesh.flags.as<uint32_t>(), s);
+  for (int li = 1; li < c->L; ++li) {
+    Level& cl = c->lev[li];
+    Level& fl = c->lev[li - 1];
+    restrict_fixed(cl.mesh.g, cl.tiles.as<TileRec>(), cl.ntiles, cl.mesh.flags.as<uint32_t>(),
+                   fl.mesh.g, fl.mesh.flags.as<uint32_t>(), s);
+  }
+  GMGS_HIP(hipStreamSynchronize(s));
+  // coarsest: assemble K_ff, factor, invert (SetupSolver.m:257-259)
+  Level& lc = c->lev[c->L - 1];
+  c->nfree = coarsest_free_dofs(lc.mesh.g, lc.mesh.node_pidx.as<long long>(), lc.mesh.nn,
+                                lc.mesh.flags.as<uint32_t>(), c->free_code, c->dofmap, s);
+  c->nfp = std::max<long long>(DB, (c->nfree + DB - 1) / DB * DB);
+  c->A.alloc(sizeof(double) * c->nfp * c->nfp);
+  assemble_coarsest(lc.mesh.g, lc.mesh.flags.as<uint32_t>(), lc.eop.as<int>(), lc.Ke64.as<double>(),
+                    lc.Ks64.empty() ? nullptr : lc.Ks64.as<double>(), c->free_code.as<long long>(),
+                    c->dofmap.as<int>(), c->nfree, c->nfp, c->A.as<double>(), s);
+  const int info = cholesky(c->A.as<double>(), c->nfp, s);
+  if (info != 0)
+    throw Status(GMGS_ERR_NOT_SPD, "coarsest-level stiffness matrix is not positive definite (pivot "
+                                       + std::to_string(info) + ")");
+  c->W.alloc(sizeof(double) * c->nfp * c->nfp);
+  c->Kinv.alloc(sizeof(double) * c->nfp * c->nfp);
+  inverse_from_cholesky(c->A.as<double>(), c->W.as<double>(), c->Kinv.as<double>(), c->nfp, s);
+  c->W.release();
+  c->gv_b.alloc_zero(sizeof(double) * c->nfp, s);
+  const long long nch = (c->nfp + 255) / 256;
+  c->gv_part.alloc_zero(sizeof(double) * nch * c->nfp, s);
+  GMGS_HIP(hipStreamSynchronize(s));
+  c->bc = true;
+  c->have_solution = false;
+  if (c->bc_fixed.data() != fixed) {
+    c->bc_fixed.assign(fixed, fixed + k);
+    c->bc_loads.assign(loads, loads + 4 * m);
+    c->bc_m = m;
+  }
+}
+
+thread_local std::string g_create_error;
+
+int fail(gmgs_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(gmgs_ctx* c, F&& f) {
+  try {
+    return f();
+  } catch (const Status& e) {
+    return fail(c, e.code, e.what());
+  } catch (const HipError& e) {
+    return fail(c, GMGS_ERR_DEVICE, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(c, GMGS_ERR_DEVICE, "host out of memory");
+  } catch (const std::exception& e) {
+    return fail(c, GMGS_ERR_DEVICE, e.what());
+  }
+}
+
+long long level_dofs(gmgs_ctx* c, int li) { return 3 * c->lev[li].mesh.nn; }
+
+void check_level(gmgs_ctx* c, int level) {
+  if (level < 1 || level > c->L) throw Status(GMGS_ERR_ARGUMENT, "level out of range");
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+void gmgs_default_params(gmgs_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->youngs_modulus = 1.0;                 // GlobalVariables.m:8
+  p->poisson_ratio = 0.3;                  // GlobalVariables.m:9
+  p->tol = 1.0e-3;                         // GlobalVariables.m:3
+  p->jacobi_weight = 0.4;                  // GlobalVariables.m:5
+  p->coarsest_resolution_control = 1.0e4;  // GlobalVariables.m:23
+  p->max_it = 20000;                       // GlobalVariables.m:4
+  p->num_levels = 0;
+  p->precision = GMGS_PREC_FP64;
+  p->flags = 0;
+  p->device = 0;
+}
+
+const char* gmgs_status_string(int st) {
+  switch (st) {
+    case GMGS_OK: return "ok";
+    case GMGS_ERR_ARGUMENT: return "invalid argument";
+    case GMGS_ERR_RESOLUTION: return "insufficient resolution";
+    case GMGS_ERR_LEVELS: return "fewer than two levels";
+    case GMGS_ERR_NO_LOADS: return "no loads";
+    case GMGS_ERR_NO_CONSTRAINTS: return "no constraints";
+    case GMGS_ERR_NODE_ID: return "node id out of range";
+    case GMGS_ERR_NONFINITE: return "non-finite input";
+    case GMGS_ERR_NOT_SPD: return "coarsest matrix not positive definite";
+    case GMGS_ERR_NOT_CONVERGED: return "maximum iterations exceeded";
+    case GMGS_ERR_STATE: return "invalid call order";
+    case GMGS_ERR_DEVICE: return "device error";
+    case GMGS_ERR_BREAKDOWN: return "PCG breakdown";
+    default: return "unknown status";
+  }
+}
+
+int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx, int64_t nz,
+                const double* bbox, gmgs_ctx** out) {
+  if (!out) return GMGS_ERR_ARGUMENT;
+  *out = nullptr;
+  if (!p || !vox || ny <= 0 || nx <= 0 || nz <= 0) return GMGS_ERR_ARGUMENT;
+  auto c = std::make_unique<gmgs_ctx>();
+  int st = guarded(c.get(), [&]() -> int {
+    c->prm = *p;
+    if (!(p->poisson_ratio > -1.0 && p->poisson_ratio < 0.5) || !(p->youngs_modulus > 0) ||
+        !(p->tol > 0) || p->max_it < 0 || !(p->jacobi_weight > 0) || p->num_levels < 0 ||
+        (p->precision != GMGS_PREC_FP64 && p->precision != GMGS_PREC_MIXED))
+      throw Status(GMGS_ERR_ARGUMENT, "invalid solver parameter");
+    c->device = p->device;
+    GMGS_HIP(hipSetDevice(c->device));
+    GMGS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess && khz > 0)
+      c->clock_khz = khz;
+    c->mixed = p->precision == GMGS_PREC_MIXED;
+    c->flexible = c->mixed ? !(p->flags & GMGS_FLAG_FR_BETA) : (p->flags & GMGS_FLAG_PR_BETA) != 0;
+    c->exact = (p->flags & GMGS_FLAG_EXACT_GALERKIN) != 0;
+    c->tables = (p->flags & GMGS_FLAG_NO_INDEX_TABLES) == 0;
+    c->nely = (int)ny;
+    c->nelx = (int)nx;
+    c->nelz = (int)nz;
+    if (bbox) {
+      for (int i = 0; i < 6; ++i) {
+        if (!std::isfinite(bbox[i])) throw Status(GMGS_ERR_NONFINITE, "non-finite bounding box");
+        c->bbox[i] = bbox[i];
+      }
+    } else {
+      c->bbox[3] = (double)nx; c->bbox[4] = (double)ny; c->bbox[5] = (double)nz;
+    }
+    for (int d = 0; d < 3; ++d)
+      if (!(c->bbox[3 + d] > c->bbox[d])) throw Status(GMGS_ERR_ARGUMENT, "empty bounding box");
+    build_mesh(c.get(), vox);
+    double probe[8][3];
+    probe_coords(c.get(), probe);
+    element_stiffness(p->youngs_modulus, p->poisson_ratio, probe, c->Ke0);
+    if (!stress_operator(p->youngs_modulus, p->poisson_ratio, probe, c->M))
+      throw Status(GMGS_ERR_ARGUMENT, "singular Gauss-point interpolation matrix");
+    double W[8][8][8];
+    child_weights(W);
+    set_galerkin_weights(W, c->stream);
+    for (int li = 0; li < c->L; ++li) upload_level_constants(c.get(), li);
+    upload(c->stressM, c->M, 48 * 24, c->stream);
+    build_operators(c.get());
+    alloc_vectors(c.get());
+    return GMGS_OK;
+  });
+  if (st != GMGS_OK) {
+    g_create_error = c->err;      // reachable through gmgs_last_error(NULL)
+    gmgs_destroy(c.release());
+    return st;
+  }
+  *out = c.release();
+  return GMGS_OK;
+}
+
+int gmgs_set_bc(gmgs_ctx* c, const int32_t* fixed, int64_t k, const double* loads, int64_t m) {
+  if (!c || c->lev.empty()) return GMGS_ERR_STATE;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    set_bc_impl(c, fixed, k, loads, m);
+    return GMGS_OK;
+  });
+}
+
+int gmgs_set_modulus(gmgs_ctx* c, const double* modulus, int64_t n) {
+  if (!c || c->lev.empty()) return GMGS_ERR_STATE;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    Level& l = L1(c);
+    hipStream_t s = c->stream;
+    if (modulus == nullptr || n == 0) {
+      c->simp = false;
+    } else {
+      if (n != l.mesh.ne) throw Status(GMGS_ERR_ARGUMENT, "modulus length must equal numElements");
+      for (int64_t i = 0; i < n; ++i)
+        if (!(modulus[i] > 0) || !std::isfinite(modulus[i]))
+          throw Status(GMGS_ERR_NONFINITE, "modulus values must be finite and positive");
+      DevBuf dm;
+      upload(dm, modulus, (size_t)n, s);
+      c->emod64.alloc_zero(sizeof(double) * l.mesh.g.N, s);
+      // scatter compact element values to their cells
+      std::vector<long long> ep = download<long long>(l.mesh.elem_pidx, n, s);
+      std::vector<double> dense((size_t)l.mesh.g.N, 0.0);
+      for (int64_t i = 0; i < n; ++i) dense[ep[i]] = modulus[i];
+      upload(c->emod64, dense.data(), dense.size(), s);
+      if (c->mixed) to_f32(c->emod64, c->emod32, l.mesh.g.N, s);
+      c->simp = true;
+    }
+    build_operators(c);
+    if (c->bc) {
+      // the coarse operators changed: redo the BC-dependent set-up
+      std::vector<int32_t> fx = c->bc_fixed;
+      std::vector<double> ld = c->bc_loads;
+      set_bc_impl(c, fx.data(), (int64_t)fx.size(), ld.data(), c->bc_m);
+      c->bc_fixed = fx;
+      c->bc_loads = ld;
+    }
+    GMGS_HIP(hipStreamSynchronize(s));
+    return GMGS_OK;
+  });
+}
+
+static int solve_into(gmgs_ctx* c, gmgs_solve_info* info, double* hist, int64_t hist_cap) {
+  hipStream_t s = c->stream;
+  Level& l = L1(c);
+  pcg_init(c, c->prm.tol);
+  LoopResult r = pcg_run(c);
+  if (c->ktime) kt_collect(c, r.its);
+  // true FP64 residual at exit: tmp = b - A x
+  LevelOp<double> op = make_op<double>(c, 0);
+  launch_apply<double, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_AX, c->x.as<double>(), 0.0,
+                               c->tmp.as<double>(), nullptr, nullptr, nullptr, s);
+  launch_sub(c->b.as<double>(), c->tmp.as<double>(), c->tmp.as<double>(), 3 * l.mesh.g.N, s);
+  const double bb = dot_host(c, c->b.as<double>(), c->b.as<double>());
+  const double rr = dot_host(c, c->tmp.as<double>(), c->tmp.as<double>());
+  const double tr = std::sqrt(rr) / std::sqrt(bb);
+  const long long nh = r.converged ? r.its - 1 : r.its;
+  if (info) {
+    info->its = r.its;
+    info->converged = r.converged;
+    info->relres = r.relres;
+    info->true_relres = tr;
+    info->solve_seconds = r.seconds;
+    info->hist_len = nh;
+  }
+  if (hist && hist_cap > 0 && nh > 0) {
+    std::vector<double> hr = download<double>(c->hist_rel, (size_t)nh, s);
+    std::vector<unsigned long long> ht = download<unsigned long long>(c->hist_t, (size_t)nh, s);
+    PcgScalars h{};
+    GMGS_HIP(hipMemcpyAsync(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost, s));
+    GMGS_HIP(hipStreamSynchronize(s));
+    for (long long i = 0; i < nh && i < hist_cap; ++i) {
+      hist[3 * i + 0] = (double)(i + 1);
+      hist[3 * i + 1] = (double)(ht[i] - h.t0) / (c->clock_khz * 1e3);
+      hist[3 * i + 2] = hr[i];
+    }
+  }
+  c->have_solution = true;
+  return r.converged ? GMGS_OK : GMGS_ERR_NOT_CONVERGED;
+}
+
+int gmgs_solve(gmgs_ctx* c, double* U, gmgs_solve_info* info, double* hist, int64_t hist_cap) {
+  if (!c || c->lev.empty()) return GMGS_ERR_STATE;
+  if (!c->bc) return fail(c, GMGS_ERR_STATE, "boundary conditions not set");
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    const int st = solve_into(c, info, hist, hist_cap);
+    if (U) {
+      Level& l = L1(c);
+      DevBuf comp;
+      comp.alloc(sizeof(double) * 3 * l.mesh.nn);
+      launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), l.mesh.nn, c->x.as<double>(),
+                                    l.mesh.g.N, comp.as<double>(), c->stream);
+      GMGS_HIP(hipMemcpyAsync(U, comp.as<void>(), sizeof(double) * 3 * l.mesh.nn,
+                              hipMemcpyDeviceToHost, c->stream));
+      GMGS_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (st == GMGS_ERR_NOT_CONVERGED) c->err = "Exceed the maximum iterate numbers";
+    return st;
+  });
+}
+
+int gmgs_stress(gmgs_ctx* c, const double* U, double* S) {
+  if (!c || c->lev.empty() || !S) return GMGS_ERR_ARGUMENT;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    Level& l = L1(c);
+    const double* u = c->x.as<double>();
+    if (U) {
+      DevBuf dU;
+      upload(dU, U, (size_t)(3 * l.mesh.nn), s);
+      launch_fill<double>(c->tmp.as<double>(), 0.0, 3 * l.mesh.g.N, s);
+      launch_scatter_compact(l.mesh.node_pidx.as<long long>(), l.mesh.nn, dU.as<double>(),
+                             c->tmp.as<double>(), l.mesh.g.N, s);
+      u = c->tmp.as<double>();
+      GMGS_HIP(hipStreamSynchronize(s));
+    } else if (!c->have_solution) {
+      throw Status(GMGS_ERR_STATE, "No Deformation Field Available, Please Compute Deformation First");
+    }
+    DevBuf dS;
+    dS.alloc(sizeof(double) * 6 * l.mesh.nn);
+    launch_stress(l.mesh.g, l.mesh.node_pidx.as<long long>(), l.mesh.nn, l.mesh.flags.as<uint32_t>(),
+                  c->stressM.as<double>(), c->simp ? c->emod64.as<double>() : nullptr, u,
+                  dS.as<double>(), s);
+    GMGS_HIP(hipMemcpyAsync(S, dS.as<void>(), sizeof(double) * 6 * l.mesh.nn, hipMemcpyDeviceToHost, s));
+    GMGS_HIP(hipStreamSynchronize(s));
+    return GMGS_OK;
+  });
+}
+
+int gmgs_adjoint(gmgs_ctx* c, const double* g, const double* U, double* lambda, double* dJdE,
+                 gmgs_solve_info* info) {
+  if (!c || c->lev.empty() || !g) return GMGS_ERR_ARGUMENT;
+  if (!c->bc) return fail(c, GMGS_ERR_STATE, "boundary conditions not set");
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    Level& l = L1(c);
+    const long long nn = l.mesh.nn, N = l.mesh.g.N;
+    // forward displacement (dense) into tmp2
+    if (U) {
+      DevBuf dU;
+      upload(dU, U, (size_t)(3 * nn), s);
+      launch_fill<double>(c->tmp2.as<double>(), 0.0, 3 * N, s);
+      launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dU.as<double>(),
+                             c->tmp2.as<double>(), N, s);
+    } else {
+      if (!c->have_solution) throw Status(GMGS_ERR_STATE, "no forward solution");
+      GMGS_HIP(hipMemcpyAsync(c->tmp2.as<double>(), c->x.as<double>(), sizeof(double) * 3 * N,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    // adjoint solve K lambda = g (fixed DOFs of g ignored): reuse the MG-PCG with b := g
+    DevBuf dg, saved;
+    saved.alloc(sizeof(double) * 3 * N);
+    GMGS_HIP(hipMemcpyAsync(saved.as<double>(), c->b.as<double>(), sizeof(double) * 3 * N,
+                            hipMemcpyDeviceToDevice, s));
+    // zero g on fixed DOFs so the adjoint system is the free-DOF one
+    std::vector<double> gz(g, g + 3 * nn);
+    {
+      std::vector<uint32_t> fl = download<uint32_t>(l.mesh.flags, (size_t)N, s);
+      std::vector<long long> np = download<long long>(l.mesh.node_pidx, (size_t)nn, s);
+      for (long long k = 0; k < nn; ++k)
+        for (int cc = 0; cc < 3; ++cc)
+          if ((fl[np[k]] >> (F_FIX_SHIFT + cc)) & 1u) gz[3 * k + cc] = 0.0;
+    }
+    upload(dg, gz.data(), gz.size(), s);
+    launch_fill<double>(c->b.as<double>(), 0.0, 3 * N, s);
+    launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dg.as<double>(), c->b.as<double>(), N, s);
+    const int st = solve_into(c, info, nullptr, 0);
+    // restore the load vector; the adjoint solution is in x
+    GMGS_HIP(hipMemcpyAsync(c->b.as<double>(), saved.as<double>(), sizeof(double) * 3 * N,
+                            hipMemcpyDeviceToDevice, s));
+    c->have_solution = false;
+    if (lambda) {
+      DevBuf comp;
+      comp.alloc(sizeof(double) * 3 * nn);
+      launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, c->x.as<double>(), N,
+                                    comp.as<double>(), s);
+      GMGS_HIP(hipMemcpyAsync(lambda, comp.as<void>(), sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+    }
+    if (dJdE) {
+      DevBuf dd;
+      dd.alloc(sizeof(double) * l.mesh.ne);
+      launch_element_energy(l.mesh.g, l.mesh.elem_pidx.as<long long>(), l.mesh.ne, L1(c).Ke64.as<double>(),
+                            c->x.as<double>(), c->tmp2.as<double>(), dd.as<double>(), s);
+      GMGS_HIP(hipMemcpyAsync(dJdE, dd.as<void>(), sizeof(double) * l.mesh.ne, hipMemcpyDeviceToHost, s));
+    }
+    GMGS_HIP(hipStreamSynchronize(s));
+    return st;
+  });
+}
+
+// ---------------------------------------------------------------- operator handles
+int gmgs_atx(gmgs_ctx* c, int level, const double* x, double* y) {
+  if (!c || c->lev.empty() || !x || !y) return GMGS_ERR_ARGUMENT;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    check_level(c, level);
+    if (!c->bc) throw Status(GMGS_ERR_STATE, "boundary conditions not set (fixed DOFs unknown)");
+    hipStream_t s = c->stream;
+    const int li = level - 1;
+    Level& l = c->lev[li];
+    const long long nn = l.mesh.nn, N = l.mesh.g.N;
+    DevBuf dx, din, dout, comp;
+    upload(dx, x, (size_t)(3 * nn), s);
+    din.alloc_zero(sizeof(double) * 3 * N, s);
+    dout.alloc_zero(sizeof(double) * 3 * N, s);
+    launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dx.as<double>(), din.as<double>(), N, s);
+    LevelOp<double> op = make_op<double>(c, li);
+    launch_apply<double, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_AX, din.as<double>(), 0.0,
+                                 dout.as<double>(), nullptr, nullptr, nullptr, s);
+    comp.alloc(sizeof(double) * 3 * nn);
+    launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, dout.as<double>(), N,
+                                  comp.as<double>(), s);
+    GMGS_HIP(hipMemcpyAsync(y, comp.as<void>(), sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+    GMGS_HIP(hipStreamSynchronize(s));
+    return GMGS_OK;
+  });
+}
+
+int gmgs_smooth(gmgs_ctx* c, int level, const double* r, double* dx, double* r_out) {
+  if (!c || c->lev.empty() || !r) return GMGS_ERR_ARGUMENT;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    check_level(c, level);
+    if (level == c->L) throw Status(GMGS_ERR_ARGUMENT, "no smoother on the coarsest level");
+    if (!c->bc) throw Status(GMGS_ERR_STATE, "boundary conditions not set");
+    hipStream_t s = c->stream;
+    const int li = level - 1;
+    Level& l = c->lev[li];
+    const long long nn = l.mesh.nn, N = l.mesh.g.N;
+    DevBuf dr, din, dout, dacc, comp;
+    upload(dr, r, (size_t)(3 * nn), s);
+    din.alloc_zero(sizeof(double) * 3 * N, s);
+    dout.alloc_zero(sizeof(double) * 3 * N, s);
+    dacc.alloc_zero(sizeof(double) * 3 * N, s);
+    launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dr.as<double>(), din.as<double>(), N, s);
+    LevelOp<double> op = make_op<double>(c, li);
+    launch_apply<double, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, din.as<double>(),
+                                 c->prm.jacobi_weight, dout.as<double>(), dacc.as<double>(), nullptr,
+                                 nullptr, s);
+    comp.alloc(sizeof(double) * 3 * nn);
+    if (dx) {
+      launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, dacc.as<double>(), N,
+                                    comp.as<double>(), s);
+      GMGS_HIP(hipMemcpyAsync(dx, comp.as<void>(), sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+      GMGS_HIP(hipStreamSynchronize(s));
+    }
+    if (r_out) {
+      launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, dout.as<double>(), N,
+                                    comp.as<double>(), s);
+      GMGS_HIP(hipMemcpyAsync(r_out, comp.as<void>(), sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+    }
+    GMGS_HIP(hipStreamSynchronize(s));
+    return GMGS_OK;
+  });
+}
+
+int gmgs_restrict(gmgs_ctx* c, int level, const double* rf, double* rc) {
+  if (!c || c->lev.empty() || !rf || !rc) return GMGS_ERR_ARGUMENT;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    check_level(c, level);
+    if (level < 2) throw Status(GMGS_ERR_ARGUMENT, "restriction targets levels >= 2");
+    hipStream_t s = c->stream;
+    Level& f = c->lev[level - 2];
+    Level& l = c->lev[level - 1];
+    DevBuf dr, din, dout, comp;
+    upload(dr, rf, (size_t)(3 * f.mesh.nn), s);
+    din.alloc_zero(sizeof(double) * 3 * f.mesh.g.N, s);
+    dout.alloc_zero(sizeof(double) * 3 * l.mesh.g.N, s);
+    launch_scatter_compact(f.mesh.node_pidx.as<long long>(), f.mesh.nn, dr.as<double>(), din.as<double>(),
+                           f.mesh.g.N, s);
+    launch_restrict<double>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles,
+                            f.mesh.g, din.as<double>(), dout.as<double>(), nullptr, s);
+    comp.alloc(sizeof(double) * 3 * l.mesh.nn);
+    launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), l.mesh.nn, dout.as<double>(),
+                                  l.mesh.g.N, comp.as<double>(), s);
+    GMGS_HIP(hipMemcpyAsync(rc, comp.as<void>(), sizeof(double) * 3 * l.mesh.nn, hipMemcpyDeviceToHost, s));
+    GMGS_HIP(hipStreamSynchronize(s));
+    return GMGS_OK;
+  });
+}
+
+int gmgs_interpolate(gmgs_ctx* c, int level, const double* xc, double* xf) {
+  if (!c || c->lev.empty() || !xc || !xf) return GMGS_ERR_ARGUMENT;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    check_level(c, level);
+    if (level < 2) throw Status(GMGS_ERR_ARGUMENT, "interpolation starts at levels >= 2");
+    hipStream_t s = c->stream;
+    Level& f = c->lev[level - 2];
+    Level& l = c->lev[level - 1];
+    DevBuf dx, din, dout, zero, comp;
+    upload(dx, xc, (size_t)(3 * l.mesh.nn), s);
+    din.alloc_zero(sizeof(double) * 3 * l.mesh.g.N, s);
+    dout.alloc_zero(sizeof(double) * 3 * f.mesh.g.N, s);
+    zero.alloc_zero(sizeof(double) * 3 * f.mesh.g.N, s);
+    launch_scatter_compact(l.mesh.node_pidx.as<long long>(), l.mesh.nn, dx.as<double>(), din.as<double>(),
+                           l.mesh.g.N, s);
+    launch_prolong_smooth<double, double, double>(
+        f.mesh.g, f.mesh.flags.as<uint32_t>(), f.tiles.as<TileRec>(), f.ntiles, l.mesh.g,
+        din.as<double>(), zero.as<double>(), zero.as<double>(), zero.as<double>(), 0.0,
+        dout.as<double>(), false, (const double*)nullptr, nullptr, nullptr, s);
+    comp.alloc(sizeof(double) * 3 * f.mesh.nn);
+    launch_gather_compact<double>(f.mesh.node_pidx.as<long long>(), f.mesh.nn, dout.as<double>(),
+                                  f.mesh.g.N, comp.as<double>(), s);
+    GMGS_HIP(hipMemcpyAsync(xf, comp.as<void>(), sizeof(double) * 3 * f.mesh.nn, hipMemcpyDeviceToHost, s));
+    GMGS_HIP(hipStreamSynchronize(s));
+    return GMGS_OK;
+  });
+}
+
+int gmgs_vcycle(gmgs_ctx* c, const double* r, double* z) {
+  if (!c || c->lev.empty() || !r || !z) return GMGS_ERR_ARGUMENT;
+  if (!c->bc) return fail(c, GMGS_ERR_STATE, "boundary conditions not set");
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    Level& l = L1(c);
+    const long long nn = l.mesh.nn, N = l.mesh.g.N;
+    DevBuf dr, comp;
+    upload(dr, r, (size_t)(3 * nn), s);
+    launch_fill<double>(c->tmp.as<double>(), 0.0, 3 * N, s);
+    launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dr.as<double>(), c->tmp.as<double>(), N, s);
+    comp.alloc(sizeof(double) * 3 * nn);
+    if (c->mixed) {
+      vcycle<float>(c, c->tmp.as<double>(), c->z[0].as<float>(), nullptr, nullptr, nullptr);
+      launch_gather_compact<float>(l.mesh.node_pidx.as<long long>(), nn, c->z[0].as<float>(), N,
+                                   comp.as<double>(), s);
+    } else {
+      vcycle<double>(c, c->tmp.as<double>(), c->z[0].as<double>(), nullptr, nullptr, nullptr);
+      launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, c->z[0].as<double>(), N,
+                                    comp.as<double>(), s);
+    }
+    GMGS_HIP(hipMemcpyAsync(z, comp.as<void>(), sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+    GMGS_HIP(hipStreamSynchronize(s));
+    return GMGS_OK;
+  });
+}
+
+// ---------------------------------------------------------------- queries
+int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count) {
+  if (!c || c->lev.empty() || !count) return GMGS_ERR_ARGUMENT;
+  return guarded(c, [&]() -> int {
+    if (key != GMGS_Q_NUM_LEVELS && key != GMGS_Q_KE0 && key != GMGS_Q_BBOX) check_level(c, level);
+    const Level* l = (level >= 1 && level <= c->L) ? &c->lev[level - 1] : nullptr;
+    const Grid* g = l ? &l->mesh.g : nullptr;
+    int64_t n = 0;
+    switch (key) {
+      case GMGS_Q_NUM_LEVELS: case GMGS_Q_NUM_ELEMENTS: case GMGS_Q_NUM_NODES:
+      case GMGS_Q_INTERMEDIATE_NUM_NODES: case GMGS_Q_STORING_STATE: case GMGS_Q_NUM_TILES:
+      case GMGS_Q_NUM_FREE_COARSEST:
+        n = 1; break;
+      case GMGS_Q_RES: n = 3; break;
+      case GMGS_Q_ELE_MAP_BACK: n = l->mesh.ne; break;
+      case GMGS_Q_ELE_MAP_FORWARD: n = (int64_t)g->ey * g->ex * g->ez; break;
+      case GMGS_Q_NOD_MAP_BACK: case GMGS_Q_NUM_NOD2ELES: n = l->mesh.nn; break;
+      case GMGS_Q_NOD_MAP_FORWARD: n = (int64_t)g->ny * g->nx * g->nz; break;
+      case GMGS_Q_ENODMAT: case GMGS_Q_ELEMENT_UPWARD_MAP: n = 8 * l->mesh.ne; break;
+      case GMGS_Q_NODES_ON_BOUNDARY: n = l->mesh.n_nob; break;
+      case GMGS_Q_ELEMENTS_ON_BOUNDARY: n = l->mesh.n_eob; break;
+      case GMGS_Q_TRANSFER_MAT: n = 27 * l->mesh.ne; break;
+      case GMGS_Q_TRANSFER_MAT_COEFFI: n = l->mesh.intermediateNumNodes; break;
+      case GMGS_Q_SOLID_NODE_MAP_C2F: n = level >= 2 ? c->lev[level - 2].mesh.nn : 0; break;
+      case GMGS_Q_UNIQUE_ELEMENTS: n = l->nuniq; break;
+      case GMGS_Q_KS: n = 576 * l->mesh.ne; break;
+      case GMGS_Q_DIAGK: case GMGS_Q_F: n = 3 * l->mesh.nn; break;
+      case GMGS_Q_FIXED_DOFS: case GMGS_Q_FREE_DOFS: {
+        std::vector<uint32_t> fl = download<uint32_t>(l->mesh.flags, (size_t)g->N, c->stream);
+        std::vector<long long> np = download<long long>(l->mesh.node_pidx, (size_t)l->mesh.nn, c->stream);
+        int64_t nf = 0;
+        for (long long k = 0; k < l->mesh.nn; ++k) nf += __builtin_popcount((fl[np[k]] >> F_FIX_SHIFT) & 7u);
+        n = key == GMGS_Q_FIXED_DOFS ? nf : 3 * l->mesh.nn - nf;
+        break;
+      }
+      case GMGS_Q_KE0: n = 576; break;
+      case GMGS_Q_BBOX: n = 6; break;
+      default: throw Status(GMGS_ERR_ARGUMENT, "unknown query key");
+    }
+    *count = n;
+    return GMGS_OK;
+  });
+}
+
+int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
+  if (!c || c->lev.empty() || !out) return GMGS_ERR_ARGUMENT;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    int64_t need = 0;
+    int st = gmgs_query_size(c, key, level, &need);
+    if (st != GMGS_OK) return st;
+    if (count < need) throw Status(GMGS_ERR_ARGUMENT, "output buffer too small");
+    hipStream_t s = c->stream;
+    const Level* l = (level >= 1 && level <= c->L) ? &c->lev[level - 1] : nullptr;
+    auto i64 = static_cast<int64_t*>(out);
+    auto i32 = static_cast<int32_t*>(out);
+    auto f64 = static_cast<double*>(out);
+    auto copy_dev = [&](const DevBuf& b, size_t bytes) {
+      if (b.empty()) throw Status(GMGS_ERR_STATE, "table not built (GMGS_FLAG_NO_INDEX_TABLES?)");
+      if (bytes) GMGS_HIP(hipMemcpyAsync(out, b.as<void>(), bytes, hipMemcpyDeviceToHost, s));
+      GMGS_HIP(hipStreamSynchronize(s));
+    };
+    switch (key) {
+      case GMGS_Q_NUM_LEVELS: i64[0] = c->L; break;
+      case GMGS_Q_RES: i64[0] = l->mesh.g.ex; i64[1] = l->mesh.g.ey; i64[2] = l->mesh.g.ez; break;
+      case GMGS_Q_NUM_ELEMENTS: i64[0] = l->mesh.ne; break;
+      case GMGS_Q_NUM_NODES: i64[0] = l->mesh.nn; break;
+      case GMGS_Q_INTERMEDIATE_NUM_NODES: i64[0] = l->mesh.intermediateNumNodes; break;
+      case GMGS_Q_STORING_STATE: i64[0] = level == 1 ? 0 : l->state; break;
+      case GMGS_Q_NUM_TILES: i64[0] = l->ntiles; break;
+      case GMGS_Q_NUM_FREE_COARSEST: i64[0] = c->nfree; break;
+      case GMGS_Q_ELE_MAP_BACK: copy_dev(l->mesh.eleMapBack, 4 * need); break;
+      case GMGS_Q_ELE_MAP_FORWARD: copy_dev(l->mesh.eleMapForward, 4 * need); break;
+      case GMGS_Q_NOD_MAP_BACK: copy_dev(l->mesh.nodMapBack, 4 * need); break;
+      case GMGS_Q_NOD_MAP_FORWARD: copy_dev(l->mesh.nodMapForward, 4 * need); break;
+      case GMGS_Q_ENODMAT: copy_dev(l->mesh.eNodMat, 4 * need); break;
+      case GMGS_Q_NUM_NOD2ELES: copy_dev(l->mesh.numNod2Eles, 4 * need); break;
+      case GMGS_Q_NODES_ON_BOUNDARY: copy_dev(l->mesh.nodesOnBoundary, 4 * need); break;
+      case GMGS_Q_ELEMENTS_ON_BOUNDARY: copy_dev(l->mesh.elementsOnBoundary, 4 * need); break;
+      case GMGS_Q_ELEMENT_UPWARD_MAP: copy_dev(l->mesh.elementUpwardMap, 4 * need); break;
+      case GMGS_Q_TRANSFER_MAT: copy_dev(l->mesh.transferMat, 4 * need); break;
+      case GMGS_Q_TRANSFER_MAT_COEFFI: copy_dev(l->mesh.transferMatCoeffi, 8 * need); break;
+      case GMGS_Q_SOLID_NODE_MAP_C2F: copy_dev(l->mesh.solidNodeMapCoarser2Finer, 4 * need); break;
+      case GMGS_Q_UNIQUE_ELEMENTS: if (need) copy_dev(l->uniq, 4 * need); break;
+      case GMGS_Q_KS: {
+        const long long ne = l->mesh.ne;
+        if (level == 1) {
+          for (long long k = 0; k < ne; ++k) std::memcpy(f64 + 576 * k, c->Ke0, sizeof(double) * 576);
+          break;
+        }
+        std::vector<long long> ep = download<long long>(l->mesh.elem_pidx, (size_t)ne, s);
+        std::vector<int> eop = download<int>(l->eop, (size_t)l->mesh.g.N, s);
+        std::vector<double> Ks = download<double>(l->Ks64, l->Ks64.bytes() / sizeof(double), s);
+        std::vector<double> Ke = download<double>(l->Ke64, 576, s);
+        for (long long k = 0; k < ne; ++k) {
+          const int e = eop[ep[k]];
+          const double* src = e == 0 ? Ke.data() : Ks.data() + (size_t)(e - 1) * 576;
+          std::memcpy(f64 + 576 * k, src, sizeof(double) * 576);
+        }
+        break;
+      }
+      case GMGS_Q_DIAGK: {
+        if (l->invd64.empty()) throw Status(GMGS_ERR_ARGUMENT, "no diagonal on the coarsest level");
+        std::vector<double> iv = download<double>(l->invd64, (size_t)(3 * l->mesh.g.N), s);
+        std::vector<long long> np = download<long long>(l->mesh.node_pidx, (size_t)l->mesh.nn, s);
+        for (long long k = 0; k < l->mesh.nn; ++k)
+          for (int cc = 0; cc < 3; ++cc) f64[3 * k + cc] = 1.0 / iv[cc * l->mesh.g.N + np[k]];
+        break;
+      }
+      case GMGS_Q_FIXED_DOFS: case GMGS_Q_FREE_DOFS: {
+        std::vector<uint32_t> fl = download<uint32_t>(l->mesh.flags, (size_t)l->mesh.g.N, s);
+        std::vector<long long> np = download<long long>(l->mesh.node_pidx, (size_t)l->mesh.nn, s);
+        int64_t j = 0;
+        for (long long k = 0; k < l->mesh.nn; ++k)
+          for (int cc = 0; cc < 3; ++cc) {
+            const bool fx = (fl[np[k]] >> (F_FIX_SHIFT + cc)) & 1u;
+            if (fx == (key == GMGS_Q_FIXED_DOFS)) i32[j++] = (int32_t)(3 * k + cc + 1);
+          }
+        break;
+      }
+      case GMGS_Q_KE0: std::memcpy(f64, c->Ke0, sizeof(double) * 576); break;
+      case GMGS_Q_BBOX: std::memcpy(f64, c->bbox, sizeof(double) * 6); break;
+      case GMGS_Q_F:
+        if (!c->bc || level != 1) throw Status(GMGS_ERR_STATE, "F_ exists on level 1 after gmgs_set_bc");
+        std::memcpy(f64, c->F.data(), sizeof(double) * c->F.size());
+        break;
+      default: throw Status(GMGS_ERR_ARGUMENT, "unknown query key");
+    }
+    return GMGS_OK;
+  });
+}
+
+int gmgs_kernel_timing(gmgs_ctx* c, int enable) {
+  if (!c || c->lev.empty()) return GMGS_ERR_ARGUMENT;
+  c->ktime = enable != 0;
+  c->kpairs.clear();
+  c->kpool_used = 0;
+  for (int i = 0; i < 8; ++i) { c->ktot[i] = 0; c->kcnt[i] = 0; }
+  return GMGS_OK;
+}
+
+int gmgs_kernel_timing_read(gmgs_ctx* c, int kernel, double* avg_ms, int64_t* count) {
+  if (!c || kernel < 0 || kernel > 5 || !avg_ms || !count) return GMGS_ERR_ARGUMENT;
+  *count = c->kcnt[kernel];
+  *avg_ms = c->kcnt[kernel] ? c->ktot[kernel] / (double)c->kcnt[kernel] : 0.0;
+  return GMGS_OK;
+}
+
+const char* gmgs_last_error(const gmgs_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+void gmgs_destroy(gmgs_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->lev.clear();
+  for (hipEvent_t e : c->kpool) (void)hipEventDestroy(e);
+  hipStream_t s = c->stream;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+}  // extern "C"

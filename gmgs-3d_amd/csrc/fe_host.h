@@ -1,0 +1,28 @@
+// Host-side finite-element constants (see fe_host.cpp).
+#pragma once
+#include "common.h"
+
+namespace gmgs {
+
+// Isotropic Hooke matrix, common/ElementElasticityMatrix.m:11-24.
+void hooke(double E, double nu, double S[6][6]);
+
+// Ke0 (24x24 row-major, DOF order [x1 y1 z1 ... z8]) by 2x2x2 Gauss
+// quadrature on the probe element, common/SetupStiffnessMatrix.m:4-22.
+void element_stiffness(double E, double nu, const double probe[8][3], double* Ke);
+
+// Nodal stress operator M = OTP*D*B (48x24 row-major; row 6n+a = stress
+// component a [xx yy zz yz zx xy] at local node n), GetCartesianStress.m:17-36.
+bool stress_operator(double E, double nu, const double probe[8][3], double* M);
+
+// Span-2 trilinear weights inNODE (27x8).
+void transfer_weights(double inNODE[27][8]);
+
+// Per child slot s (SubEleNodMat.m:4-11 order) the 8x8 weight block
+// W[s][i][k] = inNODE(SubEleNodMat(s,i), k).
+void child_weights(double W[8][8][8]);
+
+// Interior 27-point stencil assembled from 8 copies of Ke.
+void interior_stencil(const double* Ke, double S[27][3][3]);
+
+}  // namespace gmgs

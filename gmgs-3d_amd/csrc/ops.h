@@ -1,0 +1,107 @@
+// Level operators: matrix-free apply, Jacobi smoothing, transfers and the
+// fused PCG vector updates.
+#pragma once
+#include "common.h"
+
+namespace gmgs {
+
+struct TileRec {
+  int y0, x0, z, pad;   // padded coordinates of the tile origin
+};
+
+// Operator of one level in precision T.
+template <typename T>
+struct LevelOp {
+  Grid g;
+  const uint32_t* flags;     // per padded node
+  const int* eop;            // per padded cell: -1 absent, 0 universal, k>0 stored Ks[k-1]; null on level 1
+  const T* Ks;               // stored element matrices, 576 each, row-major
+  const T* Ke;               // universal element matrix of this level (576, row-major)
+  const T* stencil;          // interior 27x3x3 stencil of Ke (for F_FAST nodes)
+  const T* emod;             // per padded cell modulus factor (level-1 SIMP) or null
+  const T* invd;             // inverse diagonal, SoA 3N (zero off the solid nodes)
+};
+
+struct PcgScalars {
+  double normB;
+  double rz;          // r1'z2 of this iteration
+  double rz_prev;     // r0'z1
+  double rz_old;      // r1'z1 (flexible beta)
+  double pq;
+  double rr;
+  double alpha, beta;
+  double relres;
+  double tol;
+  int its;
+  int done;
+  int converged;
+  int max_it;
+  int flexible;
+  int pad;
+  unsigned long long t0;
+};
+
+enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1 };
+
+// ---- launches (all on stream s; 'done' may be null) ----
+template <typename T, typename TR>
+void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode,
+                  const TR* in, T omega, T* out, TR* xacc, double* partial, const int* done,
+                  hipStream_t s);
+
+template <typename T>
+void launch_restrict(const Grid& gc, const uint32_t* flags_c, const TileRec* tiles_c, long long nt,
+                     const Grid& gf, const T* rf, T* rc, const int* done, hipStream_t s);
+
+// x = omega*invd*(r + rp) + P xc on the fine level; if 'fix' zero fixed
+// components; optional dot partials vs rdot (FP64): sum rdot.x and rdot.zprev
+template <typename T, typename TR, typename TZ>
+void launch_prolong_smooth(const Grid& gf, const uint32_t* flags_f, const TileRec* tiles, long long nt,
+                           const Grid& gc, const T* xc, const T* invd, const TR* r, const T* rp,
+                           T omega, TZ* x, bool fix, const TZ* zprev, double* partial,
+                           const int* done, hipStream_t s);
+
+template <typename TZ>
+void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                     const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s);
+
+void launch_update_xr(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                      double* x, const double* p, const double* ra, const double* q, double* r,
+                      const PcgScalars* sc, double* partial, const int* done, hipStream_t s);
+
+// dot / norm partials over tiles (FP64)
+void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                const double* a, const double* b, double* partial, hipStream_t s);
+
+// scalar steps of the PCG iteration; 'which': 0 = init normB, 1 = beta, 2 = alpha, 3 = residual
+void launch_finalize(int which, const double* partial, int nparts, PcgScalars* sc,
+                     double* hist_relres, unsigned long long* hist_t, const int* done, hipStream_t s);
+
+template <typename TS, typename TD>
+void launch_convert(const TS* a, TD* b, long long n, hipStream_t s);
+
+template <typename T>
+void launch_fill(T* a, T v, long long n, hipStream_t s);
+
+// c = a - b (dense arrays)
+void launch_sub(const double* a, const double* b, double* c, long long n, hipStream_t s);
+
+// compact interleaved (3*nn) <-> dense SoA
+void launch_scatter_compact(const long long* node_pidx, long long nn, const double* comp,
+                            double* dense, long long N, hipStream_t s);
+template <typename T>
+void launch_gather_compact(const long long* node_pidx, long long nn, const T* dense, long long N,
+                           double* comp, hipStream_t s);
+
+// nodal stress (GetCartesianStress.m:32-41): S (nn x 6, column-major) from the
+// dense level-1 displacement; M = OTP*D*B (48x24)
+void launch_stress(const Grid& g, const long long* node_pidx, long long nn, const uint32_t* flags,
+                   const double* M, const double* emod, const double* u, double* S, hipStream_t s);
+
+// per compact element: out[k] = -lam_e' Ke u_e
+void launch_element_energy(const Grid& g, const long long* elem_pidx, long long ne, const double* Ke,
+                           const double* lam, const double* u, double* out, hipStream_t s);
+
+inline unsigned apply_blocks(long long ntiles) { return (unsigned)((ntiles + 3) / 4); }
+
+}  // namespace gmgs

@@ -1,0 +1,627 @@
+// Level operators on the dense padded grids.
+//
+// One wavefront owns one 16x4x1 tile of nodes; a lane owns one node and
+// pulls everything it needs from its 3x3x3 neighbourhood, so no output is
+// ever written by two lanes (no atomics, bitwise-reproducible sums).
+//
+//   apply  : y = A_l u  (kernel/AtX.m:1-58), optionally fused with the damped
+//            Jacobi step u = omega*r./diagK, r' = r - A u (JacobiSmoother.m:5-9)
+//   restrict, prolong: span-2 trilinear transfers (RestrictResidual.m,
+//            InterpolationDeviation.m) written on the structured grid.
+#include "ops.h"
+
+namespace gmgs {
+
+namespace {
+
+__device__ inline double block_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
+struct LaneNode {
+  long long n;
+  uint32_t fl;
+  bool active;
+  int yp, xp, zp;
+};
+
+__device__ inline LaneNode lane_node(const Grid& g, const uint32_t* __restrict__ flags,
+                                     const TileRec* __restrict__ tiles, long long ntiles) {
+  LaneNode r;
+  r.active = false;
+  r.n = 0;
+  r.fl = 0;
+  const int lane = threadIdx.x & 63;
+  const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  r.yp = r.xp = r.zp = 0;
+  if (t < ntiles) {
+    const TileRec tr = tiles[t];
+    r.yp = tr.y0 + (lane & 15);
+    r.xp = tr.x0 + (lane >> 4);
+    r.zp = tr.z;
+    if (r.xp < g.px) {
+      r.n = r.yp + (long long)g.py * (r.xp + (long long)g.px * r.zp);
+      r.fl = flags[r.n];
+      r.active = (r.fl & F_ELEM_MASK) != 0;
+    }
+  }
+  return r;
+}
+
+template <typename T, typename TR, int MODE>
+struct InputU {
+  const TR* __restrict__ in;
+  const T* __restrict__ invd;
+  T omega;
+  long long N;
+  __device__ inline T operator()(long long m, int d) const {
+    if (MODE == AP_AX) return (T)in[d * N + m];
+    return omega * invd[d * N + m] * (T)in[d * N + m];
+  }
+};
+
+template <typename T, typename TR, int MODE, bool HAS_EOP, bool HAS_EMOD>
+__global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __restrict__ tiles,
+                                               long long ntiles, const TR* __restrict__ in, T omega,
+                                               T* __restrict__ out, TR* __restrict__ xacc,
+                                               double* __restrict__ partial,
+                                               const int* __restrict__ done) {
+  if (done && *done) return;
+  const LaneNode ln_ = lane_node(op.g, op.flags, tiles, ntiles);
+  double dsum = 0.0;
+  if (ln_.active) {
+    const long long N = op.g.N;
+    const long long n = ln_.n;
+    const uint32_t fl = ln_.fl;
+    const long long sx = op.g.py, sz = (long long)op.g.py * op.g.px;
+    InputU<T, TR, MODE> U{in, op.invd, omega, N};
+    T a0 = 0, a1 = 0, a2 = 0;
+    if (!HAS_EMOD && (fl & F_FAST)) {
+      const T* __restrict__ S = op.stencil;
+#pragma unroll
+      for (int o = 0; o < 27; ++o) {
+        const int dy = o % 3 - 1, dx = (o / 3) % 3 - 1, dz = o / 9 - 1;
+        const long long m = n + dy + dx * sx + dz * sz;
+        const T u0 = U(m, 0), u1 = U(m, 1), u2 = U(m, 2);
+        a0 += S[o * 9 + 0] * u0 + S[o * 9 + 1] * u1 + S[o * 9 + 2] * u2;
+        a1 += S[o * 9 + 3] * u0 + S[o * 9 + 4] * u1 + S[o * 9 + 5] * u2;
+        a2 += S[o * 9 + 6] * u0 + S[o * 9 + 7] * u1 + S[o * 9 + 8] * u2;
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        if (!((fl >> p) & 1u)) continue;
+        const int pyb = p & 1, pxb = (p >> 1) & 1, pzb = p >> 2;
+        const long long cell = n - (1 - pyb) - (1 - pxb) * sx - (1 - pzb) * sz;
+        const T* __restrict__ K = op.Ke;
+        if (HAS_EOP) {
+          const int e = op.eop[cell];
+          if (e > 0) K = op.Ks + (long long)(e - 1) * 576;
+        }
+        const int ln = node_local_in(p);
+        T e0 = 0, e1 = 0, e2 = 0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const long long mm = n + (pyb - 1 + hex_ro(m)) + (pxb - 1 + hex_xo(m)) * sx
+                               + (pzb - 1 + hex_zo(m)) * sz;
+          const T u0 = U(mm, 0), u1 = U(mm, 1), u2 = U(mm, 2);
+          const T* k0 = K + (3 * ln + 0) * 24 + 3 * m;
+          const T* k1 = K + (3 * ln + 1) * 24 + 3 * m;
+          const T* k2 = K + (3 * ln + 2) * 24 + 3 * m;
+          e0 += k0[0] * u0 + k0[1] * u1 + k0[2] * u2;
+          e1 += k1[0] * u0 + k1[1] * u1 + k1[2] * u2;
+          e2 += k2[0] * u0 + k2[1] * u1 + k2[2] * u2;
+        }
+        if (HAS_EMOD) {
+          const T s = op.emod[cell];
+          e0 *= s; e1 *= s; e2 *= s;
+        }
+        a0 += e0; a1 += e1; a2 += e2;
+      }
+    }
+    // fixed DOFs: AtX output zeroed (kernel/AtX.m:57)
+    if (fl & (1u << (F_FIX_SHIFT + 0))) a0 = 0;
+    if (fl & (1u << (F_FIX_SHIFT + 1))) a1 = 0;
+    if (fl & (1u << (F_FIX_SHIFT + 2))) a2 = 0;
+    if (MODE == AP_AX) {
+      out[n] = a0;
+      out[N + n] = a1;
+      out[2 * N + n] = a2;
+      if (partial) {
+        dsum = (double)in[n] * (double)a0 + (double)in[N + n] * (double)a1
+               + (double)in[2 * N + n] * (double)a2;
+      }
+    } else {
+      out[n] = (T)in[n] - a0;
+      out[N + n] = (T)in[N + n] - a1;
+      out[2 * N + n] = (T)in[2 * N + n] - a2;
+      if (xacc) {
+        xacc[n] += (TR)U(n, 0);
+        xacc[N + n] += (TR)U(n, 1);
+        xacc[2 * N + n] += (TR)U(n, 2);
+      }
+    }
+  }
+  if (partial) {
+    const double b = block_sum(dsum);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(Grid gc, const uint32_t* __restrict__ flags,
+                                                  const TileRec* __restrict__ tiles, long long nt,
+                                                  Grid gf, const T* __restrict__ rf,
+                                                  T* __restrict__ rc, const int* __restrict__ done) {
+  if (done && *done) return;
+  const LaneNode ln_ = lane_node(gc, flags, tiles, nt);
+  if (!ln_.active) return;
+  const long long Nf = gf.N;
+  const long long sx = gf.py, sz = (long long)gf.py * gf.px;
+  const long long nf = gf.pidx(2 * (ln_.yp - 1), 2 * (ln_.xp - 1), 2 * (ln_.zp - 1));
+  T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    const int dy = o % 3 - 1, dx = (o / 3) % 3 - 1, dz = o / 9 - 1;
+    const T w = (dy ? (T)0.5 : (T)1) * (dx ? (T)0.5 : (T)1) * (dz ? (T)0.5 : (T)1);
+    const long long m = nf + dy + dx * sx + dz * sz;
+    a0 += w * rf[m];
+    a1 += w * rf[Nf + m];
+    a2 += w * rf[2 * Nf + m];
+  }
+  const long long N = gc.N;
+  rc[ln_.n] = a0;
+  rc[N + ln_.n] = a1;
+  rc[2 * N + ln_.n] = a2;
+}
+
+template <typename T, typename TR, typename TZ>
+__global__ __launch_bounds__(256) void k_prolong_smooth(
+    Grid gf, const uint32_t* __restrict__ flags, const TileRec* __restrict__ tiles, long long nt,
+    Grid gc, const T* __restrict__ xc, const T* __restrict__ invd, const TR* __restrict__ r,
+    const T* __restrict__ rp, T omega, TZ* __restrict__ x, int fix, const TZ* __restrict__ zprev,
+    double* __restrict__ partial, const int* __restrict__ done) {
+  if (done && *done) return;
+  const LaneNode ln_ = lane_node(gf, flags, tiles, nt);
+  double d0 = 0.0, d1 = 0.0;
+  if (ln_.active) {
+    const int yf = ln_.yp - 1, xf = ln_.xp - 1, zf = ln_.zp - 1;
+    const int ya = yf >> 1, yb = (yf + 1) >> 1, xa = xf >> 1, xb = (xf + 1) >> 1, za = zf >> 1,
+              zb = (zf + 1) >> 1;
+    const T wy = (yf & 1) ? (T)0.5 : (T)1, wx = (xf & 1) ? (T)0.5 : (T)1,
+            wz = (zf & 1) ? (T)0.5 : (T)1;
+    const long long Nc = gc.N, N = gf.N, n = ln_.n;
+    T p[3] = {0, 0, 0};
+    const int ys[2] = {ya, yb}, xs[2] = {xa, xb}, zs[2] = {za, zb};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int cy = ys[k & 1], cx = xs[(k >> 1) & 1], cz = zs[k >> 2];
+      // odd coordinates average two coarse nodes; even ones take one (weight on 'a')
+      const T w = (((k & 1) && !(yf & 1)) ? (T)0 : wy) * ((((k >> 1) & 1) && !(xf & 1)) ? (T)0 : wx)
+                  * (((k >> 2) && !(zf & 1)) ? (T)0 : wz);
+      const long long m = gc.pidx(cy, cx, cz);
+      p[0] += w * xc[m];
+      p[1] += w * xc[Nc + m];
+      p[2] += w * xc[2 * Nc + m];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T v = omega * invd[c * N + n] * ((T)r[c * N + n] + rp[c * N + n]) + p[c];
+      if (fix && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) v = 0;
+      x[c * N + n] = (TZ)v;
+      if (partial) {
+        d0 += (double)r[c * N + n] * (double)(TZ)v;
+        if (zprev) d1 += (double)r[c * N + n] * (double)zprev[c * N + n];
+      }
+    }
+  }
+  if (partial) {
+    const double b0 = block_sum(d0);
+    const double b1 = block_sum(d1);
+    if (threadIdx.x == 0) {
+      partial[blockIdx.x] = b0;
+      partial[gridDim.x + blockIdx.x] = b1;
+    }
+  }
+}
+
+template <typename TZ>
+__global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __restrict__ flags,
+                                                  const TileRec* __restrict__ tiles, long long nt,
+                                                  const TZ* __restrict__ z, double* __restrict__ p,
+                                                  const PcgScalars* __restrict__ sc,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  const LaneNode ln_ = lane_node(g, flags, tiles, nt);
+  if (!ln_.active) return;
+  const double beta = sc->beta;
+  const long long N = g.N, n = ln_.n;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) p[c * N + n] = (double)z[c * N + n] + beta * p[c * N + n];
+}
+
+__global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __restrict__ flags,
+                                                   const TileRec* __restrict__ tiles, long long nt,
+                                                   double* __restrict__ x, const double* __restrict__ p,
+                                                   const double* __restrict__ ra,
+                                                   const double* __restrict__ q, double* __restrict__ r,
+                                                   const PcgScalars* __restrict__ sc,
+                                                   double* __restrict__ partial,
+                                                   const int* __restrict__ done) {
+  if (done && *done) return;
+  const LaneNode ln_ = lane_node(g, flags, tiles, nt);
+  double d = 0.0;
+  if (ln_.active) {
+    const double alpha = sc->alpha;
+    const long long N = g.N, n = ln_.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[c * N + n] += alpha * p[c * N + n];
+      const double rv = ra[c * N + n] - alpha * q[c * N + n];
+      r[c * N + n] = rv;
+      d += rv * rv;
+    }
+  }
+  const double b = block_sum(d);
+  if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(256) void k_dot(Grid g, const uint32_t* __restrict__ flags,
+                                             const TileRec* __restrict__ tiles, long long nt,
+                                             const double* __restrict__ a,
+                                             const double* __restrict__ b,
+                                             double* __restrict__ partial) {
+  const LaneNode ln_ = lane_node(g, flags, tiles, nt);
+  double d = 0.0;
+  if (ln_.active) {
+    const long long N = g.N, n = ln_.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d += a[c * N + n] * b[c * N + n];
+  }
+  const double s = block_sum(d);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// one block of 256 threads; fixed-order sums
+__device__ inline double reduce_parts(const double* __restrict__ part, int n) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += part[i];
+  __shared__ double sh[256];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_finalize(int which, const double* __restrict__ part,
+                                                  int nparts, PcgScalars* __restrict__ sc,
+                                                  double* __restrict__ hist_rel,
+                                                  unsigned long long* __restrict__ hist_t,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  if (which == 0) {
+    const double s = reduce_parts(part, nparts);
+    if (threadIdx.x == 0) {
+      sc->normB = sqrt(s);
+      sc->its = 0;
+      sc->done = 0;
+      sc->converged = 0;
+      sc->rz_prev = 0.0;
+      sc->t0 = wall_clock64();
+    }
+  } else if (which == 1) {
+    const double rz = reduce_parts(part, nparts);
+    const double rzo = reduce_parts(part + nparts, nparts);
+    if (threadIdx.x == 0) {
+      sc->rz = rz;
+      sc->rz_old = rzo;
+      sc->its += 1;
+      if (sc->its == 1) sc->beta = 0.0;
+      else if (sc->flexible) sc->beta = (rz - rzo) / sc->rz_prev;   // Polak-Ribiere
+      else sc->beta = rz / sc->rz_prev;                             // CG_solver.m:26
+    }
+  } else if (which == 2) {
+    const double pq = reduce_parts(part, nparts);
+    if (threadIdx.x == 0) {
+      sc->pq = pq;
+      sc->alpha = sc->rz / pq;                                      // CG_solver.m:31
+    }
+  } else {
+    const double rr = reduce_parts(part, nparts);
+    if (threadIdx.x == 0) {
+      sc->rr = rr;
+      const double rel = sqrt(rr) / sc->normB;                      // CG_solver.m:35
+      sc->relres = rel;
+      const int k = sc->its - 1;
+      hist_rel[k] = rel;
+      hist_t[k] = wall_clock64();
+      if (rel < sc->tol) {
+        sc->converged = 1;
+        sc->done = 1;
+      } else if (!(rel == rel) || sc->its >= sc->max_it + 1) {
+        sc->done = 1;
+      }
+      sc->rz_prev = sc->rz;
+    }
+  }
+}
+
+template <typename TS, typename TD>
+__global__ void k_convert(const TS* __restrict__ a, TD* __restrict__ b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = (TD)a[i];
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ a, T v, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+__global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ c, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    c[i] = a[i] - b[i];
+}
+
+__global__ void k_scatter_compact(const long long* __restrict__ pidx, long long nn,
+                                  const double* __restrict__ comp, double* __restrict__ dense,
+                                  long long N) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  const long long p = pidx[k];
+  dense[p] = comp[3 * k];
+  dense[N + p] = comp[3 * k + 1];
+  dense[2 * N + p] = comp[3 * k + 2];
+}
+
+template <typename T>
+__global__ void k_gather_compact(const long long* __restrict__ pidx, long long nn,
+                                 const T* __restrict__ dense, long long N, double* __restrict__ comp) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  const long long p = pidx[k];
+  comp[3 * k] = (double)dense[p];
+  comp[3 * k + 1] = (double)dense[N + p];
+  comp[3 * k + 2] = (double)dense[2 * N + p];
+}
+
+__global__ __launch_bounds__(256) void k_stress(Grid g, const long long* __restrict__ npidx,
+                                              long long nn, const uint32_t* __restrict__ flags,
+                                              const double* __restrict__ M,
+                                              const double* __restrict__ emod,
+                                              const double* __restrict__ u,
+                                              double* __restrict__ S) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  const long long n = npidx[k], N = g.N;
+  const uint32_t fl = flags[n];
+  const long long sx = g.py, sz = (long long)g.py * g.px;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int p = 0; p < 8; ++p) {
+    if (!((fl >> p) & 1u)) continue;
+    const int pyb = p & 1, pxb = (p >> 1) & 1, pzb = p >> 2;
+    const long long cell = n - (1 - pyb) - (1 - pxb) * sx - (1 - pzb) * sz;
+    const int ln = node_local_in(p);
+    double e[6] = {0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < 8; ++m) {
+      const long long mm = cell + hex_ro(m) + hex_xo(m) * sx + hex_zo(m) * sz;
+      const double u0 = u[mm], u1 = u[N + mm], u2 = u[2 * N + mm];
+      for (int a = 0; a < 6; ++a) {
+        const double* r = M + (6 * ln + a) * 24 + 3 * m;
+        e[a] += r[0] * u0 + r[1] * u1 + r[2] * u2;
+      }
+    }
+    const double s = emod ? emod[cell] : 1.0;
+    for (int a = 0; a < 6; ++a) acc[a] += s * e[a];
+  }
+  const double cnt = (double)__popc(fl & F_ELEM_MASK);
+  for (int a = 0; a < 6; ++a) S[k + nn * a] = acc[a] / cnt;
+}
+
+__global__ __launch_bounds__(256) void k_element_energy(Grid g, const long long* __restrict__ epidx,
+                                                        long long ne, const double* __restrict__ Ke,
+                                                        const double* __restrict__ lam,
+                                                        const double* __restrict__ u,
+                                                        double* __restrict__ out) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ne) return;
+  const long long cell = epidx[k], N = g.N;
+  const long long sx = g.py, sz = (long long)g.py * g.px;
+  double le[24], ue[24];
+  for (int m = 0; m < 8; ++m) {
+    const long long mm = cell + hex_ro(m) + hex_xo(m) * sx + hex_zo(m) * sz;
+    for (int d = 0; d < 3; ++d) {
+      le[3 * m + d] = lam[d * N + mm];
+      ue[3 * m + d] = u[d * N + mm];
+    }
+  }
+  double acc = 0.0;
+  for (int i = 0; i < 24; ++i) {
+    double t = 0.0;
+    for (int j = 0; j < 24; ++j) t += Ke[i * 24 + j] * ue[j];
+    acc += le[i] * t;
+  }
+  out[k] = -acc;
+}
+
+}  // namespace
+
+void launch_stress(const Grid& g, const long long* node_pidx, long long nn, const uint32_t* flags,
+                   const double* M, const double* emod, const double* u, double* S, hipStream_t s) {
+  if (nn <= 0) return;
+  k_stress<<<grid1d(nn, 256), 256, 0, s>>>(g, node_pidx, nn, flags, M, emod, u, S);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_element_energy(const Grid& g, const long long* elem_pidx, long long ne, const double* Ke,
+                           const double* lam, const double* u, double* out, hipStream_t s) {
+  if (ne <= 0) return;
+  k_element_energy<<<grid1d(ne, 256), 256, 0, s>>>(g, elem_pidx, ne, Ke, lam, u, out);
+  GMGS_LAUNCH_CHECK();
+}
+
+template <typename T, typename TR>
+void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode,
+                  const TR* in, T omega, T* out, TR* xacc, double* partial, const int* done,
+                  hipStream_t s) {
+  if (ntiles <= 0) return;
+  const unsigned nb = apply_blocks(ntiles);
+  const bool eop = op.eop != nullptr, emod = op.emod != nullptr;
+#define GMGS_APPLY(MODE, E, M)                                                                 \
+  k_apply<T, TR, MODE, E, M><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, xacc, partial, done)
+  if (mode == AP_AX) {
+    if (eop) GMGS_APPLY(AP_AX, true, false);
+    else if (emod) GMGS_APPLY(AP_AX, false, true);
+    else GMGS_APPLY(AP_AX, false, false);
+  } else {
+    if (eop) GMGS_APPLY(AP_SMOOTH, true, false);
+    else if (emod) GMGS_APPLY(AP_SMOOTH, false, true);
+    else GMGS_APPLY(AP_SMOOTH, false, false);
+  }
+#undef GMGS_APPLY
+  GMGS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_restrict(const Grid& gc, const uint32_t* flags_c, const TileRec* tiles_c, long long nt,
+                     const Grid& gf, const T* rf, T* rc, const int* done, hipStream_t s) {
+  if (nt <= 0) return;
+  k_restrict<T><<<apply_blocks(nt), 256, 0, s>>>(gc, flags_c, tiles_c, nt, gf, rf, rc, done);
+  GMGS_LAUNCH_CHECK();
+}
+
+template <typename T, typename TR, typename TZ>
+void launch_prolong_smooth(const Grid& gf, const uint32_t* flags_f, const TileRec* tiles, long long nt,
+                           const Grid& gc, const T* xc, const T* invd, const TR* r, const T* rp,
+                           T omega, TZ* x, bool fix, const TZ* zprev, double* partial,
+                           const int* done, hipStream_t s) {
+  if (nt <= 0) return;
+  k_prolong_smooth<T, TR, TZ><<<apply_blocks(nt), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, invd, r,
+                                                               rp, omega, x, fix ? 1 : 0, zprev,
+                                                               partial, done);
+  GMGS_LAUNCH_CHECK();
+}
+
+template <typename TZ>
+void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                     const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s) {
+  k_update_p<TZ><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, p, sc, done);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_update_xr(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                      double* x, const double* p, const double* ra, const double* q, double* r,
+                      const PcgScalars* sc, double* partial, const int* done, hipStream_t s) {
+  k_update_xr<<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, ra, q, r, sc, partial, done);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                const double* a, const double* b, double* partial, hipStream_t s) {
+  k_dot<<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, b, partial);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_finalize(int which, const double* partial, int nparts, PcgScalars* sc,
+                     double* hist_relres, unsigned long long* hist_t, const int* done,
+                     hipStream_t s) {
+  k_finalize<<<1, 256, 0, s>>>(which, partial, nparts, sc, hist_relres, hist_t, done);
+  GMGS_LAUNCH_CHECK();
+}
+
+template <typename TS, typename TD>
+void launch_convert(const TS* a, TD* b, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  k_convert<TS, TD><<<nb, 256, 0, s>>>(a, b, n);
+  GMGS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_fill(T* a, T v, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  k_fill<T><<<nb, 256, 0, s>>>(a, v, n);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_sub(const double* a, const double* b, double* c, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  k_sub<<<nb, 256, 0, s>>>(a, b, c, n);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_scatter_compact(const long long* node_pidx, long long nn, const double* comp,
+                            double* dense, long long N, hipStream_t s) {
+  if (nn <= 0) return;
+  k_scatter_compact<<<grid1d(nn, 256), 256, 0, s>>>(node_pidx, nn, comp, dense, N);
+  GMGS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_gather_compact(const long long* node_pidx, long long nn, const T* dense, long long N,
+                           double* comp, hipStream_t s) {
+  if (nn <= 0) return;
+  k_gather_compact<T><<<grid1d(nn, 256), 256, 0, s>>>(node_pidx, nn, dense, N, comp);
+  GMGS_LAUNCH_CHECK();
+}
+
+// explicit instantiations
+template void launch_apply<double, double>(const LevelOp<double>&, const TileRec*, long long, int,
+                                           const double*, double, double*, double*, double*,
+                                           const int*, hipStream_t);
+template void launch_apply<float, float>(const LevelOp<float>&, const TileRec*, long long, int,
+                                         const float*, float, float*, float*, double*, const int*,
+                                         hipStream_t);
+template void launch_apply<float, double>(const LevelOp<float>&, const TileRec*, long long, int,
+                                          const double*, float, float*, double*, double*,
+                                          const int*, hipStream_t);
+template void launch_restrict<double>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                      const Grid&, const double*, double*, const int*, hipStream_t);
+template void launch_restrict<float>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                     const Grid&, const float*, float*, const int*, hipStream_t);
+template void launch_prolong_smooth<double, double, double>(
+    const Grid&, const uint32_t*, const TileRec*, long long, const Grid&, const double*,
+    const double*, const double*, const double*, double, double*, bool, const double*, double*,
+    const int*, hipStream_t);
+template void launch_prolong_smooth<float, float, float>(
+    const Grid&, const uint32_t*, const TileRec*, long long, const Grid&, const float*, const float*,
+    const float*, const float*, float, float*, bool, const float*, double*, const int*, hipStream_t);
+template void launch_prolong_smooth<float, double, float>(
+    const Grid&, const uint32_t*, const TileRec*, long long, const Grid&, const float*, const float*,
+    const double*, const float*, float, float*, bool, const float*, double*, const int*, hipStream_t);
+template void launch_update_p<double>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                      const double*, double*, const PcgScalars*, const int*,
+                                      hipStream_t);
+template void launch_update_p<float>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                     const float*, double*, const PcgScalars*, const int*,
+                                     hipStream_t);
+template void launch_convert<double, float>(const double*, float*, long long, hipStream_t);
+template void launch_convert<float, double>(const float*, double*, long long, hipStream_t);
+template void launch_fill<double>(double*, double, long long, hipStream_t);
+template void launch_fill<float>(float*, float, long long, hipStream_t);
+template void launch_fill<int>(int*, int, long long, hipStream_t);
+template void launch_gather_compact<double>(const long long*, long long, const double*, long long,
+                                            double*, hipStream_t);
+template void launch_gather_compact<float>(const long long*, long long, const float*, long long,
+                                           double*, hipStream_t);
+
+}  // namespace gmgs

@@ -1,0 +1,162 @@
+/*
+ * C ABI of the MI355X geometric-multigrid PCG solver for static linear
+ * elasticity on voxel hex8 meshes.
+ *
+ * It replaces the reference's MATLAB solve path (paths relative to the
+ * reference's src/):
+ *   gmgs_create    common/DiscretizeDesignDomain.m:29-120 + kernel/BuildingMeshHierarchy.m
+ *                  + kernel/SetupSolver.m:8-244 (mesh, hierarchy, coarse operators, smoothers)
+ *   gmgs_set_bc    common/ApplyBoundaryCondition.m:7-23 + kernel/SetupSolver.m:246-259
+ *   gmgs_solve     kernel/GetDeformation.m:18-21 -> kernel/CG_solver.m:1-56 (U_, iterationHist_)
+ *   gmgs_stress    kernel/GetCartesianStress.m:1-42 (cartesianStressField_)
+ *   gmgs_atx       kernel/AtX.m:1-58          (the @AtX operator handle)
+ *   gmgs_vcycle    kernel/Vcycle.m:1-36       (the @Vcycle preconditioner handle)
+ *   gmgs_smooth    kernel/JacobiSmoother.m:1-10
+ *   gmgs_restrict  kernel/RestrictResidual.m:1-18
+ *   gmgs_interpolate kernel/InterpolationDeviation.m:1-18
+ *   gmgs_query     read-back of meshHierarchy_ fields (kernel/MeshStruct.m:2-33)
+ * gmgs_set_modulus and gmgs_adjoint have no reference counterpart (per-element
+ * stiffness scaling and the adjoint gradient).
+ *
+ * Conventions: host buffers are caller-owned and copied; node ids are the
+ * reference's compact 1-based ids; vectors are node-interleaved
+ * [ux1 uy1 uz1 ux2 ...] in compact node order; 2-D arrays are column-major
+ * (MATLAB layout) unless stated otherwise.  Every call returns a gmgs_status;
+ * gmgs_last_error() gives the message.  A context owns one HIP stream on one
+ * device and is not re-entrant.
+ */
+#ifndef GMGS_H
+#define GMGS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gmgs_ctx gmgs_ctx;
+
+typedef enum gmgs_status {
+  GMGS_OK = 0,
+  GMGS_ERR_ARGUMENT = 1,        /* bad pointer, size or parameter */
+  GMGS_ERR_RESOLUTION = 2,      /* too few voxels for the hierarchy (DiscretizeDesignDomain.m:30-32) */
+  GMGS_ERR_LEVELS = 3,          /* fewer than 2 levels (SetupSolver.m:19-21) */
+  GMGS_ERR_NO_LOADS = 4,        /* ApplyBoundaryCondition.m:11-12 */
+  GMGS_ERR_NO_CONSTRAINTS = 5,  /* ApplyBoundaryCondition.m:22 */
+  GMGS_ERR_NODE_ID = 6,         /* node id out of range or not an integer */
+  GMGS_ERR_NONFINITE = 7,       /* NaN/Inf in an input */
+  GMGS_ERR_NOT_SPD = 8,         /* coarsest Cholesky failed (ignored by SetupSolver.m:259) */
+  GMGS_ERR_NOT_CONVERGED = 9,   /* maxIT_ exceeded (CG_solver.m:52-55); U is still written */
+  GMGS_ERR_STATE = 10,          /* call order (no BCs, no solution yet, ...) */
+  GMGS_ERR_DEVICE = 11,         /* HIP runtime error / out of device memory */
+  GMGS_ERR_BREAKDOWN = 12       /* NaN in the PCG recursion */
+} gmgs_status;
+
+enum { GMGS_PREC_FP64 = 0, GMGS_PREC_MIXED = 1 };
+
+enum {
+  GMGS_FLAG_EXACT_GALERKIN = 1,   /* true Galerkin coarse operators (no B3/B12 quirks) */
+  GMGS_FLAG_NO_INDEX_TABLES = 2,  /* skip eNodMat/transferMat/... read-back tables */
+  GMGS_FLAG_FR_BETA = 4,          /* Fletcher-Reeves beta also in mixed precision */
+  GMGS_FLAG_PR_BETA = 8           /* Polak-Ribiere (flexible) beta also in FP64 */
+};
+
+typedef struct gmgs_params {
+  double youngs_modulus;               /* moduls_            GlobalVariables.m:8  */
+  double poisson_ratio;                /* poissonRatio_      GlobalVariables.m:9  */
+  double tol;                          /* tol_               GlobalVariables.m:3  */
+  double jacobi_weight;                /* weightFactorJacobi_ GlobalVariables.m:5 */
+  double coarsest_resolution_control;  /* coarsestResolutionControl_ GlobalVariables.m:23 */
+  int32_t max_it;                      /* maxIT_             GlobalVariables.m:4  */
+  int32_t num_levels;                  /* 0 = level rule of DiscretizeDesignDomain.m:33-41 */
+  int32_t precision;                   /* GMGS_PREC_FP64 or GMGS_PREC_MIXED */
+  int32_t flags;                       /* GMGS_FLAG_* */
+  int32_t device;                      /* HIP device ordinal */
+  int32_t reserved[7];
+} gmgs_params;
+
+typedef struct gmgs_solve_info {
+  int32_t its;             /* CG iterations (CG_solver.m:22) */
+  int32_t converged;       /* relres < tol reached */
+  double relres;           /* recursive ||r||/||b|| at exit (CG_solver.m:35) */
+  double true_relres;      /* ||b - A U||/||b|| recomputed in FP64 at exit */
+  double solve_seconds;    /* device wall time of the PCG loop */
+  int64_t hist_len;        /* rows of iterationHist_ (non-final iterations) */
+} gmgs_solve_info;
+
+/* query keys (level = 1..numLevels, 1 = finest) */
+enum {
+  GMGS_Q_NUM_LEVELS = 0,        /* int64 */
+  GMGS_Q_RES = 1,               /* int64[3] resX resY resZ */
+  GMGS_Q_NUM_ELEMENTS = 2,      /* int64 */
+  GMGS_Q_NUM_NODES = 3,         /* int64 */
+  GMGS_Q_ELE_MAP_BACK = 4,      /* int32[numElements] */
+  GMGS_Q_ELE_MAP_FORWARD = 5,   /* int32[resX*resY*resZ] */
+  GMGS_Q_NOD_MAP_BACK = 6,      /* int32[numNodes] (finest-grid ids on levels >= 2) */
+  GMGS_Q_NOD_MAP_FORWARD = 7,   /* int32[(resX+1)(resY+1)(resZ+1)] */
+  GMGS_Q_ENODMAT = 8,           /* int32[numElements x 8] */
+  GMGS_Q_NUM_NOD2ELES = 9,      /* int32[numNodes] */
+  GMGS_Q_NODES_ON_BOUNDARY = 10,    /* int32[n] */
+  GMGS_Q_ELEMENTS_ON_BOUNDARY = 11, /* int32[n] */
+  GMGS_Q_ELEMENT_UPWARD_MAP = 12,   /* int32[numElements x 8] (level >= 2) */
+  GMGS_Q_TRANSFER_MAT = 13,         /* int32[27 x numElements] */
+  GMGS_Q_TRANSFER_MAT_COEFFI = 14,  /* double[intermediateNumNodes] */
+  GMGS_Q_SOLID_NODE_MAP_C2F = 15,   /* int32[numNodes of level-1] */
+  GMGS_Q_INTERMEDIATE_NUM_NODES = 16, /* int64 */
+  GMGS_Q_STORING_STATE = 17,        /* int64 */
+  GMGS_Q_UNIQUE_ELEMENTS = 18,      /* int32[n] (storing state 2) */
+  GMGS_Q_KS = 19,                   /* double[numElements x 576]: each element's 24x24 matrix */
+  GMGS_Q_DIAGK = 20,                /* double[numDOFs] (levels < numLevels) */
+  GMGS_Q_FIXED_DOFS = 21,           /* int32[n] sorted 1-based */
+  GMGS_Q_FREE_DOFS = 22,            /* int32[n] sorted 1-based */
+  GMGS_Q_KE0 = 23,                  /* double[576] */
+  GMGS_Q_BBOX = 24,                 /* double[6] adjusted bounding box [xmin ymin zmin xmax ymax zmax] */
+  GMGS_Q_F = 25,                    /* double[numDOFs] load vector F_ */
+  GMGS_Q_NUM_TILES = 26,            /* int64 active 16x4x1 tiles of the level */
+  GMGS_Q_NUM_FREE_COARSEST = 27     /* int64 */
+};
+
+void gmgs_default_params(gmgs_params* p);
+const char* gmgs_status_string(int status);
+
+int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx, int64_t nz,
+                const double* bbox, gmgs_ctx** out);
+int gmgs_set_bc(gmgs_ctx* c, const int32_t* fixed_nodes, int64_t k, const double* loads,
+                int64_t m);
+int gmgs_set_modulus(gmgs_ctx* c, const double* modulus, int64_t n);
+int gmgs_solve(gmgs_ctx* c, double* U, gmgs_solve_info* info, double* hist, int64_t hist_cap);
+int gmgs_stress(gmgs_ctx* c, const double* U, double* S);
+int gmgs_adjoint(gmgs_ctx* c, const double* g, const double* U, double* lambda, double* dJdE,
+                 gmgs_solve_info* info);
+
+int gmgs_atx(gmgs_ctx* c, int level, const double* x, double* y);
+int gmgs_vcycle(gmgs_ctx* c, const double* r, double* z);
+int gmgs_smooth(gmgs_ctx* c, int level, const double* r, double* dx, double* r_out);
+int gmgs_restrict(gmgs_ctx* c, int level, const double* r_fine, double* r_coarse);
+int gmgs_interpolate(gmgs_ctx* c, int level, const double* x_coarse, double* x_fine);
+
+/* Kernel timing for benchmarks (no reference counterpart): when enabled,
+ * every level-1 launch of the PCG iteration is bracketed by HIP events on the
+ * context's stream; reads give the mean duration over iterations that ran. */
+enum {
+  GMGS_K_MATVEC64 = 0,    /* q = A p, FP64 (CG_solver.m:29) */
+  GMGS_K_JACOBI64 = 1,    /* Jacobi step r - A(omega r/d), FP64 (CG_solver.m:18-19) */
+  GMGS_K_VSMOOTH1 = 2,    /* V-cycle level-1 pre-smoothing (Vcycle.m:14) */
+  GMGS_K_RESTRICT1 = 3,   /* level 1 -> 2 restriction (Vcycle.m:17) */
+  GMGS_K_PROLONG1 = 4,    /* level 2 -> 1 prolongation + post-smoothing (Vcycle.m:29-32) */
+  GMGS_K_ITERATION = 5    /* one whole PCG iteration */
+};
+int gmgs_kernel_timing(gmgs_ctx* c, int enable);
+int gmgs_kernel_timing_read(gmgs_ctx* c, int kernel, double* avg_ms, int64_t* count);
+
+int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count);
+int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count);
+
+const char* gmgs_last_error(const gmgs_ctx* c);
+void gmgs_destroy(gmgs_ctx* c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMGS_H */

@@ -1,0 +1,196 @@
+"""GPU parity: every hot-path function of libgmgs.so against the CPU oracle.
+
+Index work is compared bit for bit; floating-point work within the stated
+relative tolerances (FP64 path): element matrices 1e-12, operators 1e-11,
+V-cycle 1e-9, full solves 1e-6 in relative L2 with iteration counts within 1.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def holey_brick(seed=7, shape=(10, 12, 16)):
+    rng = np.random.default_rng(seed)
+    ny, nx, nz = shape
+    vox = np.ones(shape, np.int32)
+    vox[1:-1, 1:-1, 2:-2][rng.random((ny - 2, nx - 2, nz - 4)) < 0.15] = 0
+    return vox, np.array([[0, 0, 0], [nx, ny, nz]], float)
+
+
+def bc_bottom_top(oracle_model, nz, seed=3):
+    rng = np.random.default_rng(seed)
+    L1 = oracle_model.levels[0]
+    nyn, nxn = L1.resY + 1, L1.resX + 1
+    full = L1.nodMapBack
+    iz = (full - 1) // (nyn * nxn)
+    fixed = (np.nonzero(iz == 0)[0] + 1).astype(np.int32)
+    top = np.nonzero(iz == nz)[0] + 1
+    loads = np.concatenate([top[:, None], rng.standard_normal((top.size, 3))], axis=1)
+    return fixed, loads
+
+
+CASES = {
+    "holey_L3": dict(kind="holey", L=3, seed=7, shape=(10, 12, 16)),
+    "holey_L4": dict(kind="holey", L=4, seed=11, shape=(16, 16, 24)),
+    "ellip_L4": dict(kind="ellip", L=4, seed=4, shape=(16, 16, 24)),
+    "ellip_L2": dict(kind="ellip", L=2, seed=5, shape=(12, 10, 8)),
+}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request, oracle, pkg):
+    spec = CASES[request.param]
+    log(f"[case] {request.param} building")
+    if spec["kind"] == "holey":
+        vox, bbox = holey_brick(spec["seed"], spec["shape"])
+        m = oracle.build(vox, bbox, num_levels=spec["L"], omega=0.5)
+        fixed, loads = bc_bottom_top(m, spec["shape"][2])
+        omega = 0.5
+    else:
+        ny, nx, nz = spec["shape"]
+        p = pkg.workloads.ellipsoid_problem(nx, ny, nz, spec["L"], seed=spec["seed"], n_ell=4,
+                                            n_loads=40)
+        vox, bbox, fixed, loads, omega = p.vox, p.bbox, p.fixed, p.loads, 0.4
+        m = oracle.build(vox, bbox, num_levels=spec["L"], omega=omega)
+    oracle.apply_bc(m, loads, fixed)
+    oracle.factor_coarsest(m)
+    s = pkg.Solver(vox, bbox, num_levels=spec["L"], jacobi_weight=omega, tol=1e-8, max_it=2000)
+    s.set_bc(fixed, loads)
+    log(f"[case] {request.param} ready: L={m.numLevels} dofs={m.levels[0].numDOFs}")
+    yield dict(name=request.param, m=m, s=s, vox=vox, bbox=bbox, fixed=fixed, loads=loads,
+               omega=omega)
+    s.close()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_index_tables_bit_exact(case):
+    m, s = case["m"], case["s"]
+    assert s.num_levels == m.numLevels
+    for li, lev in enumerate(m.levels):
+        d = s.level_arrays(li + 1)
+        log(f"  level {li + 1}: ne={lev.numElements} nn={lev.numNodes}")
+        assert d["numElements"] == lev.numElements and d["numNodes"] == lev.numNodes
+        assert np.array_equal(d["eleMapBack"], lev.eleMapBack)
+        assert np.array_equal(d["eleMapForward"], lev.eleMapForward)
+        assert np.array_equal(d["nodMapBack"], lev.nodMapBack)
+        assert np.array_equal(d["nodMapForward"], lev.nodMapForward)
+        assert np.array_equal(d["eNodMat"], lev.eNodMat)
+        assert np.array_equal(d["numNod2Eles"], lev.numNod2Eles)
+        assert np.array_equal(d["nodesOnBoundary"], lev.nodesOnBoundary)
+        assert np.array_equal(d["elementsOnBoundary"], lev.elementsOnBoundary)
+        if li > 0:
+            assert np.array_equal(d["elementUpwardMap"], lev.elementUpwardMap)
+            assert np.array_equal(d["transferMat"], lev.transferMat)
+            assert np.array_equal(d["transferMatCoeffi"], lev.transferMatCoeffi)
+            assert np.array_equal(d["solidNodeMapCoarser2Finer"], lev.solidNodeMapCoarser2Finer)
+            assert d["intermediateNumNodes"] == lev.intermediateNumNodes
+
+
+def test_element_matrices_and_states(case):
+    m, s = case["m"], case["s"]
+    assert rel(s.query("KE0").reshape(24, 24), m.Ke0) < 1e-14
+    for li in range(1, m.numLevels):
+        lev = m.levels[li]
+        assert s.query("STORING_STATE", li + 1) == lev.storingState
+        Ks = s.query("KS", li + 1).reshape(-1, 24, 24)
+        if lev.storingState == 2:
+            assert np.array_equal(s.query("UNIQUE_ELEMENTS", li + 1), lev.uniqueElements)
+            st = np.zeros(lev.numElements, dtype=np.int64)
+            st[lev.uniqueElements - 1] = np.arange(1, lev.uniqueElements.size + 1)
+            ref = lev.Ks[st]
+        else:
+            ref = lev.Ks
+        log(f"  level {li + 1} state {lev.storingState}: Ks rel {rel(Ks, ref):.2e}")
+        assert rel(Ks, ref) < 1e-12
+
+
+def test_diagonal_and_fixed_dofs(case):
+    m, s = case["m"], case["s"]
+    for li in range(m.numLevels):
+        lev = m.levels[li]
+        fx = s.query("FIXED_DOFS", li + 1)
+        assert np.array_equal(fx, np.unique(lev.fixedDOFs))
+        assert np.array_equal(s.query("FREE_DOFS", li + 1), lev.freeDOFs)
+        if li < m.numLevels - 1:
+            assert rel(s.query("DIAGK", li + 1), lev.diagK) < 1e-13
+    assert np.array_equal(s.query("F"), m.F)
+
+
+def test_operators(case, oracle):
+    m, s = case["m"], case["s"]
+    rng = np.random.default_rng(1)
+    for li in range(m.numLevels):
+        lev = m.levels[li]
+        x = rng.standard_normal(lev.numDOFs)
+        y = s.atx(x, li + 1)
+        ref = oracle.atx(m, x, li + 1)
+        log(f"  AtX level {li + 1}: rel {rel(y, ref):.2e}")
+        assert rel(y, ref) < 1e-12
+        if li < m.numLevels - 1:
+            dx, r2 = s.smooth(x, li + 1)
+            xr, rr = oracle.jacobi(m, 0.0, x, li + 1)
+            assert rel(dx, xr) < 1e-13 and rel(r2, rr) < 1e-12
+        if li >= 1:
+            fine = m.levels[li - 1]
+            rf = rng.standard_normal(fine.numDOFs)
+            assert rel(s.restrict(rf, li + 1), oracle.restrict(m, rf, li + 1)) < 1e-13
+            assert rel(s.interpolate(x, li + 1), oracle.interpolate(m, x, li + 1)) < 1e-13
+
+
+def test_vcycle(case, oracle):
+    m, s = case["m"], case["s"]
+    r = np.random.default_rng(2).standard_normal(m.levels[0].numDOFs)
+    z = s.vcycle(r)
+    zr = oracle.vcycle(m, r)
+    log(f"  vcycle rel {rel(z, zr):.2e}")
+    assert rel(z, zr) < 1e-9
+
+
+def test_solve_fp64_matches_oracle(case, oracle):
+    m, s = case["m"], case["s"]
+    tol = 1e-8
+    res = oracle.cg_solver(m, m.F, tol, 2000)
+    U, info, hist = s.solve(raise_on_nonconvergence=False)
+    e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
+    log(f"  solve: gpu its {info.its} oracle its {res.its} relres {info.relres:.2e} "
+        f"true {info.true_relres:.2e} err {e:.2e} t {info.solve_seconds:.4f}s")
+    assert abs(info.its - res.its) <= 1
+    assert e < 1e-6
+    assert hist.shape[0] == info.its - (1 if info.converged else 0)
+    if res.hist.shape[0] and hist.shape[0]:
+        k = min(res.hist.shape[0], hist.shape[0])
+        assert np.allclose(hist[:k, 2], res.hist[:k, 2], rtol=1e-6)
+        assert np.array_equal(hist[:, 0], np.arange(1, hist.shape[0] + 1))
+
+
+def test_stress(case, oracle):
+    m, s = case["m"], case["s"]
+    U = np.random.default_rng(3).standard_normal(m.levels[0].numDOFs)
+    S = s.stress(U)
+    ref = oracle.cartesian_stress(m, U)
+    assert rel(S, ref) < 1e-12
+
+
+def test_mixed_precision_solve(case, oracle, pkg):
+    m = case["m"]
+    s2 = pkg.Solver(case["vox"], case["bbox"], num_levels=m.numLevels, jacobi_weight=case["omega"],
+                    tol=1e-8, max_it=2000, precision=pkg.GMGS_PREC_MIXED)
+    s2.set_bc(case["fixed"], case["loads"])
+    U, info, _ = s2.solve(raise_on_nonconvergence=False)
+    res = oracle.cg_solver(m, m.F, 1e-12, 4000)
+    e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
+    log(f"  mixed: its {info.its} relres {info.relres:.2e} true {info.true_relres:.2e} err {e:.2e}")
+    assert info.converged and info.true_relres < 1e-7
+    assert e < 1e-5
+    s2.close()
