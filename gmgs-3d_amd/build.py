@@ -15,7 +15,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libgmgs.so"
-SOURCES = ["fe_host.cpp", "mesh.hip", "ops.hip", "setup.hip", "dense.hip", "capi.cpp"]
+SOURCES = ["fe_host.cpp", "mesh.hip", "ops.hip", "ops_zm.hip", "setup.hip", "dense.hip",
+           "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GMGS_OFFLOAD_ARCH", "gfx950")

@@ -14,6 +14,7 @@
 #include "fe_host.h"
 #include "mesh.h"
 #include "ops.h"
+#include "ops_zm.h"
 #include "setup.h"
 
 using namespace gmgs;
@@ -39,6 +40,11 @@ struct Level {
   long long nuniq = 0;
   DevBuf r, rp, x;               // V-cycle work vectors (precision of the cycle)
   double scale = 1.0;            // Ke_l = Ke0 * scale
+  // z-marching operator (levels with a universal element matrix)
+  bool zm = false;
+  DevBuf coltiles;
+  long long ncol = 0;
+  int col_tx = 8, col_zc = 16;
 };
 
 }  // namespace
@@ -58,6 +64,7 @@ struct gmgs_ctx {
   bool flexible = false;
   bool exact = false;
   bool tables = true;
+  bool simple_kernels = false;
   // SIMP
   bool simp = false;
   DevBuf emod64, emod32;
@@ -159,6 +166,22 @@ LevelOp<T> make_op(gmgs_ctx* c, int li) {
     op.emod = (li == 0 && c->simp) ? c->emod32.as<float>() : nullptr;
   }
   return op;
+}
+
+// y = A_l u (or the fused smoother) on level li; returns the number of dot partials
+template <typename T, typename TR>
+int apply_level(gmgs_ctx* c, int li, int mode, const TR* in, T omega, T* out, TR* xacc,
+                double* partial, const int* done) {
+  Level& l = c->lev[li];
+  LevelOp<T> op = make_op<T>(c, li);
+  if (l.zm) {
+    launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc, mode, in, omega,
+                           out, xacc, partial, done, c->stream);
+    return (int)l.ncol;
+  }
+  launch_apply<T, TR>(op, l.tiles.as<TileRec>(), l.ntiles, mode, in, omega, out, xacc, partial, done,
+                      c->stream);
+  return (int)apply_blocks(l.ntiles);
 }
 
 template <typename T>
@@ -385,6 +408,10 @@ void build_operators(gmgs_ctx* c) {
       if (!l.Ks64.empty()) to_f32(l.Ks64, l.Ks32, (long long)(l.Ks64.bytes() / sizeof(double)), s);
       if (!l.invd64.empty()) to_f32(l.invd64, l.invd32, 3 * l.mesh.g.N, s);
     }
+    // z-marching operator where a universal element matrix exists
+    l.zm = !c->simple_kernels && (li == 0 ? !c->simp : l.state == 2);
+    if (l.zm && l.coltiles.empty())
+      l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
   }
   GMGS_HIP(hipStreamSynchronize(s));
 }
@@ -415,7 +442,8 @@ void alloc_vectors(gmgs_ctx* c) {
   c->z[1].alloc_zero(es * n3, s);
   c->b.alloc_zero(sizeof(double) * n3, s);
   c->nparts = apply_blocks(L1(c).ntiles);
-  c->part.alloc_zero(sizeof(double) * 2 * (c->nparts + 1), s);
+  const long long np_max = std::max<long long>(c->nparts, L1(c).ncol);
+  c->part.alloc_zero(sizeof(double) * 2 * (np_max + 1), s);
   c->sc.alloc_zero(sizeof(PcgScalars), s);
   const long long hcap = (long long)c->prm.max_it + 2;
   c->hist_rel.alloc_zero(sizeof(double) * hcap, s);
@@ -436,11 +464,9 @@ void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partia
   // down: level 1
   {
     Level& l = c->lev[0];
-    LevelOp<T> op = make_op<T>(c, 0);
     {
       KTime t(c, 2);
-      launch_apply<T, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, rin, omega,
-                              vec<T>(l.rp), nullptr, nullptr, done, s);
+      apply_level<T, double>(c, 0, AP_SMOOTH, rin, omega, vec<T>(l.rp), nullptr, nullptr, done);
     }
     Level& n2 = c->lev[1];
     KTime t(c, 3);
@@ -449,9 +475,7 @@ void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partia
   }
   for (int li = 1; li < L - 1; ++li) {
     Level& l = c->lev[li];
-    LevelOp<T> op = make_op<T>(c, li);
-    launch_apply<T, T>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, vec<T>(l.r), omega,
-                       vec<T>(l.rp), nullptr, nullptr, done, s);
+    apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.r), omega, vec<T>(l.rp), nullptr, nullptr, done);
     Level& nx = c->lev[li + 1];
     launch_restrict<T>(nx.mesh.g, nx.mesh.flags.as<uint32_t>(), nx.tiles.as<TileRec>(), nx.ntiles,
                        l.mesh.g, vec<T>(l.rp), vec<T>(nx.r), done, s);
@@ -496,14 +520,13 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   const int* done = &sc->done;
   double* part = c->part.as<double>();
   const int np = (int)c->nparts;
-  LevelOp<double> op = make_op<double>(c, 0);
   c->cur_iter = k;
   KTime whole(c, 5);
   // Jacobi step on (x, r) (CG_solver.m:18-19)
   {
     KTime t(c, 1);
-    launch_apply<double, double>(op, tl, l.ntiles, AP_SMOOTH, c->r.as<double>(), c->prm.jacobi_weight,
-                                 c->ra.as<double>(), c->x.as<double>(), nullptr, done, s);
+    apply_level<double, double>(c, 0, AP_SMOOTH, c->r.as<double>(), c->prm.jacobi_weight,
+                                c->ra.as<double>(), c->x.as<double>(), nullptr, done);
   }
   // z = V(r) with r'z and r'z_prev partials (CG_solver.m:21)
   T* z = c->z[k & 1].as<T>();
@@ -513,12 +536,13 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   // p = z + beta p (CG_solver.m:23-28)
   launch_update_p<T>(g, fl, tl, l.ntiles, z, c->p.as<double>(), sc, done, s);
   // q = A p, p'q (CG_solver.m:29-31)
+  int npq;
   {
     KTime t(c, 0);
-    launch_apply<double, double>(op, tl, l.ntiles, AP_AX, c->p.as<double>(), 0.0, c->q.as<double>(),
-                                 nullptr, part, done, s);
+    npq = apply_level<double, double>(c, 0, AP_AX, c->p.as<double>(), 0.0, c->q.as<double>(), nullptr,
+                                      part, done);
   }
-  launch_finalize(2, part, np, sc, nullptr, nullptr, done, s);
+  launch_finalize(2, part, npq, sc, nullptr, nullptr, done, s);
   // x += alpha p; r = r - alpha q; ||r|| (CG_solver.m:32-39)
   launch_update_xr(g, fl, tl, l.ntiles, c->x.as<double>(), c->p.as<double>(), c->ra.as<double>(),
                    c->q.as<double>(), c->r.as<double>(), sc, part, done, s);
@@ -764,6 +788,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->flexible = c->mixed ? !(p->flags & GMGS_FLAG_FR_BETA) : (p->flags & GMGS_FLAG_PR_BETA) != 0;
     c->exact = (p->flags & GMGS_FLAG_EXACT_GALERKIN) != 0;
     c->tables = (p->flags & GMGS_FLAG_NO_INDEX_TABLES) == 0;
+    c->simple_kernels = (p->flags & GMGS_FLAG_SIMPLE_KERNELS) != 0;
     c->nely = (int)ny;
     c->nelx = (int)nx;
     c->nelz = (int)nz;
@@ -1015,9 +1040,8 @@ int gmgs_atx(gmgs_ctx* c, int level, const double* x, double* y) {
     din.alloc_zero(sizeof(double) * 3 * N, s);
     dout.alloc_zero(sizeof(double) * 3 * N, s);
     launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dx.as<double>(), din.as<double>(), N, s);
-    LevelOp<double> op = make_op<double>(c, li);
-    launch_apply<double, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_AX, din.as<double>(), 0.0,
-                                 dout.as<double>(), nullptr, nullptr, nullptr, s);
+    apply_level<double, double>(c, li, AP_AX, din.as<double>(), 0.0, dout.as<double>(), nullptr,
+                                nullptr, nullptr);
     comp.alloc(sizeof(double) * 3 * nn);
     launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, dout.as<double>(), N,
                                   comp.as<double>(), s);
@@ -1044,10 +1068,8 @@ int gmgs_smooth(gmgs_ctx* c, int level, const double* r, double* dx, double* r_o
     dout.alloc_zero(sizeof(double) * 3 * N, s);
     dacc.alloc_zero(sizeof(double) * 3 * N, s);
     launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dr.as<double>(), din.as<double>(), N, s);
-    LevelOp<double> op = make_op<double>(c, li);
-    launch_apply<double, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, din.as<double>(),
-                                 c->prm.jacobi_weight, dout.as<double>(), dacc.as<double>(), nullptr,
-                                 nullptr, s);
+    apply_level<double, double>(c, li, AP_SMOOTH, din.as<double>(), c->prm.jacobi_weight,
+                                dout.as<double>(), dacc.as<double>(), nullptr, nullptr);
     comp.alloc(sizeof(double) * 3 * nn);
     if (dx) {
       launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, dacc.as<double>(), N,

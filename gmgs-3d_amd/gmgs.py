@@ -27,6 +27,7 @@ FLAG_EXACT_GALERKIN = 1
 FLAG_NO_INDEX_TABLES = 2
 FLAG_FR_BETA = 4
 FLAG_PR_BETA = 8
+FLAG_SIMPLE_KERNELS = 16
 
 Q = dict(NUM_LEVELS=0, RES=1, NUM_ELEMENTS=2, NUM_NODES=3, ELE_MAP_BACK=4, ELE_MAP_FORWARD=5,
          NOD_MAP_BACK=6, NOD_MAP_FORWARD=7, ENODMAT=8, NUM_NOD2ELES=9, NODES_ON_BOUNDARY=10,

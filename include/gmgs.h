@@ -58,7 +58,9 @@ enum {
   GMGS_FLAG_EXACT_GALERKIN = 1,   /* true Galerkin coarse operators (no B3/B12 quirks) */
   GMGS_FLAG_NO_INDEX_TABLES = 2,  /* skip eNodMat/transferMat/... read-back tables */
   GMGS_FLAG_FR_BETA = 4,          /* Fletcher-Reeves beta also in mixed precision */
-  GMGS_FLAG_PR_BETA = 8           /* Polak-Ribiere (flexible) beta also in FP64 */
+  GMGS_FLAG_PR_BETA = 8,          /* Polak-Ribiere (flexible) beta also in FP64 */
+  GMGS_FLAG_SIMPLE_KERNELS = 16   /* one-wave-per-tile element-sum operator everywhere
+                                     (no z-marching stencil kernel); for cross-checks */
 };
 
 typedef struct gmgs_params {

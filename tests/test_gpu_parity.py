@@ -182,15 +182,62 @@ def test_stress(case, oracle):
     assert rel(S, ref) < 1e-12
 
 
+def direct_solution(m, oracle):
+    import scipy.sparse.linalg as spla
+    K = oracle.assemble_level1(m)
+    fr = m.levels[0].freeDOFs - 1
+    u = np.zeros(m.levels[0].numDOFs)
+    u[fr] = spla.spsolve(K[fr][:, fr].tocsc(), m.F[fr])
+    return u
+
+
 def test_mixed_precision_solve(case, oracle, pkg):
     m = case["m"]
     s2 = pkg.Solver(case["vox"], case["bbox"], num_levels=m.numLevels, jacobi_weight=case["omega"],
                     tol=1e-8, max_it=2000, precision=pkg.GMGS_PREC_MIXED)
     s2.set_bc(case["fixed"], case["loads"])
     U, info, _ = s2.solve(raise_on_nonconvergence=False)
-    res = oracle.cg_solver(m, m.F, 1e-12, 4000)
-    e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
+    ref = direct_solution(m, oracle)
+    e = np.linalg.norm(U - ref) / np.linalg.norm(ref)
     log(f"  mixed: its {info.its} relres {info.relres:.2e} true {info.true_relres:.2e} err {e:.2e}")
     assert info.converged and info.true_relres < 1e-7
     assert e < 1e-5
     s2.close()
+
+
+def test_simple_kernels_match_oracle(case, oracle, pkg):
+    """The one-wave-per-tile element-sum operator (GMGS_FLAG_SIMPLE_KERNELS)."""
+    m = case["m"]
+    s2 = pkg.Solver(case["vox"], case["bbox"], num_levels=m.numLevels, jacobi_weight=case["omega"],
+                    tol=1e-8, max_it=2000, flags=pkg.FLAG_SIMPLE_KERNELS)
+    s2.set_bc(case["fixed"], case["loads"])
+    rng = np.random.default_rng(9)
+    for li in range(m.numLevels):
+        x = rng.standard_normal(m.levels[li].numDOFs)
+        assert rel(s2.atx(x, li + 1), oracle.atx(m, x, li + 1)) < 1e-12
+    r = rng.standard_normal(m.levels[0].numDOFs)
+    assert rel(s2.vcycle(r), oracle.vcycle(m, r)) < 1e-9
+    s2.close()
+
+
+def test_c1_box_reference_config(oracle, pkg):
+    """40x40x64 box, 4 levels, omega 0.6, rtol 1e-3, FP64."""
+    p = pkg.workloads.box_c1()
+    s = pkg.Solver(p.vox, p.bbox, coarsest_resolution_control=p.coarsest_control,
+                   jacobi_weight=p.omega, tol=p.tol, max_it=p.max_it)
+    s.set_bc(p.fixed, p.loads)
+    U, info, hist = s.solve()
+    m = oracle.build(p.vox, p.bbox, control=p.coarsest_control, omega=p.omega)
+    oracle.apply_bc(m, p.loads, p.fixed)
+    oracle.factor_coarsest(m)
+    res = oracle.cg_solver(m, m.F, p.tol, p.max_it)
+    e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
+    log(f"  C1: dofs {U.size} its gpu {info.its} oracle {res.its} err {e:.2e} "
+        f"t {info.solve_seconds * 1e3:.2f} ms")
+    assert U.size == 327795 and s.num_levels == 4
+    assert abs(info.its - res.its) <= 1 and e < 1e-6
+    k = min(hist.shape[0], res.hist.shape[0])
+    assert np.allclose(hist[:k, 2], res.hist[:k, 2], rtol=1e-6)
+    S = s.stress(U)
+    assert rel(S, oracle.cartesian_stress(m, U)) < 1e-12
+    s.close()
