@@ -5,7 +5,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -65,6 +68,7 @@ struct gmgs_ctx {
   bool exact = false;
   bool tables = true;
   bool simple_kernels = false;
+  bool autotune = false;
   // SIMP
   bool simp = false;
   DevBuf emod64, emod32;
@@ -91,6 +95,11 @@ struct gmgs_ctx {
   std::vector<hipEvent_t> kpool;
   size_t kpool_used = 0;
   double ktot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // captured PCG iteration pair (even, odd) and pinned polling buffers
+  bool use_graphs = true;
+  hipGraphExec_t gexec = nullptr;
+  PcgScalars* pinned = nullptr;
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   long long kcnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int cur_iter = 0;
 };
@@ -416,6 +425,93 @@ void build_operators(gmgs_ctx* c) {
   GMGS_HIP(hipStreamSynchronize(s));
 }
 
+// ------------------------------------------------------------------ autotuner
+// per-level choice of the z-marching column tile (TX x ZC), timed on the
+// level's own grid with the V-cycle smoother; cached per (grid, precision)
+struct TuneKey {
+  int ey, ex, ez, prec;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(ey, ex, ez, prec) < std::tie(o.ey, o.ex, o.ez, o.prec);
+  }
+};
+std::map<TuneKey, std::pair<int, int>>& tune_cache() {
+  static std::map<TuneKey, std::pair<int, int>> m;
+  return m;
+}
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+template <typename T, typename TR>
+float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, int zc, const TR* in,
+              T* out) {
+  Level& l = c->lev[li];
+  hipStream_t s = c->stream;
+  LevelOp<T> op = make_op<T>(c, li);
+  hipEvent_t a, b;
+  GMGS_HIP(hipEventCreate(&a));
+  GMGS_HIP(hipEventCreate(&b));
+  for (int w = 0; w < 2; ++w)
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr, nullptr, s);
+  GMGS_HIP(hipEventRecord(a, s));
+  const int reps = 5;
+  for (int w = 0; w < reps; ++w)
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr, nullptr, s);
+  GMGS_HIP(hipEventRecord(b, s));
+  GMGS_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  GMGS_HIP(hipEventElapsedTime(&ms, a, b));
+  GMGS_HIP(hipEventDestroy(a));
+  GMGS_HIP(hipEventDestroy(b));
+  (void)l;
+  return ms / reps;
+}
+
+void autotune_levels(gmgs_ctx* c) {
+  static const int cand[][2] = {{4, 16}, {8, 16}, {16, 16}, {8, 8}, {8, 32}, {16, 8}, {4, 32}};
+  hipStream_t s = c->stream;
+  for (int li = 0; li < c->L - 1; ++li) {
+    Level& l = c->lev[li];
+    if (!l.zm) continue;
+    const TuneKey key{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, c->mixed ? 1 : 0};
+    std::pair<int, int> best{l.col_tx, l.col_zc};
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      auto it = tune_cache().find(key);
+      if (it != tune_cache().end()) { best = it->second; cached = true; }
+    }
+    if (!cached) {
+      float best_ms = 1e30f;
+      for (const auto& cd : cand) {
+        DevBuf tl;
+        const long long nt = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
+        float ms;
+        if (li == 0) {
+          if (c->mixed) ms = time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<float>());
+          else ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<double>());
+        } else {
+          if (c->mixed) ms = time_zm<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(), l.rp.as<float>());
+          else ms = time_zm<double, double>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<double>(), l.rp.as<double>());
+        }
+        if (ms < best_ms) { best_ms = ms; best = {cd[0], cd[1]}; }
+      }
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      tune_cache()[key] = best;
+    }
+    if (best.first != l.col_tx || best.second != l.col_zc) {
+      l.col_tx = best.first;
+      l.col_zc = best.second;
+      l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
+    }
+  }
+  const long long np_max = std::max<long long>(c->nparts, L1(c).ncol);
+  if ((size_t)(sizeof(double) * 2 * (np_max + 1)) > c->part.bytes())
+    c->part.alloc_zero(sizeof(double) * 2 * (np_max + 1), s);
+  GMGS_HIP(hipStreamSynchronize(s));
+}
+
 void alloc_vectors(gmgs_ctx* c) {
   hipStream_t s = c->stream;
   const size_t es = c->mixed ? sizeof(float) : sizeof(double);
@@ -582,29 +678,73 @@ struct LoopResult {
   double relres = 0, seconds = 0;
 };
 
+void invalidate_graph(gmgs_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+}
+
+// capture iterations k = 0, 1 once; their kernels read every scalar from
+// device memory, so the pair replays for every later (even, odd) pair
+void capture_iteration_pair(gmgs_ctx* c) {
+  hipGraph_t g = nullptr;
+  GMGS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  for (int k = 0; k < 2; ++k) {
+    if (c->mixed) pcg_iteration<float>(c, k);
+    else pcg_iteration<double>(c, k);
+  }
+  GMGS_HIP(hipStreamEndCapture(c->stream, &g));
+  GMGS_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  GMGS_HIP(hipGraphDestroy(g));
+}
+
 LoopResult pcg_run(gmgs_ctx* c) {
   hipStream_t s = c->stream;
-  PcgScalars h{};
+  if (!c->pinned) {
+    GMGS_HIP(hipHostMalloc((void**)&c->pinned, 2 * sizeof(PcgScalars), hipHostMallocDefault));
+    GMGS_HIP(hipEventCreateWithFlags(&c->poll_ev[0], hipEventDisableTiming));
+    GMGS_HIP(hipEventCreateWithFlags(&c->poll_ev[1], hipEventDisableTiming));
+  }
+  const bool graph = c->use_graphs && !c->ktime;
+  if (graph && !c->gexec) capture_iteration_pair(c);
   hipEvent_t e0, e1;
   GMGS_HIP(hipEventCreate(&e0));
   GMGS_HIP(hipEventCreate(&e1));
   GMGS_HIP(hipEventRecord(e0, s));
   const int maxk = c->prm.max_it + 1;
-  int k = 0;
-  int chunk = 1;
-  while (k < maxk) {
-    const int n = std::min(chunk, maxk - k);
-    for (int i = 0; i < n; ++i, ++k) {
-      if (c->mixed) pcg_iteration<float>(c, k);
-      else pcg_iteration<double>(c, k);
+  int k = 0, chunk = 2, slot = 0, pending = -1;
+  PcgScalars h{};
+  // keep one chunk in flight while the host reads the previous chunk's flag
+  while (true) {
+    if (k < maxk) {
+      const int n = std::min(chunk, maxk - k + (graph ? 1 : 0));
+      if (graph) {
+        for (int i = 0; i < n; i += 2, k += 2) GMGS_HIP(hipGraphLaunch(c->gexec, s));
+      } else {
+        for (int i = 0; i < n; ++i, ++k) {
+          if (c->mixed) pcg_iteration<float>(c, k);
+          else pcg_iteration<double>(c, k);
+        }
+      }
+      GMGS_HIP(hipMemcpyAsync(&c->pinned[slot], c->sc.as<void>(), sizeof(PcgScalars),
+                              hipMemcpyDeviceToHost, s));
+      GMGS_HIP(hipEventRecord(c->poll_ev[slot], s));
+      chunk = std::min(chunk * 2, 16);
     }
-    GMGS_HIP(hipMemcpyAsync(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost, s));
-    GMGS_HIP(hipStreamSynchronize(s));
-    if (h.done) break;
-    chunk = std::min(chunk * 2, 8);
+    if (pending >= 0) {
+      GMGS_HIP(hipEventSynchronize(c->poll_ev[pending]));
+      if (c->pinned[pending].done) break;
+    }
+    if (k >= maxk) {
+      if (pending == slot) break;
+      pending = slot;
+      continue;
+    }
+    pending = slot;
+    slot ^= 1;
   }
   GMGS_HIP(hipEventRecord(e1, s));
   GMGS_HIP(hipEventSynchronize(e1));
+  GMGS_HIP(hipMemcpy(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost));
   float ms = 0;
   GMGS_HIP(hipEventElapsedTime(&ms, e0, e1));
   GMGS_HIP(hipEventDestroy(e0));
@@ -630,6 +770,7 @@ double dot_host(gmgs_ctx* c, const double* a, const double* b) {
 
 void set_bc_impl(gmgs_ctx* c, const int32_t* fixed, int64_t k, const double* loads, int64_t m) {
   hipStream_t s = c->stream;
+  invalidate_graph(c);
   Level& l = L1(c);
   const long long nn = l.mesh.nn;
   if (m <= 0 || !loads) throw Status(GMGS_ERR_NO_LOADS, "No Loads! Unable to Perform Static FEA!");
@@ -789,6 +930,8 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->exact = (p->flags & GMGS_FLAG_EXACT_GALERKIN) != 0;
     c->tables = (p->flags & GMGS_FLAG_NO_INDEX_TABLES) == 0;
     c->simple_kernels = (p->flags & GMGS_FLAG_SIMPLE_KERNELS) != 0;
+    c->autotune = (p->flags & GMGS_FLAG_AUTOTUNE) != 0;
+    c->use_graphs = (p->flags & GMGS_FLAG_NO_GRAPHS) == 0;
     c->nely = (int)ny;
     c->nelx = (int)nx;
     c->nelz = (int)nz;
@@ -815,6 +958,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     upload(c->stressM, c->M, 48 * 24, c->stream);
     build_operators(c.get());
     alloc_vectors(c.get());
+    if (c->autotune) autotune_levels(c.get());
     return GMGS_OK;
   });
   if (st != GMGS_OK) {
@@ -859,6 +1003,7 @@ int gmgs_set_modulus(gmgs_ctx* c, const double* modulus, int64_t n) {
       if (c->mixed) to_f32(c->emod64, c->emod32, l.mesh.g.N, s);
       c->simp = true;
     }
+    invalidate_graph(c);
     build_operators(c);
     if (c->bc) {
       // the coarse operators changed: redo the BC-dependent set-up
@@ -1183,6 +1328,7 @@ int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count) {
       case GMGS_Q_INTERMEDIATE_NUM_NODES: case GMGS_Q_STORING_STATE: case GMGS_Q_NUM_TILES:
       case GMGS_Q_NUM_FREE_COARSEST:
         n = 1; break;
+      case GMGS_Q_TILE_CONFIG: n = 3; break;
       case GMGS_Q_RES: n = 3; break;
       case GMGS_Q_ELE_MAP_BACK: n = l->mesh.ne; break;
       case GMGS_Q_ELE_MAP_FORWARD: n = (int64_t)g->ey * g->ex * g->ez; break;
@@ -1241,6 +1387,8 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
       case GMGS_Q_STORING_STATE: i64[0] = level == 1 ? 0 : l->state; break;
       case GMGS_Q_NUM_TILES: i64[0] = l->ntiles; break;
       case GMGS_Q_NUM_FREE_COARSEST: i64[0] = c->nfree; break;
+      case GMGS_Q_TILE_CONFIG:
+        i64[0] = l->zm ? 1 : 0; i64[1] = l->col_tx; i64[2] = l->col_zc; break;
       case GMGS_Q_ELE_MAP_BACK: copy_dev(l->mesh.eleMapBack, 4 * need); break;
       case GMGS_Q_ELE_MAP_FORWARD: copy_dev(l->mesh.eleMapForward, 4 * need); break;
       case GMGS_Q_NOD_MAP_BACK: copy_dev(l->mesh.nodMapBack, 4 * need); break;
@@ -1324,8 +1472,12 @@ void gmgs_destroy(gmgs_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  invalidate_graph(c);
   c->lev.clear();
   for (hipEvent_t e : c->kpool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->poll_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
   hipStream_t s = c->stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);

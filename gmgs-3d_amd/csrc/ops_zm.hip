@@ -51,7 +51,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
 
   T aM[3] = {0, 0, 0}, a0[3] = {0, 0, 0}, aP[3] = {0, 0, 0};   // outputs zz-1, zz, zz+1
   uint32_t fM = 0, f0 = 0, fP = 0;
-  T ucM[3] = {0, 0, 0}, uc0[3] = {0, 0, 0};
+  T uc0[3] = {0, 0, 0};   // centre u of plane zz-1 (the output being written)
   double dsum = 0.0;
 
   for (int zz = z0 - 1; zz <= z1; ++zz) {
@@ -153,10 +153,10 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
         const T a = (fM & (1u << (F_FIX_SHIFT + c))) ? (T)0 : aM[c];
         if (MODE == AP_AX) {
           out[c * N + n] = a;
-          if (partial) dsum += (double)ucM[c] * (double)a;
+          if (partial) dsum += (double)uc0[c] * (double)a;
         } else {
           out[c * N + n] = (T)in[c * N + n] - a;
-          if (xacc) xacc[c * N + n] += (TR)ucM[c];
+          if (xacc) xacc[c * N + n] += (TR)uc0[c];
         }
       }
     }
@@ -166,7 +166,6 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
       aM[c] = a0[c];
       a0[c] = aP[c];
       aP[c] = 0;
-      ucM[c] = uc0[c];
       uc0[c] = u[1][1][c];
     }
     fM = f0;

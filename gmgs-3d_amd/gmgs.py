@@ -28,16 +28,18 @@ FLAG_NO_INDEX_TABLES = 2
 FLAG_FR_BETA = 4
 FLAG_PR_BETA = 8
 FLAG_SIMPLE_KERNELS = 16
+FLAG_AUTOTUNE = 32
+FLAG_NO_GRAPHS = 64
 
 Q = dict(NUM_LEVELS=0, RES=1, NUM_ELEMENTS=2, NUM_NODES=3, ELE_MAP_BACK=4, ELE_MAP_FORWARD=5,
          NOD_MAP_BACK=6, NOD_MAP_FORWARD=7, ENODMAT=8, NUM_NOD2ELES=9, NODES_ON_BOUNDARY=10,
          ELEMENTS_ON_BOUNDARY=11, ELEMENT_UPWARD_MAP=12, TRANSFER_MAT=13, TRANSFER_MAT_COEFFI=14,
          SOLID_NODE_MAP_C2F=15, INTERMEDIATE_NUM_NODES=16, STORING_STATE=17, UNIQUE_ELEMENTS=18,
          KS=19, DIAGK=20, FIXED_DOFS=21, FREE_DOFS=22, KE0=23, BBOX=24, F=25, NUM_TILES=26,
-         NUM_FREE_COARSEST=27)
+         NUM_FREE_COARSEST=27, TILE_CONFIG=28)
 _Q_DOUBLE = {Q["TRANSFER_MAT_COEFFI"], Q["KS"], Q["DIAGK"], Q["KE0"], Q["BBOX"], Q["F"]}
 _Q_INT64 = {Q["NUM_LEVELS"], Q["RES"], Q["NUM_ELEMENTS"], Q["NUM_NODES"], Q["INTERMEDIATE_NUM_NODES"],
-            Q["STORING_STATE"], Q["NUM_TILES"], Q["NUM_FREE_COARSEST"]}
+            Q["STORING_STATE"], Q["NUM_TILES"], Q["NUM_FREE_COARSEST"], Q["TILE_CONFIG"]}
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "insufficient resolution", 3: "fewer than two levels",
           4: "no loads", 5: "no constraints", 6: "node id out of range", 7: "non-finite input",

@@ -59,8 +59,10 @@ enum {
   GMGS_FLAG_NO_INDEX_TABLES = 2,  /* skip eNodMat/transferMat/... read-back tables */
   GMGS_FLAG_FR_BETA = 4,          /* Fletcher-Reeves beta also in mixed precision */
   GMGS_FLAG_PR_BETA = 8,          /* Polak-Ribiere (flexible) beta also in FP64 */
-  GMGS_FLAG_SIMPLE_KERNELS = 16   /* one-wave-per-tile element-sum operator everywhere
+  GMGS_FLAG_SIMPLE_KERNELS = 16,  /* one-wave-per-tile element-sum operator everywhere
                                      (no z-marching stencil kernel); for cross-checks */
+  GMGS_FLAG_AUTOTUNE = 32,        /* time column-tile shapes per level at set-up */
+  GMGS_FLAG_NO_GRAPHS = 64        /* launch PCG iterations one kernel at a time */
 };
 
 typedef struct gmgs_params {
@@ -115,7 +117,8 @@ enum {
   GMGS_Q_BBOX = 24,                 /* double[6] adjusted bounding box [xmin ymin zmin xmax ymax zmax] */
   GMGS_Q_F = 25,                    /* double[numDOFs] load vector F_ */
   GMGS_Q_NUM_TILES = 26,            /* int64 active 16x4x1 tiles of the level */
-  GMGS_Q_NUM_FREE_COARSEST = 27     /* int64 */
+  GMGS_Q_NUM_FREE_COARSEST = 27,    /* int64 */
+  GMGS_Q_TILE_CONFIG = 28           /* int64[3]: z-marching on?, tile TX, z-chunk */
 };
 
 void gmgs_default_params(gmgs_params* p);

@@ -241,3 +241,28 @@ def test_c1_box_reference_config(oracle, pkg):
     S = s.stress(U)
     assert rel(S, oracle.cartesian_stress(m, U)) < 1e-12
     s.close()
+
+
+def test_determinism_graphs_and_autotune(pkg):
+    """Graph replay, eager launches and autotuned tiles give bitwise-identical U;
+    repeated solves are bitwise reproducible."""
+    p = pkg.workloads.ellipsoid_problem(24, 20, 32, 4, seed=3, n_ell=5, n_loads=60, tol=1e-6)
+    out = {}
+    for name, flags in [("graph", 0), ("eager", pkg.FLAG_NO_GRAPHS),
+                        ("tuned", pkg.FLAG_AUTOTUNE)]:
+        for prec in (pkg.GMGS_PREC_FP64, pkg.GMGS_PREC_MIXED):
+            s = pkg.Solver(p.vox, p.bbox, num_levels=4, tol=1e-6, max_it=3000, flags=flags,
+                           precision=prec)
+            s.set_bc(p.fixed, p.loads)
+            U1, i1, _ = s.solve(raise_on_nonconvergence=False)
+            U2, i2, _ = s.solve(raise_on_nonconvergence=False)
+            assert np.array_equal(U1, U2) and i1.its == i2.its
+            out[(name, prec)] = (U1, i1.its)
+            if name == "tuned":
+                log(f"  tuned tiles {[s.query('TILE_CONFIG', l + 1).tolist() for l in range(3)]}")
+            s.close()
+    for prec in (pkg.GMGS_PREC_FP64, pkg.GMGS_PREC_MIXED):
+        assert np.array_equal(out[("graph", prec)][0], out[("eager", prec)][0])
+        assert out[("graph", prec)][1] == out[("eager", prec)][1]
+        # a different tile shape changes no sum order inside a node
+        assert np.array_equal(out[("graph", prec)][0], out[("tuned", prec)][0])
