@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-it", type=int, default=3000)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -87,6 +89,11 @@ def main():
 
     pkg = load_package()
     wl = pkg.workloads
+    if args.copy_probe_mib:
+        nb = args.copy_probe_mib << 20
+        ms = pkg.copy_probe(nb, reps=5)
+        log(f"[bench] copy probe {args.copy_probe_mib} MiB: {ms:.4f} ms "
+            f"({2 * nb / (ms * 1e-3) / 1e9:.1f} GB/s read+write)")
     t0 = time.perf_counter()
     prob = wl.ellipsoids_c2(args.seed) if args.config == "c2" else wl.ellipsoids_c3(args.seed)
     log(f"[bench] generated {prob.name} in {time.perf_counter() - t0:.1f}s")

@@ -5,6 +5,7 @@ include/gmgs.h); this package is its Python host side.
 """
 from . import workloads  # noqa: F401
 from .gmgs import (GMGS3D, GMGSError, NotConverged, Solver, default_params,  # noqa: F401
+                   batch_solve, copy_probe,
                    load_library, library_path, exported_symbols,
                    GMGS_PREC_FP64, GMGS_PREC_MIXED, FLAG_EXACT_GALERKIN, FLAG_NO_INDEX_TABLES,
                    FLAG_FR_BETA, FLAG_PR_BETA, FLAG_SIMPLE_KERNELS,

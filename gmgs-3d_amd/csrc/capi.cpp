@@ -1018,12 +1018,20 @@ int gmgs_set_modulus(gmgs_ctx* c, const double* modulus, int64_t n) {
   });
 }
 
+static int finish_solve(gmgs_ctx* c, const LoopResult& r, gmgs_solve_info* info, double* hist,
+                        int64_t hist_cap);
+
 static int solve_into(gmgs_ctx* c, gmgs_solve_info* info, double* hist, int64_t hist_cap) {
-  hipStream_t s = c->stream;
-  Level& l = L1(c);
   pcg_init(c, c->prm.tol);
   LoopResult r = pcg_run(c);
   if (c->ktime) kt_collect(c, r.its);
+  return finish_solve(c, r, info, hist, hist_cap);
+}
+
+static int finish_solve(gmgs_ctx* c, const LoopResult& r, gmgs_solve_info* info, double* hist,
+                        int64_t hist_cap) {
+  hipStream_t s = c->stream;
+  Level& l = L1(c);
   // true FP64 residual at exit: tmp = b - A x
   LevelOp<double> op = make_op<double>(c, 0);
   launch_apply<double, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_AX, c->x.as<double>(), 0.0,
@@ -1075,6 +1083,110 @@ int gmgs_solve(gmgs_ctx* c, double* U, gmgs_solve_info* info, double* hist, int6
     }
     if (st == GMGS_ERR_NOT_CONVERGED) c->err = "Exceed the maximum iterate numbers";
     return st;
+  });
+}
+
+static void copy_solution(gmgs_ctx* c, double* U) {
+  Level& l = L1(c);
+  DevBuf comp;
+  comp.alloc(sizeof(double) * 3 * l.mesh.nn);
+  launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), l.mesh.nn, c->x.as<double>(),
+                                l.mesh.g.N, comp.as<double>(), c->stream);
+  GMGS_HIP(hipMemcpyAsync(U, comp.as<void>(), sizeof(double) * 3 * l.mesh.nn, hipMemcpyDeviceToHost,
+                          c->stream));
+  GMGS_HIP(hipStreamSynchronize(c->stream));
+}
+
+int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
+  if (!cs || n <= 0) return GMGS_ERR_ARGUMENT;
+  for (int i = 0; i < n; ++i) {
+    if (!cs[i] || cs[i]->lev.empty()) return GMGS_ERR_ARGUMENT;
+    if (!cs[i]->bc) return fail(cs[i], GMGS_ERR_STATE, "boundary conditions not set");
+    if (cs[i]->device != cs[0]->device) return fail(cs[i], GMGS_ERR_ARGUMENT, "contexts on different devices");
+  }
+  gmgs_ctx* c0 = cs[0];
+  return guarded(c0, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c0->device));
+    hipStream_t s0;
+    GMGS_HIP(hipStreamCreateWithFlags(&s0, hipStreamNonBlocking));
+    std::vector<hipEvent_t> ev(n + 2);
+    for (auto& e : ev) GMGS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    DevBuf dptrs, dflags;
+    std::vector<const int*> hp(n);
+    for (int i = 0; i < n; ++i) hp[i] = &cs[i]->sc.as<PcgScalars>()->done;
+    upload(dptrs, hp.data(), (size_t)n, s0);
+    dflags.alloc_zero(sizeof(int) * n, s0);
+    int* pinned = nullptr;
+    GMGS_HIP(hipHostMalloc((void**)&pinned, sizeof(int) * n, hipHostMallocDefault));
+    int maxk = 0;
+    // initial state of every problem, then fork/join capture of one
+    // iteration pair of every problem into one graph
+    for (int i = 0; i < n; ++i) {
+      pcg_init(cs[i], cs[i]->prm.tol);
+      GMGS_HIP(hipEventRecord(ev[i], cs[i]->stream));
+      GMGS_HIP(hipStreamWaitEvent(s0, ev[i], 0));
+      maxk = std::max(maxk, cs[i]->prm.max_it + 1);
+    }
+    GMGS_HIP(hipStreamSynchronize(s0));
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    GMGS_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+    GMGS_HIP(hipEventRecord(ev[n], s0));
+    for (int i = 0; i < n; ++i) {
+      gmgs_ctx* c = cs[i];
+      GMGS_HIP(hipStreamWaitEvent(c->stream, ev[n], 0));
+      for (int k = 0; k < 2; ++k) {
+        if (c->mixed) pcg_iteration<float>(c, k);
+        else pcg_iteration<double>(c, k);
+      }
+      GMGS_HIP(hipEventRecord(ev[i], c->stream));
+      GMGS_HIP(hipStreamWaitEvent(s0, ev[i], 0));
+    }
+    launch_gather_flags(dptrs.as<const int*>(), n, dflags.as<int>(), s0);
+    GMGS_HIP(hipStreamEndCapture(s0, &g));
+    GMGS_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    GMGS_HIP(hipGraphDestroy(g));
+    hipEvent_t t0, t1;
+    GMGS_HIP(hipEventCreate(&t0));
+    GMGS_HIP(hipEventCreate(&t1));
+    GMGS_HIP(hipEventRecord(t0, s0));
+    int k = 0;
+    int chunk = 2;
+    while (k < maxk) {
+      for (int i = 0; i < chunk && k < maxk; i += 2, k += 2) GMGS_HIP(hipGraphLaunch(ge, s0));
+      GMGS_HIP(hipMemcpyAsync(pinned, dflags.as<int>(), sizeof(int) * n, hipMemcpyDeviceToHost, s0));
+      GMGS_HIP(hipEventRecord(ev[n + 1], s0));
+      GMGS_HIP(hipEventSynchronize(ev[n + 1]));
+      bool all = true;
+      for (int i = 0; i < n; ++i) all = all && pinned[i];
+      if (all) break;
+      chunk = std::min(chunk * 2, 16);
+    }
+    GMGS_HIP(hipEventRecord(t1, s0));
+    GMGS_HIP(hipEventSynchronize(t1));
+    float ms = 0;
+    GMGS_HIP(hipEventElapsedTime(&ms, t0, t1));
+    int status = GMGS_OK;
+    for (int i = 0; i < n; ++i) {
+      gmgs_ctx* c = cs[i];
+      PcgScalars h{};
+      GMGS_HIP(hipMemcpy(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost));
+      LoopResult r;
+      r.its = h.its;
+      r.converged = h.converged;
+      r.relres = h.relres;
+      r.seconds = ms * 1e-3;
+      const int st = finish_solve(c, r, infos ? &infos[i] : nullptr, nullptr, 0);
+      if (st != GMGS_OK && status == GMGS_OK) status = st;
+      if (U && U[i]) copy_solution(c, U[i]);
+    }
+    GMGS_HIP(hipGraphExecDestroy(ge));
+    GMGS_HIP(hipEventDestroy(t0));
+    GMGS_HIP(hipEventDestroy(t1));
+    for (auto& e : ev) GMGS_HIP(hipEventDestroy(e));
+    GMGS_HIP(hipHostFree(pinned));
+    GMGS_HIP(hipStreamDestroy(s0));
+    return status;
   });
 }
 
@@ -1446,6 +1558,34 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
         break;
       default: throw Status(GMGS_ERR_ARGUMENT, "unknown query key");
     }
+    return GMGS_OK;
+  });
+}
+
+int gmgs_copy_probe(int device, int64_t nbytes, int reps, double* avg_ms) {
+  if (nbytes < 16 || reps < 1 || !avg_ms) return GMGS_ERR_ARGUMENT;
+  return guarded(nullptr, [&]() -> int {
+    GMGS_HIP(hipSetDevice(device));
+    hipStream_t s;
+    GMGS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DevBuf a, b;
+    a.alloc_zero((size_t)nbytes, s);
+    b.alloc_zero((size_t)nbytes, s);
+    const long long nd = nbytes / 8;
+    launch_copy_probe(a.as<double>(), b.as<double>(), nd, s);
+    hipEvent_t e0, e1;
+    GMGS_HIP(hipEventCreate(&e0));
+    GMGS_HIP(hipEventCreate(&e1));
+    GMGS_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) launch_copy_probe(a.as<double>(), b.as<double>(), nd, s);
+    GMGS_HIP(hipEventRecord(e1, s));
+    GMGS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GMGS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    GMGS_HIP(hipEventDestroy(e0));
+    GMGS_HIP(hipEventDestroy(e1));
+    GMGS_HIP(hipStreamDestroy(s));
     return GMGS_OK;
   });
 }

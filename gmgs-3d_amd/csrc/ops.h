@@ -102,6 +102,12 @@ void launch_stress(const Grid& g, const long long* node_pidx, long long nn, cons
 void launch_element_energy(const Grid& g, const long long* elem_pidx, long long ne, const double* Ke,
                            const double* lam, const double* u, double* out, hipStream_t s);
 
+// plain device copy of n doubles (HBM counter calibration)
+void launch_copy_probe(const double* a, double* b, long long n_doubles, hipStream_t s);
+
+// out[i] = *flags[i] (per-problem 'done' flags of a batch)
+void launch_gather_flags(const int* const* flags, int n, int* out, hipStream_t s);
+
 inline unsigned apply_blocks(long long ntiles) { return (unsigned)((ntiles + 3) / 4); }
 
 }  // namespace gmgs

@@ -461,7 +461,31 @@ __global__ __launch_bounds__(256) void k_element_energy(Grid g, const long long*
   out[k] = -acc;
 }
 
+__global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ a,
+                                                    double2* __restrict__ b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+__global__ void k_gather_flags(const int* const* __restrict__ flags, int n, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = *flags[i];
+}
+
 }  // namespace
+
+void launch_copy_probe(const double* a, double* b, long long n_doubles, hipStream_t s) {
+  const long long n2 = n_doubles / 2;
+  k_copy_probe<<<(unsigned)std::min<long long>((n2 + 255) / 256, 65536), 256, 0, s>>>(
+      reinterpret_cast<const double2*>(a), reinterpret_cast<double2*>(b), n2);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_gather_flags(const int* const* flags, int n, int* out, hipStream_t s) {
+  k_gather_flags<<<(n + 255) / 256, 256, 0, s>>>(flags, n, out);
+  GMGS_LAUNCH_CHECK();
+}
 
 void launch_stress(const Grid& g, const long long* node_pidx, long long nn, const uint32_t* flags,
                    const double* M, const double* emod, const double* u, double* S, hipStream_t s) {

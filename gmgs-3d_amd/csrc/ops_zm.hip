@@ -29,6 +29,20 @@ __host__ __device__ constexpr bool stencil_nz(int c, int d, int dy, int dx, int 
        : (dy != 0 && dz != 0);
 }
 
+// Coefficients are read through a constant-address-space pointer kept opaque
+// to loop-invariant code motion: every plane re-reads them with scalar loads
+// (scalar cache) into SGPR operands instead of hoisting hundreds of values
+// into vector registers.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
+template <typename T>
+__device__ inline cptr<T> opaque(const T* p) {
+  cptr<T> q = (cptr<T>)p;
+  asm volatile("" : "+s"(q));
+  return q;
+}
+
 template <typename T, typename TR, int MODE, bool HAS_EOP, int TX>
 __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTile* __restrict__ tiles,
                                                       int zc, const TR* __restrict__ in, T omega,
@@ -36,6 +50,8 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
                                                       double* __restrict__ partial,
                                                       const int* __restrict__ done) {
   constexpr int TY = 16, NT = TY * TX, SY = TY + 2, SX = TX + 2;
+  constexpr int NSTAGE = 3 * SX * SY;
+  constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
   if (done && *done) return;
   __shared__ T sU[3][SX][SY];
   const ColTile t = tiles[blockIdx.x];
@@ -46,29 +62,48 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   const bool lane_ok = xp < g.px;
   const long long colidx = yp + sx * xp;
   const int z0 = t.z0, z1 = min(t.z0 + zc, g.pz - 1);   // outputs [z0, z1)
-  const T* __restrict__ S = op.stencil;
-  const T* __restrict__ Ke = op.Ke;
+
+  // this thread's share of a staged plane: global offset (without z) or -1
+  long long soff[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + i * NT;
+    soff[i] = -1;
+    if (e < NSTAGE) {
+      const int c = e / (SX * SY), rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
+      const int gy = t.y0 - 1 + yy, gx = t.x0 - 1 + xx;
+      if (gy >= 0 && gy < g.py && gx >= 0 && gx < g.px) soff[i] = gy + sx * gx + c * N;
+    }
+  }
+  auto fetch = [&](int zz, T* v) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      v[i] = 0;
+      if (soff[i] >= 0) {
+        const long long m = soff[i] + sz * zz;
+        if (MODE == AP_AX) v[i] = (T)in[m];
+        else v[i] = omega * op.invd[m] * (T)in[m];
+      }
+    }
+  };
 
   T aM[3] = {0, 0, 0}, a0[3] = {0, 0, 0}, aP[3] = {0, 0, 0};   // outputs zz-1, zz, zz+1
   uint32_t fM = 0, f0 = 0, fP = 0;
   T uc0[3] = {0, 0, 0};   // centre u of plane zz-1 (the output being written)
   double dsum = 0.0;
+  T stage[PER];
+  fetch(z0 - 1, stage);
 
   for (int zz = z0 - 1; zz <= z1; ++zz) {
     __syncthreads();
-    // stage plane zz (with halo) in LDS
-    for (int e = tid; e < 3 * SX * SY; e += NT) {
-      const int c = e / (SX * SY), rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
-      const int gy = t.y0 - 1 + yy, gx = t.x0 - 1 + xx;
-      T v = 0;
-      if (gy >= 0 && gy < g.py && gx >= 0 && gx < g.px) {
-        const long long m = gy + sx * gx + sz * zz + c * N;
-        if (MODE == AP_AX) v = (T)in[m];
-        else v = omega * op.invd[m] * (T)in[m];
-      }
-      sU[c][xx][yy] = v;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * NT;
+      if (e < NSTAGE) (&sU[0][0][0])[e] = stage[i];
     }
     __syncthreads();
+    if (zz < z1) fetch(zz + 1, stage);               // next plane in flight during compute
+    const cptr<T> S = opaque(op.stencil);
     // flags of the output at zz+1 (only outputs in [z0, z1) are owned)
     fP = (lane_ok && zz + 1 >= z0 && zz + 1 < z1) ? op.flags[colidx + sz * (zz + 1)] : 0u;
     T u[3][3][3];   // [dy+1][dx+1][c]
@@ -82,7 +117,6 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const int oz = 1 - k;            // k=0: output zz-1 (oz=+1), k=1: zz (0), k=2: zz+1 (-1)
-      T* acc = k == 0 ? aM : (k == 1 ? a0 : aP);
 #pragma unroll
       for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -92,7 +126,12 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
           for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int d = 0; d < 3; ++d)
-              if (stencil_nz(c, d, a - 1, b - 1, oz)) acc[c] += S[o * 9 + c * 3 + d] * u[a][b][d];
+              if (stencil_nz(c, d, a - 1, b - 1, oz)) {
+                const T v = S[o * 9 + c * 3 + d] * u[a][b][d];
+                if (k == 0) aM[c] += v;
+                else if (k == 1) a0[c] += v;
+                else aP[c] += v;
+              }
         }
     }
     // corrections for nodes that miss an element or touch a stored one
@@ -100,10 +139,11 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     for (int k = 0; k < 3; ++k) {
       const int oz = 1 - k;
       const uint32_t f = k == 0 ? fM : (k == 1 ? f0 : fP);
-      T* acc = k == 0 ? aM : (k == 1 ? a0 : aP);
       if (!(f & F_ELEM_MASK) || (f & F_FAST)) continue;
+      const cptr<T> Ke = opaque(op.Ke);
       const int zout = zz - oz;
       const long long nout = colidx + sz * zout;
+      T d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         const int pyb = p & 1, pxb = (p >> 1) & 1, pzb = p >> 2;
@@ -119,15 +159,14 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
         }
         const int ln = node_local_in(p);
         const int zo = oz + 1 - pzb;                       // local z of the plane in the element
-        T d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           if (hex_zo(m) != zo) continue;
           const int a = pyb + hex_ro(m), b = pxb + hex_xo(m);   // = dy+1, dx+1
           const T u0 = u[a][b][0], u1 = u[a][b][1], u2 = u[a][b][2];
-          const T* k0 = Ke + (3 * ln + 0) * 24 + 3 * m;
-          const T* k1 = Ke + (3 * ln + 1) * 24 + 3 * m;
-          const T* k2 = Ke + (3 * ln + 2) * 24 + 3 * m;
+          const cptr<T> k0 = Ke + (3 * ln + 0) * 24 + 3 * m;
+          const cptr<T> k1 = Ke + (3 * ln + 1) * 24 + 3 * m;
+          const cptr<T> k2 = Ke + (3 * ln + 2) * 24 + 3 * m;
           T e0 = k0[0] * u0 + k0[1] * u1 + k0[2] * u2;
           T e1 = k1[0] * u0 + k1[1] * u1 + k1[2] * u2;
           T e2 = k2[0] * u0 + k2[1] * u1 + k2[2] * u2;
@@ -141,8 +180,10 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
           }
           d0 += e0; d1 += e1; d2 += e2;
         }
-        acc[0] -= d0; acc[1] -= d1; acc[2] -= d2;
       }
+      if (k == 0) { aM[0] -= d0; aM[1] -= d1; aM[2] -= d2; }
+      else if (k == 1) { a0[0] -= d0; a0[1] -= d1; a0[2] -= d2; }
+      else { aP[0] -= d0; aP[1] -= d1; aP[2] -= d2; }
     }
     // output zz-1 is complete
     const int zo_ = zz - 1;

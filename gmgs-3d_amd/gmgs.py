@@ -105,6 +105,10 @@ def load_library() -> C.CDLL:
     lib.gmgs_interpolate.argtypes = [P, C.c_int, dp, dp]
     lib.gmgs_query_size.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_int64)]
     lib.gmgs_query.argtypes = [P, C.c_int, C.c_int, P, C.c_int64]
+    lib.gmgs_batch_solve.argtypes = [C.POINTER(P), C.c_int, C.POINTER(dp), C.POINTER(SolveInfo)]
+    lib.gmgs_batch_solve.restype = C.c_int
+    lib.gmgs_copy_probe.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    lib.gmgs_copy_probe.restype = C.c_int
     lib.gmgs_kernel_timing.argtypes = [P, C.c_int]
     lib.gmgs_kernel_timing.restype = C.c_int
     lib.gmgs_kernel_timing_read.argtypes = [P, C.c_int, C.POINTER(C.c_double),
@@ -127,7 +131,7 @@ def exported_symbols():
             "gmgs_set_modulus", "gmgs_solve", "gmgs_stress", "gmgs_adjoint", "gmgs_atx",
             "gmgs_vcycle", "gmgs_smooth", "gmgs_restrict", "gmgs_interpolate", "gmgs_query_size",
             "gmgs_query", "gmgs_last_error", "gmgs_destroy", "gmgs_kernel_timing",
-            "gmgs_kernel_timing_read"]
+            "gmgs_kernel_timing_read", "gmgs_batch_solve", "gmgs_copy_probe"]
 
 
 def _dptr(a: np.ndarray):
@@ -330,6 +334,31 @@ class Solver:
         out = np.zeros(3 * self.num_nodes(level - 1))
         self._check(self._lib.gmgs_interpolate(self._ctx, level, _dptr(x), _dptr(out)))
         return out
+
+
+def copy_probe(nbytes: int, reps: int = 10, device: int = 0) -> float:
+    """Mean duration (ms) of a plain device copy of nbytes."""
+    lib = load_library()
+    ms = C.c_double()
+    st = lib.gmgs_copy_probe(device, nbytes, reps, C.byref(ms))
+    if st != 0:
+        raise GMGSError(st, lib.gmgs_last_error(None).decode())
+    return ms.value
+
+
+def batch_solve(solvers):
+    """Solve every solver's problem together (gmgs_batch_solve); returns
+    [(U, SolveInfo)] in order."""
+    lib = load_library()
+    n = len(solvers)
+    ctxs = (C.c_void_p * n)(*[s._ctx.value for s in solvers])
+    Us = [np.zeros(3 * s.num_nodes(1)) for s in solvers]
+    ptrs = (C.POINTER(C.c_double) * n)(*[_dptr(u) for u in Us])
+    infos = (SolveInfo * n)()
+    st = lib.gmgs_batch_solve(ctxs, n, ptrs, infos)
+    if st not in (0, 9):
+        raise GMGSError(st, lib.gmgs_last_error(solvers[0]._ctx).decode())
+    return [(Us[i], infos[i]) for i in range(n)]
 
 
 class GMGS3D:

@@ -134,6 +134,13 @@ int gmgs_stress(gmgs_ctx* c, const double* U, double* S);
 int gmgs_adjoint(gmgs_ctx* c, const double* g, const double* U, double* lambda, double* dJdE,
                  gmgs_solve_info* info);
 
+/* Batched solve (no reference counterpart): solves the problems of n contexts
+ * (same device, BCs set) together.  One captured graph holds a PCG iteration
+ * pair of every problem as parallel branches and is replayed until every
+ * problem's device-side convergence flag is set; U[i] (may be NULL) and
+ * infos[i] receive each problem's result. */
+int gmgs_batch_solve(gmgs_ctx** ctxs, int n, double** U, gmgs_solve_info* infos);
+
 int gmgs_atx(gmgs_ctx* c, int level, const double* x, double* y);
 int gmgs_vcycle(gmgs_ctx* c, const double* r, double* z);
 int gmgs_smooth(gmgs_ctx* c, int level, const double* r, double* dx, double* r_out);
@@ -152,6 +159,9 @@ enum {
   GMGS_K_ITERATION = 5    /* one whole PCG iteration */
 };
 int gmgs_kernel_timing(gmgs_ctx* c, int enable);
+/* Plain device copy of nbytes (read + write), reps times; mean duration.  Used
+ * to calibrate HBM traffic counters against a kernel of known size. */
+int gmgs_copy_probe(int device, int64_t nbytes, int reps, double* avg_ms);
 int gmgs_kernel_timing_read(gmgs_ctx* c, int kernel, double* avg_ms, int64_t* count);
 
 int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count);
