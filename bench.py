@@ -98,7 +98,7 @@ def main():
     prob = wl.ellipsoids_c2(args.seed) if args.config == "c2" else wl.ellipsoids_c3(args.seed)
     log(f"[bench] generated {prob.name} in {time.perf_counter() - t0:.1f}s")
     common = dict(num_levels=prob.num_levels, jacobi_weight=prob.omega, tol=1e-6,
-                  max_it=args.max_it, flags=pkg.FLAG_NO_INDEX_TABLES)
+                  max_it=args.max_it, flags=pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE)
 
     t0 = time.perf_counter()
     s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, **common)

@@ -38,6 +38,8 @@ struct Level {
   DevBuf Ks64, Ks32;             // stored element matrices
   long long nKs = 0;
   DevBuf Ke64, Ke32, S64, S32;   // universal matrix and interior stencil of this level
+  DevBuf C64, C32;               // class coefficients of the stencil
+  bool classes_ok = false;
   DevBuf invd64, invd32;
   DevBuf uniq;                   // int32 compact ids of unique elements (state 2)
   long long nuniq = 0;
@@ -165,12 +167,14 @@ LevelOp<T> make_op(gmgs_ctx* c, int li) {
     op.Ks = l.Ks64.empty() ? nullptr : l.Ks64.as<double>();
     op.Ke = l.Ke64.as<double>();
     op.stencil = l.S64.as<double>();
+    op.cls = l.C64.empty() ? nullptr : l.C64.as<double>();
     op.invd = l.invd64.empty() ? nullptr : l.invd64.as<double>();
     op.emod = (li == 0 && c->simp) ? c->emod64.as<double>() : nullptr;
   } else {
     op.Ks = l.Ks32.empty() ? nullptr : l.Ks32.as<float>();
     op.Ke = l.Ke32.as<float>();
     op.stencil = l.S32.as<float>();
+    op.cls = l.C32.empty() ? nullptr : l.C32.as<float>();
     op.invd = l.invd32.empty() ? nullptr : l.invd32.as<float>();
     op.emod = (li == 0 && c->simp) ? c->emod32.as<float>() : nullptr;
   }
@@ -295,14 +299,19 @@ void upload_level_constants(gmgs_ctx* c, int li) {
   double St[27][3][3];
   interior_stencil(K.data(), St);
   std::memcpy(S.data(), St, sizeof(double) * 243);
+  double cls[30];
+  l.classes_ok = stencil_classes(St, cls, 1e-12);
   upload(l.Ke64, K.data(), 576, s);
   upload(l.S64, S.data(), 243, s);
+  upload(l.C64, cls, 30, s);
   if (c->mixed) {
-    std::vector<float> Kf(576), Sf(243);
+    std::vector<float> Kf(576), Sf(243), Cf(30);
     for (int i = 0; i < 576; ++i) Kf[i] = (float)K[i];
     for (int i = 0; i < 243; ++i) Sf[i] = (float)S[i];
+    for (int i = 0; i < 30; ++i) Cf[i] = (float)cls[i];
     upload(l.Ke32, Kf.data(), 576, s);
     upload(l.S32, Sf.data(), 243, s);
+    upload(l.C32, Cf.data(), 30, s);
   }
 }
 
@@ -418,7 +427,7 @@ void build_operators(gmgs_ctx* c) {
       if (!l.invd64.empty()) to_f32(l.invd64, l.invd32, 3 * l.mesh.g.N, s);
     }
     // z-marching operator where a universal element matrix exists
-    l.zm = !c->simple_kernels && (li == 0 ? !c->simp : l.state == 2);
+    l.zm = !c->simple_kernels && l.classes_ok && (li == 0 ? !c->simp : l.state == 2);
     if (l.zm && l.coltiles.empty())
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
   }
@@ -469,7 +478,8 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
 }
 
 void autotune_levels(gmgs_ctx* c) {
-  static const int cand[][2] = {{4, 16}, {8, 16}, {16, 16}, {8, 8}, {8, 32}, {16, 8}, {4, 32}};
+  static const int cand[][2] = {{8, 16}, {4, 16}, {16, 16}, {8, 8}, {4, 8}, {16, 8}, {8, 4},
+                                {4, 4}, {16, 4}, {8, 32}, {4, 32}};
   hipStream_t s = c->stream;
   for (int li = 0; li < c->L - 1; ++li) {
     Level& l = c->lev[li];

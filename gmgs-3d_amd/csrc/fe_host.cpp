@@ -223,4 +223,40 @@ void interior_stencil(const double* Ke, double S[27][3][3]) {
   }
 }
 
+bool stencil_classes(const double S[27][3][3], double cls[30], double tol) {
+  auto at = [&](int dy, int dx, int dz, int c, int d) {
+    return S[(dy + 1) + 3 * (dx + 1) + 9 * (dz + 1)][c][d];
+  };
+  for (int c = 0; c < 3; ++c)
+    for (int q = 0; q < 8; ++q) cls[8 * c + q] = at(q & 1, (q >> 1) & 1, q >> 2, c, c);
+  for (int q = 0; q < 2; ++q) {
+    cls[24 + q] = at(1, 1, q, 0, 1);   // x-y, |dz| = q
+    cls[26 + q] = at(q, 1, 1, 0, 2);   // x-z, |dy| = q
+    cls[28 + q] = at(1, q, 1, 1, 2);   // y-z, |dx| = q
+  }
+  double mx = 0.0;
+  for (int o = 0; o < 27; ++o)
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) mx = std::max(mx, std::fabs(S[o][c][d]));
+  auto sg = [](int v) { return v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0); };
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int off[3] = {dx, dy, dz};   // axis of component c: x <-> dx, y <-> dy, z <-> dz
+        for (int c = 0; c < 3; ++c)
+          for (int d = 0; d < 3; ++d) {
+            double model;
+            if (c == d) {
+              model = cls[8 * c + std::abs(dy) + 2 * std::abs(dx) + 4 * std::abs(dz)];
+            } else {
+              const int e = 3 - c - d;
+              const int pair = (c + d == 1) ? 24 : (c + d == 2) ? 26 : 28;
+              model = sg(off[c]) * sg(off[d]) * cls[pair + std::abs(off[e])];
+            }
+            if (std::fabs(model - at(dy, dx, dz, c, d)) > tol * mx) return false;
+          }
+      }
+  return true;
+}
+
 }  // namespace gmgs

@@ -25,4 +25,11 @@ void child_weights(double W[8][8][8]);
 // Interior 27-point stencil assembled from 8 copies of Ke.
 void interior_stencil(const double* Ke, double S[27][3][3]);
 
+// The 30 class coefficients of a brick stencil (reflection symmetry in each
+// axis): cls[0..23] diagonal a_c[|dy| + 2|dx| + 4|dz|] for c = x, y, z;
+// cls[24..25] G_xy[|dz|], cls[26..27] G_xz[|dy|], cls[28..29] G_yz[|dx|] with
+// S_cd(o) = sign(o_c) sign(o_d) G_cd(|o_e|).  Returns false when some entry
+// of S deviates from that model by more than tol * max|S|.
+bool stencil_classes(const double S[27][3][3], double cls[30], double tol);
+
 }  // namespace gmgs

@@ -18,6 +18,7 @@ struct LevelOp {
   const T* Ks;               // stored element matrices, 576 each, row-major
   const T* Ke;               // universal element matrix of this level (576, row-major)
   const T* stencil;          // interior 27x3x3 stencil of Ke (for F_FAST nodes)
+  const T* cls;              // its 30 class coefficients (z-marching kernel)
   const T* emod;             // per padded cell modulus factor (level-1 SIMP) or null
   const T* invd;             // inverse diagonal, SoA 3N (zero off the solid nodes)
 };

@@ -19,15 +19,6 @@ namespace gmgs {
 
 namespace {
 
-// structural non-zeros of the assembled brick stencil: components c and d
-// couple only through offsets that move along both of their axes
-// (component 0 = x <-> axis dx, 1 = y <-> dy, 2 = z <-> dz)
-__host__ __device__ constexpr bool stencil_nz(int c, int d, int dy, int dx, int dz) {
-  return c == d ? true
-       : ((c == 0 && d == 1) || (c == 1 && d == 0)) ? (dx != 0 && dy != 0)
-       : ((c == 0 && d == 2) || (c == 2 && d == 0)) ? (dx != 0 && dz != 0)
-       : (dy != 0 && dz != 0);
-}
 
 // Coefficients are read through a constant-address-space pointer kept opaque
 // to loop-invariant code motion: every plane re-reads them with scalar loads
@@ -103,36 +94,52 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     }
     __syncthreads();
     if (zz < z1) fetch(zz + 1, stage);               // next plane in flight during compute
-    const cptr<T> S = opaque(op.stencil);
     // flags of the output at zz+1 (only outputs in [z0, z1) are owned)
     fP = (lane_ok && zz + 1 >= z0 && zz + 1 < z1) ? op.flags[colidx + sz * (zz + 1)] : 0u;
-    T u[3][3][3];   // [dy+1][dx+1][c]
+    // neighbour (dy, dx) = (a-1, b-1), component c, read from the staged plane
+#define GMGS_U(a, b, c) sU[c][lx + (b)][ly + (a)]
+    // class sums of the in-plane neighbourhood (shared by the three outputs)
+    T C0[3], Cy[3], Cx[3], Cxy[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) {
+      C0[c] = GMGS_U(1, 1, c);
+      Cy[c] = GMGS_U(0, 1, c) + GMGS_U(2, 1, c);
+      Cx[c] = GMGS_U(1, 0, c) + GMGS_U(1, 2, c);
+      Cxy[c] = (GMGS_U(0, 0, c) + GMGS_U(2, 2, c)) + (GMGS_U(0, 2, c) + GMGS_U(2, 0, c));
+    }
+    const T Dxy_x = (GMGS_U(2, 2, 0) + GMGS_U(0, 0, 0)) - (GMGS_U(2, 0, 0) + GMGS_U(0, 2, 0));
+    const T Dxy_y = (GMGS_U(2, 2, 1) + GMGS_U(0, 0, 1)) - (GMGS_U(2, 0, 1) + GMGS_U(0, 2, 1));
+    const T Dx0_x = GMGS_U(1, 2, 0) - GMGS_U(1, 0, 0);
+    const T Dx1_x = (GMGS_U(0, 2, 0) - GMGS_U(0, 0, 0)) + (GMGS_U(2, 2, 0) - GMGS_U(2, 0, 0));
+    const T Dx0_z = GMGS_U(1, 2, 2) - GMGS_U(1, 0, 2);
+    const T Dx1_z = (GMGS_U(0, 2, 2) - GMGS_U(0, 0, 2)) + (GMGS_U(2, 2, 2) - GMGS_U(2, 0, 2));
+    const T Dy0_y = GMGS_U(2, 1, 1) - GMGS_U(0, 1, 1);
+    const T Dy1_y = (GMGS_U(2, 0, 1) - GMGS_U(0, 0, 1)) + (GMGS_U(2, 2, 1) - GMGS_U(0, 2, 1));
+    const T Dy0_z = GMGS_U(2, 1, 2) - GMGS_U(0, 1, 2);
+    const T Dy1_z = (GMGS_U(2, 0, 2) - GMGS_U(0, 0, 2)) + (GMGS_U(2, 2, 2) - GMGS_U(0, 2, 2));
+    {
+      const cptr<T> C = opaque(op.cls);
+      T cf[30];
 #pragma unroll
-      for (int b = 0; b < 3; ++b)
+      for (int i = 0; i < 30; ++i) cf[i] = C[i];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) u[a][b][c] = sU[c][lx + b][ly + a];
-    // stencil contributions: plane zz is at oz = zz - z_out for z_out = zz+1, zz, zz-1
+      for (int k = 0; k < 3; ++k) {
+        const int oz = 1 - k, q = oz != 0 ? 1 : 0;
+        T* acc = k == 0 ? aM : (k == 1 ? a0 : aP);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int oz = 1 - k;            // k=0: output zz-1 (oz=+1), k=1: zz (0), k=2: zz+1 (-1)
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          const int o = a + 3 * b + 9 * (oz + 1);
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-              if (stencil_nz(c, d, a - 1, b - 1, oz)) {
-                const T v = S[o * 9 + c * 3 + d] * u[a][b][d];
-                if (k == 0) aM[c] += v;
-                else if (k == 1) a0[c] += v;
-                else aP[c] += v;
-              }
+        for (int c = 0; c < 3; ++c)
+          acc[c] += cf[8 * c + 4 * q] * C0[c] + cf[8 * c + 1 + 4 * q] * Cy[c]
+                    + cf[8 * c + 2 + 4 * q] * Cx[c] + cf[8 * c + 3 + 4 * q] * Cxy[c];
+        acc[0] += cf[24 + q] * Dxy_y;
+        acc[1] += cf[24 + q] * Dxy_x;
+        if (oz != 0) {
+          const T sgn = (T)oz;
+          acc[0] += sgn * (cf[26] * Dx0_z + cf[27] * Dx1_z);
+          acc[2] += sgn * (cf[26] * Dx0_x + cf[27] * Dx1_x);
+          acc[1] += sgn * (cf[28] * Dy0_z + cf[29] * Dy1_z);
+          acc[2] += sgn * (cf[28] * Dy0_y + cf[29] * Dy1_y);
         }
+      }
     }
     // corrections for nodes that miss an element or touch a stored one
 #pragma unroll
@@ -163,7 +170,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
         for (int m = 0; m < 8; ++m) {
           if (hex_zo(m) != zo) continue;
           const int a = pyb + hex_ro(m), b = pxb + hex_xo(m);   // = dy+1, dx+1
-          const T u0 = u[a][b][0], u1 = u[a][b][1], u2 = u[a][b][2];
+          const T u0 = GMGS_U(a, b, 0), u1 = GMGS_U(a, b, 1), u2 = GMGS_U(a, b, 2);
           const cptr<T> k0 = Ke + (3 * ln + 0) * 24 + 3 * m;
           const cptr<T> k1 = Ke + (3 * ln + 1) * 24 + 3 * m;
           const cptr<T> k2 = Ke + (3 * ln + 2) * 24 + 3 * m;
@@ -207,8 +214,9 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
       aM[c] = a0[c];
       a0[c] = aP[c];
       aP[c] = 0;
-      uc0[c] = u[1][1][c];
+      uc0[c] = C0[c];
     }
+#undef GMGS_U
     fM = f0;
     f0 = fP;
   }

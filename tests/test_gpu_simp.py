@@ -26,8 +26,10 @@ def cant(pkg, oracle):
     from importlib import import_module
     ag = import_module("gmgs3d_amd.autograd")
     E = ag.simp_modulus(rho)
+    # the flexible (Polak-Ribiere) beta: Fletcher-Reeves stagnates near 1e-10
+    # on this 1e6-contrast random density field
     s = pkg.Solver(p.vox, p.bbox, num_levels=p.num_levels, tol=1e-12, max_it=5000,
-                   flags=pkg.FLAG_EXACT_GALERKIN)
+                   flags=pkg.FLAG_EXACT_GALERKIN | pkg.FLAG_PR_BETA)
     s.set_bc(p.fixed, p.loads)
     s.set_modulus(E)
     m = oracle.build(p.vox, p.bbox, num_levels=p.num_levels, omega=0.4, exact_galerkin=True)
@@ -48,7 +50,7 @@ def test_simp_operator_and_solution(cant, oracle):
     z, zr = s.vcycle(x), oracle.vcycle(m, x)
     assert np.abs(z - zr).max() < 1e-9 * np.abs(zr).max()
     U, info, _ = s.solve()
-    res = oracle.cg_solver(m, m.F, 1e-12, 5000)
+    res = oracle.cg_solver(m, m.F, 1e-12, 5000, flexible=True)
     e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
     log(f"  simp solve its gpu {info.its} oracle {res.its} err {e:.2e}")
     assert e < 1e-6 and abs(info.its - res.its) <= 1
@@ -60,7 +62,7 @@ def test_adjoint_matches_oracle_and_self_adjoint(cant, oracle):
     f = s.query("F")
     lam, dJdE, _ = s.adjoint(f, U)
     assert np.linalg.norm(lam - U) / np.linalg.norm(U) < 1e-9        # lambda = u for g = f
-    ref_u = oracle.cg_solver(m, m.F, 1e-13, 5000).x
+    ref_u = oracle.cg_solver(m, m.F, 1e-13, 5000, flexible=True).x
     ref = oracle.element_energy_gradient(m, ref_u, ref_u)
     assert np.abs(dJdE - ref).max() < 1e-6 * np.abs(ref).max()
     assert np.all(dJdE <= 1e-12 * np.abs(dJdE).max())              # compliance decreases with E
