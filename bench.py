@@ -2,11 +2,13 @@
 """Benchmark: multigrid-PCG throughput on one MI355X.
 
 One step = one full solve of the synthetic ellipsoid-union elasticity problem
-to relative residual 1e-6 with the mixed-precision solver (FP32 V-cycle inside
-an FP64 flexible PCG), inputs resident in HBM, set-up excluded.  The FP64
-V-cycle solve of the same problem runs once beside it, and the dominant
-kernel (the level-1 FP64 matrix-free matvec q = A p) is timed with HIP events
-on the solver's stream during the timed steps.
+(337x294x512 voxels, 6 levels, by default) to relative residual 1e-6 with the
+mixed-precision solver (FP32 V-cycle inside an FP64 flexible PCG): b resident
+in HBM, set-up excluded, the solution left on the device.  The FP64 V-cycle
+solve of the same problem is timed the same way beside it.  The level-1 FP64
+operators (matvec q = A p and the Jacobi step) are then timed with HIP events
+on the solver's stream over back-to-back launches, and one instrumented solve
+gives the per-kernel breakdown.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3]
 """
@@ -52,12 +54,15 @@ def read_traffic(name: str):
     return None
 
 
-def cpu_baseline(problem, its_fp64, iters):
-    """Oracle FP64 PCG (reference algorithm) on the host: set-up untimed, then
-    ``iters`` timed iterations; time to solution = per-iteration time x the
-    FP64 iteration count."""
+def cpu_baseline(seed, iters):
+    """Oracle FP64 PCG (reference algorithm) on the host, on the 169x147x256
+    ellipsoid problem of the same generator: set-up untimed, then ``iters``
+    timed iterations.  Returns (seconds per iteration per DOF, DOFs, seconds
+    per iteration)."""
     sys.path.insert(0, str(REPO))
     from oracle import gmgs_oracle as oracle
+    pkg = sys.modules["gmgs3d_amd"]
+    problem = pkg.workloads.ellipsoids_c2(seed)
     m = oracle.build(problem.vox, problem.bbox, num_levels=problem.num_levels, omega=problem.omega)
     oracle.apply_bc(m, problem.loads, problem.fixed)
     oracle.factor_coarsest(m)
@@ -65,7 +70,20 @@ def cpu_baseline(problem, its_fp64, iters):
     oracle.cg_solver(m, m.F, 1e-30, iters - 1)      # exactly `iters` iterations
     dt = (time.perf_counter() - t0) / iters
     ndofs = m.levels[0].numDOFs
-    return ndofs / (dt * its_fp64) / 1e6, dt
+    return dt / ndofs, ndofs, dt, problem.name
+
+
+def timed_solves(solver, steps, tag):
+    """``steps`` solves from b on the device to convergence (graph-replayed
+    iterations, solution left on the device); wall time per solve in ms."""
+    times, info = [], None
+    for k in range(steps):
+        t0 = time.perf_counter()
+        _, info, _ = solver.solve(raise_on_nonconvergence=False, copy=False)
+        times.append((time.perf_counter() - t0) * 1e3)
+        log(f"[bench] {tag} step {k}: its {info.its} relres {info.relres:.3e} "
+            f"true {info.true_relres:.3e} {times[-1]:.1f} ms (device {info.solve_seconds * 1e3:.1f})")
+    return float(np.mean(times)), info
 
 
 def main():
@@ -73,11 +91,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--config", default="c3", choices=["c2", "c3"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-it", type=int, default=3000)
+    ap.add_argument("--matvec-reps", type=int, default=20)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -107,46 +126,51 @@ def main():
     nn = s.num_nodes(1)
     ndofs = 3 * nn
     log(f"[bench] mixed solver set up in {setup_s:.2f}s: {ndofs} DOFs, {s.num_levels} levels")
-
-    for _ in range(args.warmup):
-        s.solve(raise_on_nonconvergence=False)
-    s.kernel_timing(True)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        U, info, _ = s.solve(raise_on_nonconvergence=False)
-        log(f"[bench] step {k}: its {info.its} relres {info.relres:.3e} "
-            f"true {info.true_relres:.3e} {info.solve_seconds * 1e3:.1f} ms")
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) / args.steps * 1e3
-    kt = s.kernel_times()
+    U = None
+    for _ in range(max(args.warmup, 1)):
+        U, _, _ = s.solve(raise_on_nonconvergence=False)
+    ms_per_step, info = timed_solves(s, args.steps, "mixed")
     value = ndofs / (ms_per_step * 1e-3) / 1e6
 
     # FP64 V-cycle (reference algorithm, Fletcher-Reeves) on the same problem
     s64 = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_FP64, **common)
     s64.set_bc(prob.fixed, prob.loads)
-    s64.solve(raise_on_nonconvergence=False)
-    t0 = time.perf_counter()
-    U64, info64, _ = s64.solve(raise_on_nonconvergence=False)
-    ms64 = (time.perf_counter() - t0) * 1e3
+    U64, _, _ = s64.solve(raise_on_nonconvergence=False)
+    ms64, info64 = timed_solves(s64, args.steps, "fp64")
     err = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
-    log(f"[bench] fp64: its {info64.its} {ms64:.1f} ms; mixed vs fp64 rel L2 {err:.2e}")
+    log(f"[bench] mixed vs fp64 rel L2 {err:.2e}; speed-up {ms64 / ms_per_step:.3f}")
 
-    # roofline of the level-1 FP64 matvec: x read once, y written once, one
-    # 32-bit flag word per node
-    mv_ms, mv_n = kt["matvec64"]
+    # the level-1 FP64 operators, back-to-back launches between one event pair
+    mv_ms = s.time_operator("matvec64", args.matvec_reps)
+    jac_ms = s.time_operator("jacobi64", args.matvec_reps)
+    # algorithmic bytes: q = A p reads p and writes q once, plus one 32-bit
+    # flag word per node; the Jacobi step reads r and 1/diag and writes r_a
+    # (its x += omega r/d rides in the update of x and r)
     alg_bytes = 2 * 8 * ndofs + 4 * nn
-    achieved = alg_bytes / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else None
-    traffic = read_traffic(f"matvec64_{args.config}")
+    jac_bytes = 3 * 8 * ndofs + 4 * nn
+    achieved = alg_bytes / (mv_ms * 1e-3) / 1e9
+    log(f"[bench] matvec64 {mv_ms:.4f} ms ({achieved:.1f} GB/s), jacobi64 {jac_ms:.4f} ms "
+        f"({jac_bytes / (jac_ms * 1e-3) / 1e9:.1f} GB/s)")
+    traffic = read_traffic(f"matvec64_{prob.name}")
+
+    # per-kernel breakdown from one instrumented solve (events around each
+    # level-1 launch, graphs off)
+    s.kernel_timing(True)
+    s.solve(raise_on_nonconvergence=False, copy=False)
+    kt = s.kernel_times()
+    s.kernel_timing(False)
 
     cpu = None
-    if not args.no_cpu_baseline and args.config == "c2":
+    if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline: oracle set-up + {args.cpu_iters} timed iterations")
-        v, dt = cpu_baseline(prob, info64.its, args.cpu_iters)
-        cpu = {"value": round(v, 5), "unit": "MDOF/s", "cores": 1, "kind": "port",
-               "sample": (f"numpy FP64 oracle (reference algorithm) on the same {prob.name} "
-                          f"problem: {args.cpu_iters} PCG iterations timed ({dt:.2f} s each), "
-                          f"scaled to the {info64.its} iterations the FP64 solve needs; "
-                          f"set-up excluded; numpy single-threaded")}
+        per_dof, sdofs, dt, sname = cpu_baseline(args.seed, args.cpu_iters)
+        t_cpu = per_dof * ndofs * info64.its
+        cpu = {"value": round(ndofs / t_cpu / 1e6, 5), "unit": "MDOF/s", "cores": 1, "kind": "port",
+               "sample": (f"numpy FP64 oracle (reference algorithm, numpy single-threaded) on "
+                          f"{sname} ({sdofs} DOFs, same generator): set-up untimed, "
+                          f"{args.cpu_iters} PCG iterations timed ({dt:.2f} s each); the per-DOF "
+                          f"iteration cost is scaled to this workload's {ndofs} DOFs and the "
+                          f"{info64.its} iterations its FP64 solve needs")}
     line = {
         "metric": "MDOF/s to rel. residual 1e-6, mixed-prec vs FP64 V-cycle; matvec HBM GB/s",
         "value": round(value, 3),
@@ -169,12 +193,16 @@ def main():
                    "speedup_vs_fp64": round(ms64 / ms_per_step, 3),
                    "mixed_vs_fp64_rel_l2": err,
                    "setup_s": round(setup_s, 3)},
-        "roofline": {"kernel": "k_apply<f64,AX> (level-1 matvec q = A p)", "bound": "hbm",
-                     "achieved": round(achieved, 1) if achieved else None,
+        "roofline": {"kernel": "k_apply_zm<f64,AX> (level-1 matvec q = A p)", "bound": "hbm",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_ms": round(mv_ms, 5), "launches": mv_n},
+                     "avg_launch_ms": round(mv_ms, 5), "launches": args.matvec_reps},
+        "roofline_jacobi64": {"kernel": "k_apply_zm<f64,SMOOTH> (level-1 Jacobi step)",
+                              "achieved": round(jac_bytes / (jac_ms * 1e-3) / 1e9, 1),
+                              "frac": round(jac_bytes / (jac_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                              "alg_bytes_per_launch": jac_bytes, "avg_launch_ms": round(jac_ms, 5)},
         "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
         "cpu_baseline": cpu,
     }

@@ -48,6 +48,7 @@ struct Level {
   // z-marching operator (levels with a universal element matrix)
   bool zm = false;
   DevBuf coltiles;
+  DevBuf colz;                   // per padded column: solid-node z range
   long long ncol = 0;
   int col_tx = 8, col_zc = 16;
 };
@@ -162,6 +163,7 @@ LevelOp<T> make_op(gmgs_ctx* c, int li) {
   LevelOp<T> op;
   op.g = l.mesh.g;
   op.flags = l.mesh.flags.as<uint32_t>();
+  op.colz = l.colz.empty() ? nullptr : l.colz.as<int>();
   op.eop = li == 0 ? nullptr : l.eop.as<int>();
   if constexpr (std::is_same<T, double>::value) {
     op.Ks = l.Ks64.empty() ? nullptr : l.Ks64.as<double>();
@@ -188,9 +190,12 @@ int apply_level(gmgs_ctx* c, int li, int mode, const TR* in, T omega, T* out, TR
   Level& l = c->lev[li];
   LevelOp<T> op = make_op<T>(c, li);
   if (l.zm) {
-    launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc, mode, in, omega,
-                           out, xacc, partial, done, c->stream);
-    return (int)l.ncol;
+    const int np = launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc,
+                                          mode, in, omega, out, partial, done, c->stream);
+    if (xacc)
+      launch_jacobi_acc<T, TR>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles,
+                               op.invd, in, omega, xacc, c->stream);
+    return np;
   }
   launch_apply<T, TR>(op, l.tiles.as<TileRec>(), l.ntiles, mode, in, omega, out, xacc, partial, done,
                       c->stream);
@@ -428,8 +433,10 @@ void build_operators(gmgs_ctx* c) {
     }
     // z-marching operator where a universal element matrix exists
     l.zm = !c->simple_kernels && l.classes_ok && (li == 0 ? !c->simp : l.state == 2);
-    if (l.zm && l.coltiles.empty())
+    if (l.zm && l.coltiles.empty()) {
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
+      build_colrange(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.colz, s);
+    }
   }
   GMGS_HIP(hipStreamSynchronize(s));
 }
@@ -462,11 +469,11 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
   GMGS_HIP(hipEventCreate(&a));
   GMGS_HIP(hipEventCreate(&b));
   for (int w = 0; w < 2; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr, nullptr, s);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr, s);
   GMGS_HIP(hipEventRecord(a, s));
   const int reps = 5;
   for (int w = 0; w < reps; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr, nullptr, s);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr, s);
   GMGS_HIP(hipEventRecord(b, s));
   GMGS_HIP(hipEventSynchronize(b));
   float ms = 0;
@@ -475,6 +482,14 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
   GMGS_HIP(hipEventDestroy(b));
   (void)l;
   return ms / reps;
+}
+
+// dot partials any level-1 launch may write: the tile kernels (one per 4
+// tiles), the pointwise kernels, or the z-marching operator (one per tile)
+long long partial_capacity(gmgs_ctx* c) {
+  const Level& l = L1(c);
+  long long n = std::max<long long>(apply_blocks(l.ntiles), c->nparts);
+  return std::max<long long>(n, l.ncol);
 }
 
 void autotune_levels(gmgs_ctx* c) {
@@ -516,7 +531,7 @@ void autotune_levels(gmgs_ctx* c) {
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
     }
   }
-  const long long np_max = std::max<long long>(c->nparts, L1(c).ncol);
+  const long long np_max = partial_capacity(c);
   if ((size_t)(sizeof(double) * 2 * (np_max + 1)) > c->part.bytes())
     c->part.alloc_zero(sizeof(double) * 2 * (np_max + 1), s);
   GMGS_HIP(hipStreamSynchronize(s));
@@ -547,8 +562,8 @@ void alloc_vectors(gmgs_ctx* c) {
   c->z[0].alloc_zero(es * n3, s);
   c->z[1].alloc_zero(es * n3, s);
   c->b.alloc_zero(sizeof(double) * n3, s);
-  c->nparts = apply_blocks(L1(c).ntiles);
-  const long long np_max = std::max<long long>(c->nparts, L1(c).ncol);
+  c->nparts = pointwise_blocks(L1(c).ntiles);
+  const long long np_max = partial_capacity(c);
   c->part.alloc_zero(sizeof(double) * 2 * (np_max + 1), s);
   c->sc.alloc_zero(sizeof(PcgScalars), s);
   const long long hcap = (long long)c->prm.max_it + 2;
@@ -628,11 +643,12 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   const int np = (int)c->nparts;
   c->cur_iter = k;
   KTime whole(c, 5);
-  // Jacobi step on (x, r) (CG_solver.m:18-19)
+  // Jacobi step on (x, r) (CG_solver.m:18-19): r_a = r - A(omega r/d) here,
+  // x += omega r/d in the update of x below (same sum, r is unchanged until then)
   {
     KTime t(c, 1);
     apply_level<double, double>(c, 0, AP_SMOOTH, c->r.as<double>(), c->prm.jacobi_weight,
-                                c->ra.as<double>(), c->x.as<double>(), nullptr, done);
+                                c->ra.as<double>(), nullptr, nullptr, done);
   }
   // z = V(r) with r'z and r'z_prev partials (CG_solver.m:21)
   T* z = c->z[k & 1].as<T>();
@@ -651,7 +667,8 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   launch_finalize(2, part, npq, sc, nullptr, nullptr, done, s);
   // x += alpha p; r = r - alpha q; ||r|| (CG_solver.m:32-39)
   launch_update_xr(g, fl, tl, l.ntiles, c->x.as<double>(), c->p.as<double>(), c->ra.as<double>(),
-                   c->q.as<double>(), c->r.as<double>(), sc, part, done, s);
+                   c->q.as<double>(), c->r.as<double>(), l.invd64.as<double>(), c->prm.jacobi_weight, sc,
+                   part, done, s);
   launch_finalize(3, part, np, sc, c->hist_rel.as<double>(), c->hist_t.as<unsigned long long>(),
                   done, s);
 }
@@ -1614,6 +1631,37 @@ int gmgs_kernel_timing_read(gmgs_ctx* c, int kernel, double* avg_ms, int64_t* co
   *count = c->kcnt[kernel];
   *avg_ms = c->kcnt[kernel] ? c->ktot[kernel] / (double)c->kcnt[kernel] : 0.0;
   return GMGS_OK;
+}
+
+int gmgs_time_operator(gmgs_ctx* c, int kernel, int reps, double* avg_ms) {
+  if (!c || c->lev.empty() || reps < 1 || !avg_ms) return GMGS_ERR_ARGUMENT;
+  if (kernel != GMGS_K_MATVEC64 && kernel != GMGS_K_JACOBI64) return GMGS_ERR_ARGUMENT;
+  if (!c->bc) return fail(c, GMGS_ERR_STATE, "boundary conditions not set");
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    auto launch = [&]() {
+      if (kernel == GMGS_K_MATVEC64)
+        apply_level<double, double>(c, 0, AP_AX, c->p.as<double>(), 0.0, c->q.as<double>(), nullptr,
+                                    c->part.as<double>(), nullptr);
+      else
+        apply_level<double, double>(c, 0, AP_SMOOTH, c->r.as<double>(), c->prm.jacobi_weight,
+                                    c->tmp.as<double>(), nullptr, nullptr, nullptr);
+    };
+    launch();
+    hipEvent_t e0, e1;
+    GMGS_HIP(hipEventCreate(&e0));
+    GMGS_HIP(hipEventCreate(&e1));
+    GMGS_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i) launch();
+    GMGS_HIP(hipEventRecord(e1, c->stream));
+    GMGS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GMGS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    GMGS_HIP(hipEventDestroy(e0));
+    GMGS_HIP(hipEventDestroy(e1));
+    return GMGS_OK;
+  });
 }
 
 const char* gmgs_last_error(const gmgs_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }

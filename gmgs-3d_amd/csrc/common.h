@@ -29,6 +29,7 @@ constexpr uint32_t F_ELEM_MASK = 0xFFu;      // bit p: incident element p presen
 constexpr uint32_t F_FIX_SHIFT = 8;          // bits 8..10: fixed component c
 constexpr uint32_t F_FIX_MASK = 0x7u << 8;
 constexpr uint32_t F_FAST = 1u << 11;        // all 8 incident elements present and universal
+constexpr uint32_t F_STORED_SHIFT = 12;      // bits 12..19: incident element p present and stored
 
 // Element position p around a node (bits: py | px<<1 | pz<<2): the element
 // whose minimum corner is (y-1+py, x-1+px, z-1+pz).  The node is that

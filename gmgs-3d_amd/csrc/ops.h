@@ -21,6 +21,7 @@ struct LevelOp {
   const T* cls;              // its 30 class coefficients (z-marching kernel)
   const T* emod;             // per padded cell modulus factor (level-1 SIMP) or null
   const T* invd;             // inverse diagonal, SoA 3N (zero off the solid nodes)
+  const int* colz;           // per padded column (y + py x): solid-node z range lo | hi << 16, or null
 };
 
 struct PcgScalars {
@@ -66,9 +67,17 @@ template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                      const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s);
 
+// x = (x + omega r / d) + alpha p (the Jacobi increment of the iteration, from
+// r before it is replaced), r = r_a - alpha q, partials of r.r
 void launch_update_xr(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                       double* x, const double* p, const double* ra, const double* q, double* r,
-                      const PcgScalars* sc, double* partial, const int* done, hipStream_t s);
+                      const double* invd, double omega, const PcgScalars* sc, double* partial,
+                      const int* done, hipStream_t s);
+
+// x += omega r / d on the solid nodes
+template <typename T, typename TR>
+void launch_jacobi_acc(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                       const T* invd, const TR* r, T omega, TR* x, hipStream_t s);
 
 // dot / norm partials over tiles (FP64)
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
@@ -110,5 +119,11 @@ void launch_copy_probe(const double* a, double* b, long long n_doubles, hipStrea
 void launch_gather_flags(const int* const* flags, int n, int* out, hipStream_t s);
 
 inline unsigned apply_blocks(long long ntiles) { return (unsigned)((ntiles + 3) / 4); }
+// grid of the pointwise kernels (update_p, update_xr, dot, prolong_smooth):
+// also their number of dot partials
+inline unsigned pointwise_blocks(long long ntiles) {
+  const unsigned b = apply_blocks(ntiles);
+  return b < 2048u ? b : 2048u;
+}
 
 }  // namespace gmgs

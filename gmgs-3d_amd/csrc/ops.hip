@@ -33,14 +33,15 @@ struct LaneNode {
   int yp, xp, zp;
 };
 
-__device__ inline LaneNode lane_node(const Grid& g, const uint32_t* __restrict__ flags,
-                                     const TileRec* __restrict__ tiles, long long ntiles) {
+// node of this lane in tile t (one 64-node tile per wave)
+__device__ inline LaneNode lane_node_at(const Grid& g, const uint32_t* __restrict__ flags,
+                                        const TileRec* __restrict__ tiles, long long ntiles,
+                                        long long t) {
   LaneNode r;
   r.active = false;
   r.n = 0;
   r.fl = 0;
   const int lane = threadIdx.x & 63;
-  const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   r.yp = r.xp = r.zp = 0;
   if (t < ntiles) {
     const TileRec tr = tiles[t];
@@ -55,6 +56,18 @@ __device__ inline LaneNode lane_node(const Grid& g, const uint32_t* __restrict__
   }
   return r;
 }
+
+// one tile per wave, four per block, tile = 4 blockIdx + wave
+__device__ inline LaneNode lane_node(const Grid& g, const uint32_t* __restrict__ flags,
+                                     const TileRec* __restrict__ tiles, long long ntiles) {
+  return lane_node_at(g, flags, tiles, ntiles, (long long)blockIdx.x * 4 + (threadIdx.x >> 6));
+}
+
+// tile loop of the pointwise kernels: a fixed grid walks the tile list, so a
+// kernel that reduces leaves one partial per block, gridDim.x in all
+#define GMGS_TILE_LOOP(g, flags, tiles, nt, ln_)                                              \
+  for (long long tb_ = (long long)blockIdx.x * 4; tb_ < (nt); tb_ += (long long)gridDim.x * 4) \
+    if (const LaneNode ln_ = lane_node_at(g, flags, tiles, nt, tb_ + (threadIdx.x >> 6)); true)
 
 template <typename T, typename TR, int MODE>
 struct InputU {
@@ -190,9 +203,9 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(
     const T* __restrict__ rp, T omega, TZ* __restrict__ x, int fix, const TZ* __restrict__ zprev,
     double* __restrict__ partial, const int* __restrict__ done) {
   if (done && *done) return;
-  const LaneNode ln_ = lane_node(gf, flags, tiles, nt);
   double d0 = 0.0, d1 = 0.0;
-  if (ln_.active) {
+  GMGS_TILE_LOOP(gf, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
     const int yf = ln_.yp - 1, xf = ln_.xp - 1, zf = ln_.zp - 1;
     const int ya = yf >> 1, yb = (yf + 1) >> 1, xa = xf >> 1, xb = (xf + 1) >> 1, za = zf >> 1,
               zb = (zf + 1) >> 1;
@@ -240,31 +253,37 @@ __global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __rest
                                                   const PcgScalars* __restrict__ sc,
                                                   const int* __restrict__ done) {
   if (done && *done) return;
-  const LaneNode ln_ = lane_node(g, flags, tiles, nt);
-  if (!ln_.active) return;
   const double beta = sc->beta;
-  const long long N = g.N, n = ln_.n;
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+    const long long n = ln_.n;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) p[c * N + n] = (double)z[c * N + n] + beta * p[c * N + n];
+    for (int c = 0; c < 3; ++c) p[c * N + n] = (double)z[c * N + n] + beta * p[c * N + n];
+  }
 }
 
+// x = (x + omega r / d) + alpha p with the iteration's Jacobi increment
+// taken from r before it is replaced; r = r_a - alpha q; partials of r.r
 __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __restrict__ flags,
                                                    const TileRec* __restrict__ tiles, long long nt,
                                                    double* __restrict__ x, const double* __restrict__ p,
                                                    const double* __restrict__ ra,
                                                    const double* __restrict__ q, double* __restrict__ r,
+                                                   const double* __restrict__ invd, double omega,
                                                    const PcgScalars* __restrict__ sc,
                                                    double* __restrict__ partial,
                                                    const int* __restrict__ done) {
   if (done && *done) return;
-  const LaneNode ln_ = lane_node(g, flags, tiles, nt);
   double d = 0.0;
-  if (ln_.active) {
-    const double alpha = sc->alpha;
+  const double alpha = sc->alpha;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
     const long long N = g.N, n = ln_.n;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      x[c * N + n] += alpha * p[c * N + n];
+      const double u = omega * invd[c * N + n] * r[c * N + n];
+      x[c * N + n] = (x[c * N + n] + u) + alpha * p[c * N + n];
       const double rv = ra[c * N + n] - alpha * q[c * N + n];
       r[c * N + n] = rv;
       d += rv * rv;
@@ -274,14 +293,29 @@ __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __res
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
+// x += omega r / d on the solid nodes
+template <typename T, typename TR>
+__global__ __launch_bounds__(256) void k_jacobi_acc(Grid g, const uint32_t* __restrict__ flags,
+                                                    const TileRec* __restrict__ tiles, long long nt,
+                                                    const T* __restrict__ invd, const TR* __restrict__ r,
+                                                    T omega, TR* __restrict__ x) {
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+    const long long n = ln_.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c * N + n] += (TR)(omega * invd[c * N + n] * (T)r[c * N + n]);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_dot(Grid g, const uint32_t* __restrict__ flags,
                                              const TileRec* __restrict__ tiles, long long nt,
                                              const double* __restrict__ a,
                                              const double* __restrict__ b,
                                              double* __restrict__ partial) {
-  const LaneNode ln_ = lane_node(g, flags, tiles, nt);
   double d = 0.0;
-  if (ln_.active) {
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
     const long long N = g.N, n = ln_.n;
 #pragma unroll
     for (int c = 0; c < 3; ++c) d += a[c * N + n] * b[c * N + n];
@@ -461,8 +495,8 @@ __global__ __launch_bounds__(256) void k_element_energy(Grid g, const long long*
   out[k] = -acc;
 }
 
-__global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ a,
-                                                    double2* __restrict__ b, long long n) {
+__global__ __launch_bounds__(256) void k_copy_probe(const double* __restrict__ a,
+                                                    double* __restrict__ b, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     b[i] = a[i];
@@ -476,9 +510,9 @@ __global__ void k_gather_flags(const int* const* __restrict__ flags, int n, int*
 }  // namespace
 
 void launch_copy_probe(const double* a, double* b, long long n_doubles, hipStream_t s) {
-  const long long n2 = n_doubles / 2;
-  k_copy_probe<<<(unsigned)std::min<long long>((n2 + 255) / 256, 65536), 256, 0, s>>>(
-      reinterpret_cast<const double2*>(a), reinterpret_cast<double2*>(b), n2);
+  // one double per lane, the access width of the solver's kernels
+  k_copy_probe<<<(unsigned)std::min<long long>((n_doubles + 255) / 256, 65536), 256, 0, s>>>(
+      a, b, n_doubles);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -537,7 +571,7 @@ void launch_prolong_smooth(const Grid& gf, const uint32_t* flags_f, const TileRe
                            T omega, TZ* x, bool fix, const TZ* zprev, double* partial,
                            const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_prolong_smooth<T, TR, TZ><<<apply_blocks(nt), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, invd, r,
+  k_prolong_smooth<T, TR, TZ><<<pointwise_blocks(nt), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, invd, r,
                                                                rp, omega, x, fix ? 1 : 0, zprev,
                                                                partial, done);
   GMGS_LAUNCH_CHECK();
@@ -546,20 +580,37 @@ void launch_prolong_smooth(const Grid& gf, const uint32_t* flags_f, const TileRe
 template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                      const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s) {
-  k_update_p<TZ><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, p, sc, done);
+  k_update_p<TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, p, sc, done);
   GMGS_LAUNCH_CHECK();
 }
 
 void launch_update_xr(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                       double* x, const double* p, const double* ra, const double* q, double* r,
-                      const PcgScalars* sc, double* partial, const int* done, hipStream_t s) {
-  k_update_xr<<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, ra, q, r, sc, partial, done);
+                      const double* invd, double omega, const PcgScalars* sc, double* partial,
+                      const int* done, hipStream_t s) {
+  k_update_xr<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, ra, q, r, invd, omega, sc,
+                                                   partial, done);
   GMGS_LAUNCH_CHECK();
 }
 
+template <typename T, typename TR>
+void launch_jacobi_acc(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                       const T* invd, const TR* r, T omega, TR* x, hipStream_t s) {
+  if (nt <= 0) return;
+  k_jacobi_acc<T, TR><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, r, omega, x);
+  GMGS_LAUNCH_CHECK();
+}
+template void launch_jacobi_acc<double, double>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                                const double*, const double*, double, double*,
+                                                hipStream_t);
+template void launch_jacobi_acc<float, float>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                              const float*, const float*, float, float*, hipStream_t);
+template void launch_jacobi_acc<float, double>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                               const float*, const double*, float, double*, hipStream_t);
+
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                 const double* a, const double* b, double* partial, hipStream_t s) {
-  k_dot<<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, b, partial);
+  k_dot<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, b, partial);
   GMGS_LAUNCH_CHECK();
 }
 

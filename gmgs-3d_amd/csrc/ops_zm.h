@@ -13,10 +13,15 @@ struct ColTile {
 long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, DevBuf& tiles,
                          hipStream_t s);
 
-// same contract as launch_apply (one partial per block -> ntiles partials)
+// per padded column (y + py x) the z range [lo, hi] of its solid nodes packed
+// as lo | hi << 16; an empty column holds lo = 0xffff, hi = 0
+void build_colrange(const Grid& g, const uint32_t* flags, DevBuf& colz, hipStream_t s);
+
+// same contract as launch_apply without the x += u accumulation; returns the
+// number of dot partials written (one per tile)
 template <typename T, typename TR>
-void launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
-                     int mode, const TR* in, T omega, T* out, TR* xacc, double* partial,
-                     const int* done, hipStream_t s);
+int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
+                    int mode, const TR* in, T omega, T* out, double* partial, const int* done,
+                    hipStream_t s);
 
 }  // namespace gmgs

@@ -6,11 +6,19 @@
 // plane's contribution to the three outputs it touches (z-1, z, z+1), so a
 // value is fetched from memory about once per plane instead of 27 times.
 //
-// Interior nodes use the assembled 27-point stencil of the universal element
-// matrix with its structural zeros skipped at compile time; a node missing an
-// element, or touching a stored (non-universal) element, subtracts that
-// element's universal contribution plane by plane and adds the stored one.
-// Same result as the per-element sum of kernel/AtX.m:1-58.
+// Void nodes hold zero in every vector, so a staged value outside its
+// column's solid z range is not loaded.  Tiles are numbered y-fastest and the
+// hardware deals consecutive workgroups to the 8 XCDs in turn; the block ->
+// tile map below gives each XCD a contiguous run of tiles instead, so the
+// halo rows a tile shares with its neighbours are read while they are still
+// in that XCD's L2.
+//
+// Every solid node gets the stencil of the universal element matrix (its 30
+// class coefficients).  A node that misses an incident element, or touches a
+// stored (non-universal) element, is then corrected element by element; the
+// wave gathers those (node, element) pairs and spreads them over its lanes,
+// so the few boundary nodes of a wave do not serialise it.  Together: the
+// per-element sum of kernel/AtX.m:1-58.
 #include "ops.h"
 #include "ops_zm.h"
 #include "mesh.h"
@@ -37,67 +45,214 @@ __device__ inline cptr<T> opaque(const T* p) {
 template <typename T, typename TR, int MODE, bool HAS_EOP, int TX>
 __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTile* __restrict__ tiles,
                                                       int zc, const TR* __restrict__ in, T omega,
-                                                      T* __restrict__ out, TR* __restrict__ xacc,
-                                                      double* __restrict__ partial,
+                                                      T* __restrict__ out, double* __restrict__ partial,
                                                       const int* __restrict__ done) {
   constexpr int TY = 16, NT = TY * TX, SY = TY + 2, SX = TX + 2;
   constexpr int NSTAGE = 3 * SX * SY;
   constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
+  constexpr bool SM = MODE == AP_SMOOTH;
   if (done && *done) return;
-  __shared__ T sU[3][SX][SY];
-  const ColTile t = tiles[blockIdx.x];
-  const int tid = threadIdx.x, ly = tid % TY, lx = tid / TY;
+  // four plane buffers: plane k (counted from the tile's first) is staged in
+  // buffer k % 4 and read while computing planes k, k + 1 and k + 2 (stencil,
+  // output completion, element corrections)
+  __shared__ T sU[4][3][SX][SY];
+  __shared__ T sR[SM ? 2 : 1][3][TX][TY];            // SMOOTH: centre values of r
+  __shared__ T sK[576];                              // universal element matrix
+  // XCD-contiguous tile order: blocks b = 8 j + k (k = XCD) of the first
+  // 8 floor(n / 8) blocks take tile k floor(n / 8) + j; the rest keep their own
+  const int nb = (int)gridDim.x, q8 = nb >> 3, bid = (int)blockIdx.x;
+  const int ti = bid < (q8 << 3) ? (bid & 7) * q8 + (bid >> 3) : bid;
+  const ColTile t = tiles[ti];
+  const int tid = threadIdx.x, ly = tid % TY, lx = tid / TY, lane = tid & 63;
+  const int lx_w = (tid >> 6) * (64 / TY);           // lx of the wave's first lane
   const Grid g = op.g;
   const long long N = g.N, sx = g.py, sz = (long long)g.py * g.px;
   const int yp = t.y0 + ly, xp = t.x0 + lx;
   const bool lane_ok = xp < g.px;
   const long long colidx = yp + sx * xp;
   const int z0 = t.z0, z1 = min(t.z0 + zc, g.pz - 1);   // outputs [z0, z1)
+  int own_lo = 1, own_hi = 0;                            // own column's solid range
+  if (lane_ok) {
+    const int r = op.colz[colidx];
+    own_lo = r & 0xffff;
+    own_hi = r >> 16;
+  }
+  for (int i = tid; i < 576; i += NT) sK[i] = op.Ke[i];
 
   // this thread's share of a staged plane: global offset (without z) or -1
-  long long soff[PER];
+  // for a position outside the grid or on a column without solid nodes, and
+  // that column's solid z range (lo in the low 16 bits, hi - lo above)
+  int soff[PER];
+  int zr[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = tid + i * NT;
     soff[i] = -1;
+    zr[i] = 0;
     if (e < NSTAGE) {
       const int c = e / (SX * SY), rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
       const int gy = t.y0 - 1 + yy, gx = t.x0 - 1 + xx;
-      if (gy >= 0 && gy < g.py && gx >= 0 && gx < g.px) soff[i] = gy + sx * gx + c * N;
+      if (gy >= 0 && gy < g.py && gx >= 0 && gx < g.px) {
+        const long long col = gy + sx * gx;
+        const int r = op.colz[col];
+        const int lo = r & 0xffff, hi = r >> 16;
+        if (hi >= lo) {
+          soff[i] = (int)(col + c * N);
+          zr[i] = lo | ((hi - lo) << 16);
+        }
+      }
     }
   }
-  auto fetch = [&](int zz, T* v) {
+  // raw loads only (no arithmetic on them until they are staged, two planes
+  // later): the input and, for the smoother, the inverse diagonal
+  auto fetch = [&](int zz, TR* vr, T* vd) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      v[i] = 0;
-      if (soff[i] >= 0) {
-        const long long m = soff[i] + sz * zz;
-        if (MODE == AP_AX) v[i] = (T)in[m];
-        else v[i] = omega * op.invd[m] * (T)in[m];
+      vr[i] = 0;
+      if (SM) vd[i] = 0;
+      if (soff[i] >= 0 && (unsigned)(zz - (zr[i] & 0xffff)) <= (unsigned)(zr[i] >> 16)) {
+        const long long m = (long long)soff[i] + sz * zz;
+        vr[i] = in[m];
+        if (SM) vd[i] = op.invd[m];
       }
     }
   };
 
   T aM[3] = {0, 0, 0}, a0[3] = {0, 0, 0}, aP[3] = {0, 0, 0};   // outputs zz-1, zz, zz+1
   uint32_t fM = 0, f0 = 0, fP = 0;
-  T uc0[3] = {0, 0, 0};   // centre u of plane zz-1 (the output being written)
   double dsum = 0.0;
-  T stage[PER];
-  fetch(z0 - 1, stage);
+  int kb = 0;                                       // buffer of the current plane
+  // two planes of loads in flight: plane zz is staged from one register set
+  // while the other holds plane zz + 1; z0 < z1 for every tile
+  TR rA[PER], rB[PER];
+  T dA[SM ? PER : 1], dB[SM ? PER : 1];
+  fetch(z0 - 1, rA, dA);
+  fetch(z0, rB, dB);
+  // flags of the outputs this lane owns, loaded one plane before they are used
+  auto flags_at = [&](const int z) -> uint32_t {
+    return (lane_ok && z >= z0 && z < z1 && z >= own_lo && z <= own_hi) ? op.flags[colidx + sz * z]
+                                                                          : 0u;
+  };
+  uint32_t fN = flags_at(z0);
 
-  for (int zz = z0 - 1; zz <= z1; ++zz) {
-    __syncthreads();
+  auto plane = [&](const int zz, TR (&sr)[PER], T (&sd)[SM ? PER : 1]) {
+    T (*const sB)[SX][SY] = sU[kb];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int e = tid + i * NT;
-      if (e < NSTAGE) (&sU[0][0][0])[e] = stage[i];
+      if (e < NSTAGE) {
+        T v = (T)sr[i];
+        if (SM) v = omega * sd[i] * v;
+        (&sB[0][0][0])[e] = v;
+      }
     }
+    // one barrier per plane: buffer kb was last read while computing planes
+    // zz - 4 .. zz - 2, which every thread finished before the barrier of
+    // plane zz - 1
     __syncthreads();
-    if (zz < z1) fetch(zz + 1, stage);               // next plane in flight during compute
+    if (SM) {
+      // centre values of r for the output completed at plane zz + 1; buffer
+      // zz % 2 was last read at plane zz - 1, before this plane's barrier
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * NT;
+        if (e < NSTAGE) {
+          const int c = e / (SX * SY), rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
+          if (xx >= 1 && xx <= TX && yy >= 1 && yy <= TY) sR[zz & 1][c][xx - 1][yy - 1] = (T)sr[i];
+        }
+      }
+    }
+    // planes zz-2, zz-1, zz are staged in buffers b2, b1, kb
+    const int b1 = (kb + 3) & 3, b2 = (kb + 2) & 3;
+    // Element corrections of output zz-1.  An owned node lacking incident
+    // element p subtracts that element's universal row block, one touching a
+    // stored element p swaps the universal block for the stored one.  The
+    // wave lists its (node, element) pairs, spreads them over its 64 lanes
+    // (72 multiply-adds each), and every owner sums its own pairs in
+    // ascending p.  Done before the next loads are issued and before the
+    // class sums exist, where the fewest registers are live; aM is additive.
+    {
+      uint32_t cm = 0;
+      if (fM & F_ELEM_MASK) cm = ((~fM) & F_ELEM_MASK) | ((fM >> F_STORED_SHIFT) & 0xFFu);
+      const int cnt = __popc(cm);
+      int off = cnt;                                   // inclusive prefix over the wave
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(off, d);
+        if (lane >= d) off += v;
+      }
+      const int total = __shfl(off, 63);
+      off -= cnt;
+      if (total > 0) {                                 // uniform over the wave
+        int mcnt = cnt;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) mcnt = max(mcnt, __shfl_xor(mcnt, d));
+        T D0 = 0, D1 = 0, D2 = 0;
+        for (int rb = 0; rb < total; rb += 64) {
+          const int j = rb + lane;
+          int own = 0;                                 // last lane whose first pair is <= j
+#pragma unroll
+          for (int st = 32; st > 0; st >>= 1) {
+            const int o = __shfl(off, own + st);
+            if (o <= j) own += st;
+          }
+          const uint32_t cmo = __shfl(cm, own);
+          const uint32_t fo = __shfl(fM, own);
+          const int offo = __shfl(off, own);
+          T e0 = 0, e1 = 0, e2 = 0;
+          if (j < total) {
+            uint32_t mm = cmo;
+            for (int q = j - offo; q > 0; --q) mm &= mm - 1;
+            const int p = __ffs(mm) - 1;
+            const int pyb = p & 1, pxb = (p >> 1) & 1, pzb = p >> 2;
+            const int oly = own % TY + 1, olx = lx_w + own / TY + 1;   // staged coords of the node
+            const int ln = node_local_in(p);
+            const T* Kst = nullptr;
+            if (HAS_EOP && ((fo >> p) & 1u)) {
+              const long long no = (t.y0 + oly - 1) + sx * (t.x0 + olx - 1) + sz * (zz - 1);
+              const long long cell = no - (1 - pyb) - (1 - pxb) * sx - (1 - pzb) * sz;
+              Kst = op.Ks + (long long)(op.eop[cell] - 1) * 576;
+            }
+#pragma unroll 1
+            for (int m = 0; m < 8; ++m) {
+              const int dy = hex_ro(m) - (1 - pyb), dx = hex_xo(m) - (1 - pxb), dz = hex_zo(m) - (1 - pzb);
+              const int bz = dz < 0 ? b2 : (dz == 0 ? b1 : kb);
+              const T u0 = sU[bz][0][olx + dx][oly + dy], u1 = sU[bz][1][olx + dx][oly + dy],
+                      u2 = sU[bz][2][olx + dx][oly + dy];
+              const T* k = sK + 3 * ln * 24 + 3 * m;
+              e0 += k[0] * u0 + k[1] * u1 + k[2] * u2;
+              e1 += k[24] * u0 + k[25] * u1 + k[26] * u2;
+              e2 += k[48] * u0 + k[49] * u1 + k[50] * u2;
+              if (HAS_EOP && Kst) {
+                const T* ks = Kst + 3 * ln * 24 + 3 * m;
+                e0 -= ks[0] * u0 + ks[1] * u1 + ks[2] * u2;
+                e1 -= ks[24] * u0 + ks[25] * u1 + ks[26] * u2;
+                e2 -= ks[48] * u0 + ks[49] * u1 + ks[50] * u2;
+              }
+            }
+          }
+          // owners collect this round's pairs, in order
+          for (int q = 0; q < mcnt; ++q) {
+            const int idx = off + q, src = (idx - rb) & 63;
+            const T v0 = __shfl(e0, src), v1 = __shfl(e1, src), v2 = __shfl(e2, src);
+            if (q < cnt && idx >= rb && idx < rb + 64) {
+              D0 += v0;
+              D1 += v1;
+              D2 += v2;
+            }
+          }
+        }
+        aM[0] -= D0;
+        aM[1] -= D1;
+        aM[2] -= D2;
+      }
+    }
+    if (zz + 2 <= z1) fetch(zz + 2, sr, sd);
     // flags of the output at zz+1 (only outputs in [z0, z1) are owned)
-    fP = (lane_ok && zz + 1 >= z0 && zz + 1 < z1) ? op.flags[colidx + sz * (zz + 1)] : 0u;
+    fP = fN;
+    fN = flags_at(zz + 2);
     // neighbour (dy, dx) = (a-1, b-1), component c, read from the staged plane
-#define GMGS_U(a, b, c) sU[c][lx + (b)][ly + (a)]
+#define GMGS_U(a, b, c) sB[c][lx + (b)][ly + (a)]
     // class sums of the in-plane neighbourhood (shared by the three outputs)
     T C0[3], Cy[3], Cx[3], Cxy[3];
 #pragma unroll
@@ -117,6 +272,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     const T Dy1_y = (GMGS_U(2, 0, 1) - GMGS_U(0, 0, 1)) + (GMGS_U(2, 2, 1) - GMGS_U(0, 2, 1));
     const T Dy0_z = GMGS_U(2, 1, 2) - GMGS_U(0, 1, 2);
     const T Dy1_z = (GMGS_U(2, 0, 2) - GMGS_U(0, 0, 2)) + (GMGS_U(2, 2, 2) - GMGS_U(0, 2, 2));
+#undef GMGS_U
     {
       const cptr<T> C = opaque(op.cls);
       T cf[30];
@@ -141,58 +297,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
         }
       }
     }
-    // corrections for nodes that miss an element or touch a stored one
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int oz = 1 - k;
-      const uint32_t f = k == 0 ? fM : (k == 1 ? f0 : fP);
-      if (!(f & F_ELEM_MASK) || (f & F_FAST)) continue;
-      const cptr<T> Ke = opaque(op.Ke);
-      const int zout = zz - oz;
-      const long long nout = colidx + sz * zout;
-      T d0 = 0, d1 = 0, d2 = 0;
-#pragma unroll
-      for (int p = 0; p < 8; ++p) {
-        const int pyb = p & 1, pxb = (p >> 1) & 1, pzb = p >> 2;
-        if (pzb != oz && pzb != oz + 1) continue;          // element does not reach plane zz
-        const bool present = (f >> p) & 1u;
-        const T* Kst = nullptr;
-        if (present) {
-          if (!HAS_EOP) continue;
-          const long long cell = nout - (1 - pyb) - (1 - pxb) * sx - (1 - pzb) * sz;
-          const int e = op.eop[cell];
-          if (e <= 0) continue;
-          Kst = op.Ks + (long long)(e - 1) * 576;
-        }
-        const int ln = node_local_in(p);
-        const int zo = oz + 1 - pzb;                       // local z of the plane in the element
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-          if (hex_zo(m) != zo) continue;
-          const int a = pyb + hex_ro(m), b = pxb + hex_xo(m);   // = dy+1, dx+1
-          const T u0 = GMGS_U(a, b, 0), u1 = GMGS_U(a, b, 1), u2 = GMGS_U(a, b, 2);
-          const cptr<T> k0 = Ke + (3 * ln + 0) * 24 + 3 * m;
-          const cptr<T> k1 = Ke + (3 * ln + 1) * 24 + 3 * m;
-          const cptr<T> k2 = Ke + (3 * ln + 2) * 24 + 3 * m;
-          T e0 = k0[0] * u0 + k0[1] * u1 + k0[2] * u2;
-          T e1 = k1[0] * u0 + k1[1] * u1 + k1[2] * u2;
-          T e2 = k2[0] * u0 + k2[1] * u1 + k2[2] * u2;
-          if (HAS_EOP && Kst) {
-            const T* s0 = Kst + (3 * ln + 0) * 24 + 3 * m;
-            const T* s1 = Kst + (3 * ln + 1) * 24 + 3 * m;
-            const T* s2 = Kst + (3 * ln + 2) * 24 + 3 * m;
-            e0 -= s0[0] * u0 + s0[1] * u1 + s0[2] * u2;
-            e1 -= s1[0] * u0 + s1[1] * u1 + s1[2] * u2;
-            e2 -= s2[0] * u0 + s2[1] * u1 + s2[2] * u2;
-          }
-          d0 += e0; d1 += e1; d2 += e2;
-        }
-      }
-      if (k == 0) { aM[0] -= d0; aM[1] -= d1; aM[2] -= d2; }
-      else if (k == 1) { a0[0] -= d0; a0[1] -= d1; a0[2] -= d2; }
-      else { aP[0] -= d0; aP[1] -= d1; aP[2] -= d2; }
-    }
-    // output zz-1 is complete
+    // write output zz-1
     const int zo_ = zz - 1;
     if (lane_ok && zo_ >= z0 && zo_ < z1 && (fM & F_ELEM_MASK)) {
       const long long n = colidx + sz * zo_;
@@ -201,10 +306,9 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
         const T a = (fM & (1u << (F_FIX_SHIFT + c))) ? (T)0 : aM[c];
         if (MODE == AP_AX) {
           out[c * N + n] = a;
-          if (partial) dsum += (double)uc0[c] * (double)a;
+          if (partial) dsum += (double)sU[b1][c][lx + 1][ly + 1] * (double)a;
         } else {
-          out[c * N + n] = (T)in[c * N + n] - a;
-          if (xacc) xacc[c * N + n] += (TR)uc0[c];
+          out[c * N + n] = sR[zo_ & 1][c][lx][ly] - a;
         }
       }
     }
@@ -214,11 +318,14 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
       aM[c] = a0[c];
       a0[c] = aP[c];
       aP[c] = 0;
-      uc0[c] = C0[c];
     }
-#undef GMGS_U
     fM = f0;
     f0 = fP;
+    kb = (kb + 1) & 3;
+  };
+  for (int zz = z0 - 1; zz <= z1; zz += 2) {
+    plane(zz, rA, dA);
+    if (zz + 1 <= z1) plane(zz + 1, rB, dB);
   }
   if (partial) {
     // block reduction (fixed order)
@@ -263,7 +370,28 @@ __global__ void k_coltile_emit(const int* __restrict__ mark, const int* __restri
   out[scan[t]] = r;
 }
 
+__global__ void k_colrange(Grid g, const uint32_t* __restrict__ flags, int* __restrict__ colz) {
+  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ncol = (long long)g.py * g.px, sz = ncol;
+  if (col >= ncol) return;
+  int lo = 0xffff, hi = 0;
+  for (int z = 0; z < g.pz; ++z)
+    if (flags[col + sz * z] & F_ELEM_MASK) {
+      lo = min(lo, z);
+      hi = z;
+    }
+  colz[col] = lo | (hi << 16);
+}
+
 }  // namespace
+
+void build_colrange(const Grid& g, const uint32_t* flags, DevBuf& colz, hipStream_t s) {
+  if (g.pz > 32767) throw std::invalid_argument("z extent too large for the column range table");
+  const long long ncol = (long long)g.py * g.px;
+  colz.alloc(sizeof(int) * ncol);
+  k_colrange<<<grid1d(ncol, 256), 256, 0, s>>>(g, flags, colz.as<int>());
+  GMGS_LAUNCH_CHECK();
+}
 
 long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, DevBuf& tiles,
                          hipStream_t s) {
@@ -284,19 +412,20 @@ long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, D
 }
 
 template <typename T, typename TR>
-void launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
-                     int mode, const TR* in, T omega, T* out, TR* xacc, double* partial,
-                     const int* done, hipStream_t s) {
-  if (ntiles <= 0) return;
+int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
+                    int mode, const TR* in, T omega, T* out, double* partial, const int* done,
+                    hipStream_t s) {
+  if (ntiles <= 0) return 0;
+  if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("grid too large for the z-marching operator");
   const bool eop = op.eop != nullptr;
-#define GMGS_ZM(MODE, E, TXV)                                                                    \
+#define GMGS_ZM(MODE, E, TXV)                                                                        \
   k_apply_zm<T, TR, MODE, E, TXV><<<(unsigned)ntiles, 16 * TXV, 0, s>>>(op, tiles, zc, in, omega, out, \
-                                                                        xacc, partial, done)
-#define GMGS_ZM_TX(MODE, E)            \
-  do {                                 \
-    if (tx == 4) GMGS_ZM(MODE, E, 4);  \
+                                                                        partial, done)
+#define GMGS_ZM_TX(MODE, E)               \
+  do {                                    \
+    if (tx == 4) GMGS_ZM(MODE, E, 4);     \
     else if (tx == 16) GMGS_ZM(MODE, E, 16); \
-    else GMGS_ZM(MODE, E, 8);          \
+    else GMGS_ZM(MODE, E, 8);             \
   } while (0)
   if (mode == AP_AX) {
     if (eop) GMGS_ZM_TX(AP_AX, true);
@@ -308,16 +437,17 @@ void launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntile
 #undef GMGS_ZM_TX
 #undef GMGS_ZM
   GMGS_LAUNCH_CHECK();
+  return (int)ntiles;
 }
 
-template void launch_apply_zm<double, double>(const LevelOp<double>&, const ColTile*, long long, int, int,
-                                              int, const double*, double, double*, double*, double*,
-                                              const int*, hipStream_t);
-template void launch_apply_zm<float, float>(const LevelOp<float>&, const ColTile*, long long, int, int,
-                                            int, const float*, float, float*, float*, double*,
-                                            const int*, hipStream_t);
-template void launch_apply_zm<float, double>(const LevelOp<float>&, const ColTile*, long long, int, int,
-                                             int, const double*, float, float*, double*, double*,
-                                             const int*, hipStream_t);
+template int launch_apply_zm<double, double>(const LevelOp<double>&, const ColTile*, long long, int, int,
+                                             int, const double*, double, double*, double*, const int*,
+                                             hipStream_t);
+template int launch_apply_zm<float, float>(const LevelOp<float>&, const ColTile*, long long, int, int,
+                                           int, const float*, float, float*, double*, const int*,
+                                           hipStream_t);
+template int launch_apply_zm<float, double>(const LevelOp<float>&, const ColTile*, long long, int, int,
+                                            int, const double*, float, float*, double*, const int*,
+                                            hipStream_t);
 
 }  // namespace gmgs

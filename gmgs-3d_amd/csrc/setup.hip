@@ -88,16 +88,19 @@ __global__ void k_mark_fast(Grid g, const TileRec* __restrict__ tiles, long long
   long long n;
   int yp, xp, zp;
   if (!lane_pos(g, tiles, nt, n, yp, xp, zp)) return;
-  uint32_t f = flags[n] & ~F_FAST;
+  uint32_t f = flags[n] & ~(F_FAST | (0xFFu << F_STORED_SHIFT));
   bool fast = allow && ((f & F_ELEM_MASK) == 0xFFu);
-  if (fast && eop) {
+  uint32_t stored = 0;
+  if (eop) {
     const long long sx = g.py, sz = (long long)g.py * g.px;
     for (int p = 0; p < 8; ++p) {
+      if (!((f >> p) & 1u)) continue;
       const long long cell = n - (1 - (p & 1)) - (1 - ((p >> 1) & 1)) * sx - (1 - (p >> 2)) * sz;
-      fast = fast && (eop[cell] == 0);
+      if (eop[cell] > 0) stored |= 1u << p;
     }
+    fast = fast && stored == 0;
   }
-  flags[n] = f | (fast ? F_FAST : 0u);
+  flags[n] = f | (fast ? F_FAST : 0u) | (stored << F_STORED_SHIFT);
 }
 
 __global__ void k_child_masks(const long long* __restrict__ epc, long long ne, Grid gc, Grid gf,

@@ -13,13 +13,15 @@ names (src/GlobalVariables.m:3-38) and exposes its entry points:
 from __future__ import annotations
 
 import ctypes as C
+import os
 import warnings
 from pathlib import Path
 from typing import Optional
 
 import numpy as np
 
-_LIB_PATH = Path(__file__).resolve().parent / "libgmgs.so"
+# the in-tree build; GMGS_LIBRARY names another build of the same library
+_LIB_PATH = Path(os.environ.get("GMGS_LIBRARY") or Path(__file__).resolve().parent / "libgmgs.so")
 
 GMGS_PREC_FP64 = 0
 GMGS_PREC_MIXED = 1
@@ -114,6 +116,8 @@ def load_library() -> C.CDLL:
     lib.gmgs_kernel_timing_read.argtypes = [P, C.c_int, C.POINTER(C.c_double),
                                             C.POINTER(C.c_int64)]
     lib.gmgs_kernel_timing_read.restype = C.c_int
+    lib.gmgs_time_operator.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    lib.gmgs_time_operator.restype = C.c_int
     lib.gmgs_last_error.argtypes = [P]
     lib.gmgs_last_error.restype = C.c_char_p
     lib.gmgs_destroy.argtypes = [P]
@@ -131,7 +135,8 @@ def exported_symbols():
             "gmgs_set_modulus", "gmgs_solve", "gmgs_stress", "gmgs_adjoint", "gmgs_atx",
             "gmgs_vcycle", "gmgs_smooth", "gmgs_restrict", "gmgs_interpolate", "gmgs_query_size",
             "gmgs_query", "gmgs_last_error", "gmgs_destroy", "gmgs_kernel_timing",
-            "gmgs_kernel_timing_read", "gmgs_batch_solve", "gmgs_copy_probe"]
+            "gmgs_kernel_timing_read", "gmgs_batch_solve", "gmgs_copy_probe",
+            "gmgs_time_operator"]
 
 
 def _dptr(a: np.ndarray):
@@ -256,13 +261,17 @@ class Solver:
         m = np.ascontiguousarray(np.asarray(modulus, dtype=np.float64).ravel())
         self._check(self._lib.gmgs_set_modulus(self._ctx, _dptr(m), m.size))
 
-    def solve(self, raise_on_nonconvergence=True):
+    def solve(self, raise_on_nonconvergence=True, copy=True):
+        """GetDeformation: returns (U, info, iteration history).  With
+        ``copy=False`` the solution stays on the device (U is None); a later
+        ``stress()`` or ``adjoint()`` uses it there."""
         n = 3 * self.num_nodes(1)
-        U = np.zeros(n)
+        U = np.zeros(n) if copy else None
         info = SolveInfo()
         cap = int(self.params.max_it) + 2
         hist = np.zeros(3 * cap)
-        st = self._lib.gmgs_solve(self._ctx, _dptr(U), C.byref(info), _dptr(hist), cap)
+        st = self._lib.gmgs_solve(self._ctx, _dptr(U) if copy else None, C.byref(info),
+                                  _dptr(hist), cap)
         if st == 9 and not raise_on_nonconvergence:
             warnings.warn("Exceed the maximum iterate numbers")
         else:
@@ -302,6 +311,14 @@ class Solver:
             self._check(self._lib.gmgs_kernel_timing_read(self._ctx, k, C.byref(ms), C.byref(n)))
             out[name] = (ms.value, n.value)
         return out
+
+    def time_operator(self, kernel="matvec64", reps=20):
+        """Mean duration (ms) of ``reps`` back-to-back launches of a level-1
+        FP64 operator ("matvec64" or "jacobi64") on device-resident vectors."""
+        ms = C.c_double()
+        self._check(self._lib.gmgs_time_operator(self._ctx, self.KERNELS[kernel], int(reps),
+                                                 C.byref(ms)))
+        return ms.value
 
     # ---- operator handles (CG_solver(@AtX, @Vcycle, ...))
     def atx(self, x, level=1):

@@ -163,6 +163,13 @@ int gmgs_kernel_timing(gmgs_ctx* c, int enable);
  * to calibrate HBM traffic counters against a kernel of known size. */
 int gmgs_copy_probe(int device, int64_t nbytes, int reps, double* avg_ms);
 int gmgs_kernel_timing_read(gmgs_ctx* c, int kernel, double* avg_ms, int64_t* count);
+/* Benchmark hook (no reference counterpart): launches one level-1 FP64
+ * operator of the PCG iteration reps times back to back on the context's
+ * stream, between one pair of HIP events, and returns the mean duration.
+ * kernel = GMGS_K_MATVEC64 (q = A p with its p'q partials) or
+ * GMGS_K_JACOBI64 (the Jacobi step into scratch vectors).  Needs
+ * gmgs_set_bc; leaves the solution untouched. */
+int gmgs_time_operator(gmgs_ctx* c, int kernel, int reps, double* avg_ms);
 
 int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count);
 int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count);
