@@ -39,6 +39,7 @@ struct Level {
   long long nKs = 0;
   DevBuf Ke64, Ke32, S64, S32;   // universal matrix and interior stencil of this level
   DevBuf C64, C32;               // class coefficients of the stencil
+  DevBuf Kc64, Kc32;             // canonical corner row block of Ke (72)
   bool classes_ok = false;
   DevBuf invd64, invd32;
   DevBuf uniq;                   // int32 compact ids of unique elements (state 2)
@@ -170,6 +171,7 @@ LevelOp<T> make_op(gmgs_ctx* c, int li) {
     op.Ke = l.Ke64.as<double>();
     op.stencil = l.S64.as<double>();
     op.cls = l.C64.empty() ? nullptr : l.C64.as<double>();
+    op.corner = l.Kc64.empty() ? nullptr : l.Kc64.as<double>();
     op.invd = l.invd64.empty() ? nullptr : l.invd64.as<double>();
     op.emod = (li == 0 && c->simp) ? c->emod64.as<double>() : nullptr;
   } else {
@@ -177,6 +179,7 @@ LevelOp<T> make_op(gmgs_ctx* c, int li) {
     op.Ke = l.Ke32.as<float>();
     op.stencil = l.S32.as<float>();
     op.cls = l.C32.empty() ? nullptr : l.C32.as<float>();
+    op.corner = l.Kc32.empty() ? nullptr : l.Kc32.as<float>();
     op.invd = l.invd32.empty() ? nullptr : l.invd32.as<float>();
     op.emod = (li == 0 && c->simp) ? c->emod32.as<float>() : nullptr;
   }
@@ -304,19 +307,24 @@ void upload_level_constants(gmgs_ctx* c, int li) {
   double St[27][3][3];
   interior_stencil(K.data(), St);
   std::memcpy(S.data(), St, sizeof(double) * 243);
-  double cls[30];
-  l.classes_ok = stencil_classes(St, cls, 1e-12);
+  double cls[30], Kc[72];
+  // the z-marching operator needs the stencil's reflection classes and the
+  // element matrix's corner reflection symmetry
+  l.classes_ok = stencil_classes(St, cls, 1e-12) && corner_blocks(K.data(), Kc, 1e-12);
   upload(l.Ke64, K.data(), 576, s);
   upload(l.S64, S.data(), 243, s);
   upload(l.C64, cls, 30, s);
+  upload(l.Kc64, Kc, 72, s);
   if (c->mixed) {
-    std::vector<float> Kf(576), Sf(243), Cf(30);
+    std::vector<float> Kf(576), Sf(243), Cf(30), Kcf(72);
     for (int i = 0; i < 576; ++i) Kf[i] = (float)K[i];
     for (int i = 0; i < 243; ++i) Sf[i] = (float)S[i];
     for (int i = 0; i < 30; ++i) Cf[i] = (float)cls[i];
+    for (int i = 0; i < 72; ++i) Kcf[i] = (float)Kc[i];
     upload(l.Ke32, Kf.data(), 576, s);
     upload(l.S32, Sf.data(), 243, s);
     upload(l.C32, Cf.data(), 30, s);
+    upload(l.Kc32, Kcf.data(), 72, s);
   }
 }
 

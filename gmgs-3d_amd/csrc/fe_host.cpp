@@ -259,4 +259,24 @@ bool stencil_classes(const double S[27][3][3], double cls[30], double tol) {
   return true;
 }
 
+bool corner_blocks(const double* K, double Cc[72], double tol) {
+  double kmax = 0.0;
+  for (int i = 0; i < 576; ++i) kmax = std::max(kmax, std::fabs(K[i]));
+  for (int k = 0; k < 8; ++k)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Cc[(k * 3 + i) * 3 + j] = K[(3 * hex_local(0) + i) * 24 + 3 * hex_local(k) + j];
+  auto sgn = [](int a, int c) {
+    const int bit = c == 0 ? 1 : (c == 1 ? 0 : 2);
+    return ((a >> bit) & 1) ? -1.0 : 1.0;
+  };
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const double ref = sgn(a, i) * sgn(a, j) * Cc[((a ^ b) * 3 + i) * 3 + j];
+          if (std::fabs(K[(3 * hex_local(a) + i) * 24 + 3 * hex_local(b) + j] - ref) > tol * kmax) return false;
+        }
+  return true;
+}
+
 }  // namespace gmgs

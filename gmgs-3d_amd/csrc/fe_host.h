@@ -32,4 +32,12 @@ void interior_stencil(const double* Ke, double S[27][3][3]);
 // of S deviates from that model by more than tol * max|S|.
 bool stencil_classes(const double S[27][3][3], double cls[30], double tol);
 
+// Canonical row block of a box element matrix, Cc[k][i][j] = K[3 L(0) + i][3 L(k) + j]
+// for corner codes k = ro + 2 xo + 4 zo (L = hex_local).  Reflecting the box
+// maps every block onto one of these: K[3 L(a) + i][3 L(b) + j] =
+// s_i(a) s_j(a) Cc[a ^ b][i][j], s_c(a) = -1 when corner a lies on the far side
+// of the axis carrying component c (x: bit 1, y: bit 0, z: bit 2).  Returns
+// false when K deviates from that by more than tol * max|K|.
+bool corner_blocks(const double* K, double Cc[72], double tol);
+
 }  // namespace gmgs

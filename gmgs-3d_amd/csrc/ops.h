@@ -22,6 +22,7 @@ struct LevelOp {
   const T* emod;             // per padded cell modulus factor (level-1 SIMP) or null
   const T* invd;             // inverse diagonal, SoA 3N (zero off the solid nodes)
   const int* colz;           // per padded column (y + py x): solid-node z range lo | hi << 16, or null
+  const T* corner;           // canonical row block of Ke (72): every block is a reflection of one
 };
 
 struct PcgScalars {

@@ -57,7 +57,6 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   // output completion, element corrections)
   __shared__ T sU[4][3][SX][SY];
   __shared__ T sR[SM ? 2 : 1][3][TX][TY];            // SMOOTH: centre values of r
-  __shared__ T sK[576];                              // universal element matrix
   // XCD-contiguous tile order: blocks b = 8 j + k (k = XCD) of the first
   // 8 floor(n / 8) blocks take tile k floor(n / 8) + j; the rest keep their own
   const int nb = (int)gridDim.x, q8 = nb >> 3, bid = (int)blockIdx.x;
@@ -77,7 +76,6 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     own_lo = r & 0xffff;
     own_hi = r >> 16;
   }
-  for (int i = tid; i < 576; i += NT) sK[i] = op.Ke[i];
 
   // this thread's share of a staged plane: global offset (without z) or -1
   // for a position outside the grid or on a column without solid nodes, and
@@ -167,13 +165,15 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     // Element corrections of output zz-1.  An owned node lacking incident
     // element p subtracts that element's universal row block, one touching a
     // stored element p swaps the universal block for the stored one.  The
-    // wave lists its (node, element) pairs, spreads them over its 64 lanes
-    // (72 multiply-adds each), and every owner sums its own pairs in
-    // ascending p.  Done before the next loads are issued and before the
-    // class sums exist, where the fewest registers are live; aM is additive.
-    {
-      uint32_t cm = 0;
-      if (fM & F_ELEM_MASK) cm = ((~fM) & F_ELEM_MASK) | ((fM >> F_STORED_SHIFT) & 0xFFu);
+    // wave lists its (node, element) pairs, spreads them over its 64 lanes,
+    // and every owner sums its own pairs in ascending p.  A universal block
+    // is a reflection of the canonical corner block (uniform coefficients):
+    // reflect the element's displacements, multiply, reflect back.  Done
+    // before the next loads are issued and before the class sums exist, where
+    // the fewest registers are live; aM is additive.
+    uint32_t cm = 0;
+    if (fM & F_ELEM_MASK) cm = ((~fM) & F_ELEM_MASK) | ((fM >> F_STORED_SHIFT) & 0xFFu);
+    if (__ballot(cm != 0u)) {
       const int cnt = __popc(cm);
       int off = cnt;                                   // inclusive prefix over the wave
 #pragma unroll
@@ -183,69 +183,77 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
       }
       const int total = __shfl(off, 63);
       off -= cnt;
-      if (total > 0) {                                 // uniform over the wave
-        int mcnt = cnt;
+      int mcnt = cnt;
 #pragma unroll
-        for (int d = 32; d > 0; d >>= 1) mcnt = max(mcnt, __shfl_xor(mcnt, d));
-        T D0 = 0, D1 = 0, D2 = 0;
-        for (int rb = 0; rb < total; rb += 64) {
-          const int j = rb + lane;
-          int own = 0;                                 // last lane whose first pair is <= j
+      for (int d = 32; d > 0; d >>= 1) mcnt = max(mcnt, __shfl_xor(mcnt, d));
+      T D0 = 0, D1 = 0, D2 = 0;
+      for (int rb = 0; rb < total; rb += 64) {
+        const int j = rb + lane;
+        int own = 0;                                   // last lane whose first pair is <= j
 #pragma unroll
-          for (int st = 32; st > 0; st >>= 1) {
-            const int o = __shfl(off, own + st);
-            if (o <= j) own += st;
-          }
-          const uint32_t cmo = __shfl(cm, own);
-          const uint32_t fo = __shfl(fM, own);
-          const int offo = __shfl(off, own);
-          T e0 = 0, e1 = 0, e2 = 0;
-          if (j < total) {
-            uint32_t mm = cmo;
-            for (int q = j - offo; q > 0; --q) mm &= mm - 1;
-            const int p = __ffs(mm) - 1;
+        for (int st = 32; st > 0; st >>= 1) {
+          const int o = __shfl(off, own + st);
+          if (o <= j) own += st;
+        }
+        const uint32_t cmo = __shfl(cm, own);
+        const uint32_t fo = __shfl(fM, own);
+        const int offo = __shfl(off, own);
+        T e0 = 0, e1 = 0, e2 = 0;
+        if (j < total) {
+          uint32_t mm = cmo;
+          for (int q = j - offo; q > 0; --q) mm &= mm - 1;
+          const int p = __ffs(mm) - 1;
+          const int cn = p ^ 7;                        // corner code of the node in element p
+          const T sgx = (cn & 2) ? (T)-1 : (T)1, sgy = (cn & 1) ? (T)-1 : (T)1,
+                  sgz = (cn & 4) ? (T)-1 : (T)1;
+          const int oly = own % TY + 1, olx = lx_w + own / TY + 1;   // staged coords of the node
+          const T* Kst = nullptr;
+          if (HAS_EOP && ((fo >> p) & 1u)) {
             const int pyb = p & 1, pxb = (p >> 1) & 1, pzb = p >> 2;
-            const int oly = own % TY + 1, olx = lx_w + own / TY + 1;   // staged coords of the node
-            const int ln = node_local_in(p);
-            const T* Kst = nullptr;
-            if (HAS_EOP && ((fo >> p) & 1u)) {
-              const long long no = (t.y0 + oly - 1) + sx * (t.x0 + olx - 1) + sz * (zz - 1);
-              const long long cell = no - (1 - pyb) - (1 - pxb) * sx - (1 - pzb) * sz;
-              Kst = op.Ks + (long long)(op.eop[cell] - 1) * 576;
-            }
-#pragma unroll 1
-            for (int m = 0; m < 8; ++m) {
-              const int dy = hex_ro(m) - (1 - pyb), dx = hex_xo(m) - (1 - pxb), dz = hex_zo(m) - (1 - pzb);
-              const int bz = dz < 0 ? b2 : (dz == 0 ? b1 : kb);
-              const T u0 = sU[bz][0][olx + dx][oly + dy], u1 = sU[bz][1][olx + dx][oly + dy],
-                      u2 = sU[bz][2][olx + dx][oly + dy];
-              const T* k = sK + 3 * ln * 24 + 3 * m;
-              e0 += k[0] * u0 + k[1] * u1 + k[2] * u2;
-              e1 += k[24] * u0 + k[25] * u1 + k[26] * u2;
-              e2 += k[48] * u0 + k[49] * u1 + k[50] * u2;
-              if (HAS_EOP && Kst) {
-                const T* ks = Kst + 3 * ln * 24 + 3 * m;
-                e0 -= ks[0] * u0 + ks[1] * u1 + ks[2] * u2;
-                e1 -= ks[24] * u0 + ks[25] * u1 + ks[26] * u2;
-                e2 -= ks[48] * u0 + ks[49] * u1 + ks[50] * u2;
-              }
+            const long long no = (t.y0 + oly - 1) + sx * (t.x0 + olx - 1) + sz * (zz - 1);
+            const long long cell = no - (1 - pyb) - (1 - pxb) * sx - (1 - pzb) * sz;
+            Kst = op.Ks + (long long)(op.eop[cell] - 1) * 576 + 3 * hex_local(cn) * 24;
+          }
+          const cptr<T> Cc = opaque(op.corner);
+          T h0 = 0, h1 = 0, h2 = 0;                    // stored block times u
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            // canonical corner k is corner k ^ cn of the element
+            const int dy = (cn & 1) ? -(k & 1) : (k & 1);
+            const int dx = (cn & 2) ? -((k >> 1) & 1) : ((k >> 1) & 1);
+            const int dz = (cn & 4) ? -(k >> 2) : (k >> 2);
+            const int bz = dz < 0 ? b2 : (dz == 0 ? b1 : kb);
+            const T u0 = sU[bz][0][olx + dx][oly + dy], u1 = sU[bz][1][olx + dx][oly + dy],
+                    u2 = sU[bz][2][olx + dx][oly + dy];
+            const T v0 = sgx * u0, v1 = sgy * u1, v2 = sgz * u2;
+            e0 += Cc[9 * k + 0] * v0 + Cc[9 * k + 1] * v1 + Cc[9 * k + 2] * v2;
+            e1 += Cc[9 * k + 3] * v0 + Cc[9 * k + 4] * v1 + Cc[9 * k + 5] * v2;
+            e2 += Cc[9 * k + 6] * v0 + Cc[9 * k + 7] * v1 + Cc[9 * k + 8] * v2;
+            if (HAS_EOP && Kst) {
+              const T* ks = Kst + 3 * hex_local(k ^ cn);
+              h0 += ks[0] * u0 + ks[1] * u1 + ks[2] * u2;
+              h1 += ks[24] * u0 + ks[25] * u1 + ks[26] * u2;
+              h2 += ks[48] * u0 + ks[49] * u1 + ks[50] * u2;
             }
           }
-          // owners collect this round's pairs, in order
-          for (int q = 0; q < mcnt; ++q) {
-            const int idx = off + q, src = (idx - rb) & 63;
-            const T v0 = __shfl(e0, src), v1 = __shfl(e1, src), v2 = __shfl(e2, src);
-            if (q < cnt && idx >= rb && idx < rb + 64) {
-              D0 += v0;
-              D1 += v1;
-              D2 += v2;
-            }
+          e0 = sgx * e0 - h0;
+          e1 = sgy * e1 - h1;
+          e2 = sgz * e2 - h2;
+        }
+        // owners collect this round's pairs, in order
+        for (int q = 0; q < mcnt; ++q) {
+          const int idx = off + q, src = (idx - rb) & 63;
+          const T v0 = __shfl(e0, src), v1 = __shfl(e1, src), v2 = __shfl(e2, src);
+          if (q < cnt && idx >= rb && idx < rb + 64) {
+            D0 += v0;
+            D1 += v1;
+            D2 += v2;
           }
         }
-        aM[0] -= D0;
-        aM[1] -= D1;
-        aM[2] -= D2;
       }
+      aM[0] -= D0;
+      aM[1] -= D1;
+      aM[2] -= D2;
     }
     if (zz + 2 <= z1) fetch(zz + 2, sr, sd);
     // flags of the output at zz+1 (only outputs in [z0, z1) are owned)
