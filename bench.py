@@ -3,9 +3,11 @@
 
 One step = one full solve of the synthetic ellipsoid-union elasticity problem
 (337x294x512 voxels, 6 levels, by default) to relative residual 1e-6 with the
-mixed-precision solver (FP32 V-cycle inside an FP64 flexible PCG): b resident
-in HBM, set-up excluded, the solution left on the device.  The FP64 V-cycle
-solve of the same problem is timed the same way beside it.  The level-1 FP64
+mixed-precision solver (FP32 symmetric V(2,2) cycle inside an FP64 flexible
+PCG): b resident in HBM, set-up excluded, the solution left on the device.
+The reference algorithm in FP64 (the baseline of the speed-up), the same
+symmetric cycle in FP64 and the reference cycle in FP32 are timed the same
+way beside it.  The level-1 FP64
 operators (matvec q = A p and the Jacobi step) are then timed with HIP events
 on the solver's stream over back-to-back launches, and one instrumented solve
 gives the per-kernel breakdown.
@@ -117,10 +119,14 @@ def main():
     prob = wl.ellipsoids_c2(args.seed) if args.config == "c2" else wl.ellipsoids_c3(args.seed)
     log(f"[bench] generated {prob.name} in {time.perf_counter() - t0:.1f}s")
     common = dict(num_levels=prob.num_levels, jacobi_weight=prob.omega, tol=1e-6,
-                  max_it=args.max_it, flags=pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE)
+                  max_it=args.max_it)
+    base_flags = pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE
+    sym_flags = base_flags | pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP
 
+    # headline: FP32 symmetric V(2,2) cycle inside the FP64 flexible PCG
     t0 = time.perf_counter()
-    s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, **common)
+    s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, flags=sym_flags,
+                   vcycle_sweeps=2, **common)
     s.set_bc(prob.fixed, prob.loads)
     setup_s = time.perf_counter() - t0
     nn = s.num_nodes(1)
@@ -132,13 +138,24 @@ def main():
     ms_per_step, info = timed_solves(s, args.steps, "mixed")
     value = ndofs / (ms_per_step * 1e-3) / 1e6
 
-    # FP64 V-cycle (reference algorithm, Fletcher-Reeves) on the same problem
-    s64 = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_FP64, **common)
-    s64.set_bc(prob.fixed, prob.loads)
-    U64, _, _ = s64.solve(raise_on_nonconvergence=False)
-    ms64, info64 = timed_solves(s64, args.steps, "fp64")
+    def other(tag, precision, flags):
+        o = pkg.Solver(prob.vox, prob.bbox, precision=precision, flags=flags, vcycle_sweeps=2, **common)
+        o.set_bc(prob.fixed, prob.loads)
+        Uo, _, _ = o.solve(raise_on_nonconvergence=False)
+        ms, inf = timed_solves(o, args.steps, tag)
+        o.close()
+        return Uo, ms, inf
+
+    # the reference algorithm in FP64 (kernel/Vcycle.m cycle, Jacobi step,
+    # Fletcher-Reeves): the baseline of the speed-up
+    U64, ms64, info64 = other("fp64", pkg.GMGS_PREC_FP64, base_flags)
     err = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
-    log(f"[bench] mixed vs fp64 rel L2 {err:.2e}; speed-up {ms64 / ms_per_step:.3f}")
+    # the same symmetric cycle in FP64 (precision-only comparison) and the
+    # reference cycle in FP32
+    _, ms64s, info64s = other("fp64-sym", pkg.GMGS_PREC_FP64, sym_flags)
+    _, msmr, infomr = other("mixed-ref", pkg.GMGS_PREC_MIXED, base_flags)
+    log(f"[bench] mixed vs fp64 rel L2 {err:.2e}; speed-up {ms64 / ms_per_step:.3f} "
+        f"(precision only {ms64s / ms_per_step:.3f})")
 
     # the level-1 FP64 operators, back-to-back launches between one event pair
     mv_ms = s.time_operator("matvec64", args.matvec_reps)
@@ -185,12 +202,18 @@ def main():
         "dtype": "f32 V-cycle / f64 PCG",
         "data": "synthetic (seeded ellipsoid-union voxel solid)",
         "config": {"workload": prob.name, "dofs": ndofs, "levels": s.num_levels,
-                   "rtol": 1e-6, "its_mixed": info.its, "its_fp64": info64.its,
+                   "rtol": 1e-6,
+                   "mixed": "FP32 symmetric V(2,2) cycle, no Jacobi step, FP64 flexible PCG",
+                   "fp64": "reference algorithm: kernel/Vcycle.m V-cycle, Jacobi step, FP64 PCG",
+                   "its_mixed": info.its, "its_fp64": info64.its,
                    "converged_mixed": bool(info.converged), "converged_fp64": bool(info64.converged),
                    "true_relres_mixed": info.true_relres, "true_relres_fp64": info64.true_relres,
                    "fp64_ms": round(ms64, 3),
                    "fp64_mdofs": round(ndofs / (ms64 * 1e-3) / 1e6, 3),
                    "speedup_vs_fp64": round(ms64 / ms_per_step, 3),
+                   "fp64_symmetric_ms": round(ms64s, 3), "its_fp64_symmetric": info64s.its,
+                   "speedup_precision_only": round(ms64s / ms_per_step, 3),
+                   "mixed_reference_cycle_ms": round(msmr, 3), "its_mixed_reference_cycle": infomr.its,
                    "mixed_vs_fp64_rel_l2": err,
                    "setup_s": round(setup_s, 3)},
         "roofline": {"kernel": "k_apply_zm<f64,AX> (level-1 matvec q = A p)", "bound": "hbm",
@@ -208,7 +231,6 @@ def main():
     }
     print(json.dumps(line), flush=True)
     s.close()
-    s64.close()
 
 
 if __name__ == "__main__":

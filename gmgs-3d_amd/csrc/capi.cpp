@@ -42,9 +42,11 @@ struct Level {
   DevBuf Kc64, Kc32;             // canonical corner row block of Ke (72)
   bool classes_ok = false;
   DevBuf invd64, invd32;
+  DevBuf invdm;                  // symmetric cycle: inverse diagonal on the free DOFs (cycle precision)
   DevBuf uniq;                   // int32 compact ids of unique elements (state 2)
   long long nuniq = 0;
   DevBuf r, rp, x;               // V-cycle work vectors (precision of the cycle)
+  DevBuf w1, w2, e;              // symmetric cycle: second sweep, post-smoothing residuals, P x_c
   double scale = 1.0;            // Ke_l = Ke0 * scale
   // z-marching operator (levels with a universal element matrix)
   bool zm = false;
@@ -73,6 +75,9 @@ struct gmgs_ctx {
   bool tables = true;
   bool simple_kernels = false;
   bool autotune = false;
+  bool sym = false;              // symmetric V(nu, nu) cycle
+  int sweeps = 2;
+  bool no_jacobi = false;        // no Jacobi step opening each PCG iteration
   // SIMP
   bool simp = false;
   DevBuf emod64, emod32;
@@ -127,7 +132,7 @@ struct KTime {
   int kernel;
   hipEvent_t a = nullptr;
   KTime(gmgs_ctx* cc, int k) : c(cc), kernel(k) {
-    if (c->ktime) {
+    if (c->ktime && k >= 0) {   // a negative id times nothing
       a = kt_event(c);
       GMGS_HIP(hipEventRecord(a, c->stream));
     }
@@ -158,10 +163,13 @@ template <typename T> const T* Kptr(const Level& l);
 template <> const double* Kptr<double>(const Level& l) { return l.Ke64.as<double>(); }
 template <> const float* Kptr<float>(const Level& l) { return l.Ke32.as<float>(); }
 
+// the operator of level li; 'cyc' selects the symmetric cycle's free-DOF
+// smoother (masked inverse diagonal, fixed outputs zero)
 template <typename T>
-LevelOp<T> make_op(gmgs_ctx* c, int li) {
+LevelOp<T> make_op(gmgs_ctx* c, int li, bool cyc = false) {
   Level& l = c->lev[li];
   LevelOp<T> op;
+  op.zero_fixed = 0;
   op.g = l.mesh.g;
   op.flags = l.mesh.flags.as<uint32_t>();
   op.colz = l.colz.empty() ? nullptr : l.colz.as<int>();
@@ -183,18 +191,31 @@ LevelOp<T> make_op(gmgs_ctx* c, int li) {
     op.invd = l.invd32.empty() ? nullptr : l.invd32.as<float>();
     op.emod = (li == 0 && c->simp) ? c->emod32.as<float>() : nullptr;
   }
+  if (cyc && c->sym && !l.invdm.empty()) {
+    op.invd = l.invdm.as<T>();
+    op.zero_fixed = 1;
+  }
   return op;
 }
 
 // y = A_l u (or the fused smoother) on level li; returns the number of dot partials
+// AX / SMOOTH / RESID (out = in2 - A in) on level li; returns the number of
+// dot partials written
 template <typename T, typename TR>
 int apply_level(gmgs_ctx* c, int li, int mode, const TR* in, T omega, T* out, TR* xacc,
-                double* partial, const int* done) {
+                double* partial, const int* done, const T* in2 = nullptr, bool cyc = false) {
   Level& l = c->lev[li];
-  LevelOp<T> op = make_op<T>(c, li);
+  LevelOp<T> op = make_op<T>(c, li, cyc);
+  if (mode == AP_RESID && !l.zm) {
+    launch_apply<T, TR>(op, l.tiles.as<TileRec>(), l.ntiles, AP_AX, in, omega, out, nullptr, nullptr,
+                        done, c->stream);
+    launch_rsub<T>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, in2, out,
+                   done, c->stream);
+    return 0;
+  }
   if (l.zm) {
     const int np = launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc,
-                                          mode, in, omega, out, partial, done, c->stream);
+                                          mode, in, omega, in2, out, partial, done, c->stream);
     if (xacc)
       launch_jacobi_acc<T, TR>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles,
                                op.invd, in, omega, xacc, c->stream);
@@ -477,11 +498,11 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
   GMGS_HIP(hipEventCreate(&a));
   GMGS_HIP(hipEventCreate(&b));
   for (int w = 0; w < 2; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr, s);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, nullptr, out, nullptr, nullptr, s);
   GMGS_HIP(hipEventRecord(a, s));
   const int reps = 5;
   for (int w = 0; w < reps; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr, s);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, nullptr, out, nullptr, nullptr, s);
   GMGS_HIP(hipEventRecord(b, s));
   GMGS_HIP(hipEventSynchronize(b));
   float ms = 0;
@@ -557,6 +578,11 @@ void alloc_vectors(gmgs_ctx* c) {
       l.r.alloc_zero(es * n3, s);
       l.x.alloc_zero(es * n3, s);
       if (li < c->L - 1) l.rp.alloc_zero(es * n3, s);
+    }
+    if (c->sym && li < c->L - 1) {
+      l.w1.alloc_zero(es * n3, s);
+      l.w2.alloc_zero(es * n3, s);
+      l.e.alloc_zero(es * n3, s);
     }
   }
   const long long n3 = 3 * L1(c).mesh.g.N;
@@ -637,6 +663,73 @@ void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partia
   }
 }
 
+// Symmetric V(nu, nu) cycle (no reference counterpart): nu pre-smoothing
+// sweeps R_s = R_{s-1} - A(omega R_{s-1}/d) from the level's input R_0, the
+// coarse correction e = P x_c, the residual recomputed as Q_0 = R_nu - A e
+// (A x_pre = R_0 - R_nu), nu post-smoothing sweeps Q_s from it, and
+// x = omega (sum_{s<nu} R_s + sum_{s<nu} Q_s)/d + e.  Unlike the stale
+// post-smoothing of kernel/Vcycle.m:27-33 this is a symmetric preconditioner.
+template <typename T>
+void vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
+  hipStream_t s = c->stream;
+  const int L = c->L, nu = c->sweeps;
+  const T omega = (T)c->prm.jacobi_weight;
+  for (int li = 0; li < L - 1; ++li) {
+    Level& l = c->lev[li];
+    {
+      KTime t(c, li == 0 ? 2 : -1);
+      if (li == 0) apply_level<T, double>(c, 0, AP_SMOOTH, rin, omega, vec<T>(l.rp), nullptr, nullptr, done,
+                                        nullptr, true);
+      else apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.r), omega, vec<T>(l.rp), nullptr, nullptr, done,
+                                 nullptr, true);
+      if (nu == 2)
+        apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.rp), omega, vec<T>(l.w1), nullptr, nullptr, done,
+                          nullptr, true);
+    }
+    T* Rnu = nu == 2 ? vec<T>(l.w1) : vec<T>(l.rp);
+    Level& nx = c->lev[li + 1];
+    KTime t(c, li == 0 ? 3 : -1);
+    launch_restrict<T>(nx.mesh.g, nx.mesh.flags.as<uint32_t>(), nx.tiles.as<TileRec>(), nx.ntiles,
+                       l.mesh.g, Rnu, vec<T>(nx.r), done, s);
+  }
+  {
+    Level& lc = c->lev[L - 1];
+    coarsest_apply<T>(c->Kinv.as<double>(), c->nfp, c->nfree, c->free_code.as<long long>(),
+                      vec<T>(lc.r), vec<T>(lc.x), lc.mesh.g.N, c->gv_b.as<double>(),
+                      c->gv_part.as<double>(), done, s);
+  }
+  for (int li = L - 2; li >= 0; --li) {
+    Level& l = c->lev[li];
+    Level& cr = c->lev[li + 1];
+    KTime t(c, li == 0 ? 4 : -1);
+    const Grid& g = l.mesh.g;
+    const uint32_t* fl = l.mesh.flags.as<uint32_t>();
+    const TileRec* tl = l.tiles.as<TileRec>();
+    const T* invd = make_op<T>(c, li, true).invd;
+    T* Rnu = nu == 2 ? vec<T>(l.w1) : vec<T>(l.rp);
+    launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
+    apply_level<T, T>(c, li, AP_RESID, vec<T>(l.e), (T)0, vec<T>(l.w2), nullptr, nullptr, done, Rnu, true);
+    if (nu == 2)
+      apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.w2), omega, vec<T>(l.w1), nullptr, nullptr, done,
+                        nullptr, true);
+    const T* a = nu == 2 ? vec<T>(l.rp) : nullptr;       // R_1
+    const T* b = vec<T>(l.w2);                            // Q_0
+    const T* q1 = nu == 2 ? vec<T>(l.w1) : nullptr;      // Q_1
+    if (li == 0)
+      launch_combine<T, double, T>(g, fl, tl, l.ntiles, invd, omega, rin, a, b, q1, vec<T>(l.e), z, true,
+                                   zprev, partial, done, s);
+    else
+      launch_combine<T, T, T>(g, fl, tl, l.ntiles, invd, omega, vec<T>(l.r), a, b, q1, vec<T>(l.e),
+                              vec<T>(l.x), false, (const T*)nullptr, nullptr, done, s);
+  }
+}
+
+template <typename T>
+void run_vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
+  if (c->sym) vcycle_sym<T>(c, rin, z, zprev, partial, done);
+  else vcycle<T>(c, rin, z, zprev, partial, done);
+}
+
 // ------------------------------------------------------------------ PCG
 template <typename T>
 void pcg_iteration(gmgs_ctx* c, int k) {
@@ -652,8 +745,10 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   c->cur_iter = k;
   KTime whole(c, 5);
   // Jacobi step on (x, r) (CG_solver.m:18-19): r_a = r - A(omega r/d) here,
-  // x += omega r/d in the update of x below (same sum, r is unchanged until then)
-  {
+  // x += omega r/d in the update of x below (same sum, r is unchanged until
+  // then).  Without it the iteration works on r itself.
+  const double* ra = c->no_jacobi ? c->r.as<double>() : c->ra.as<double>();
+  if (!c->no_jacobi) {
     KTime t(c, 1);
     apply_level<double, double>(c, 0, AP_SMOOTH, c->r.as<double>(), c->prm.jacobi_weight,
                                 c->ra.as<double>(), nullptr, nullptr, done);
@@ -661,7 +756,7 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   // z = V(r) with r'z and r'z_prev partials (CG_solver.m:21)
   T* z = c->z[k & 1].as<T>();
   const T* zprev = c->flexible ? c->z[(k + 1) & 1].as<T>() : nullptr;
-  vcycle<T>(c, c->ra.as<double>(), z, zprev, part, done);
+  run_vcycle<T>(c, ra, z, zprev, part, done);
   launch_finalize(1, part, np, sc, nullptr, nullptr, done, s);
   // p = z + beta p (CG_solver.m:23-28)
   launch_update_p<T>(g, fl, tl, l.ntiles, z, c->p.as<double>(), sc, done, s);
@@ -674,9 +769,9 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   }
   launch_finalize(2, part, npq, sc, nullptr, nullptr, done, s);
   // x += alpha p; r = r - alpha q; ||r|| (CG_solver.m:32-39)
-  launch_update_xr(g, fl, tl, l.ntiles, c->x.as<double>(), c->p.as<double>(), c->ra.as<double>(),
-                   c->q.as<double>(), c->r.as<double>(), l.invd64.as<double>(), c->prm.jacobi_weight, sc,
-                   part, done, s);
+  launch_update_xr(g, fl, tl, l.ntiles, c->x.as<double>(), c->p.as<double>(), ra, c->q.as<double>(),
+                   c->r.as<double>(), c->no_jacobi ? nullptr : l.invd64.as<double>(),
+                   c->prm.jacobi_weight, sc, part, done, s);
   launch_finalize(3, part, np, sc, c->hist_rel.as<double>(), c->hist_t.as<unsigned long long>(),
                   done, s);
 }
@@ -843,6 +938,23 @@ void set_bc_impl(gmgs_ctx* c, const int32_t* fixed, int64_t k, const double* loa
     restrict_fixed(cl.mesh.g, cl.tiles.as<TileRec>(), cl.ntiles, cl.mesh.flags.as<uint32_t>(),
                    fl.mesh.g, fl.mesh.flags.as<uint32_t>(), s);
   }
+  // free-DOF inverse diagonals of the symmetric cycle (cycle precision)
+  if (c->sym) {
+    for (int li = 0; li < c->L - 1; ++li) {
+      Level& lv = c->lev[li];
+      const long long n3 = 3 * lv.mesh.g.N;
+      const uint32_t* fg = lv.mesh.flags.as<uint32_t>();
+      if (c->mixed) {
+        lv.invdm.alloc_zero(sizeof(float) * n3, s);
+        launch_mask_fixed<float>(lv.mesh.g, fg, lv.tiles.as<TileRec>(), lv.ntiles, lv.invd32.as<float>(),
+                                 lv.invdm.as<float>(), s);
+      } else {
+        lv.invdm.alloc_zero(sizeof(double) * n3, s);
+        launch_mask_fixed<double>(lv.mesh.g, fg, lv.tiles.as<TileRec>(), lv.ntiles, lv.invd64.as<double>(),
+                                  lv.invdm.as<double>(), s);
+      }
+    }
+  }
   GMGS_HIP(hipStreamSynchronize(s));
   // coarsest: assemble K_ff, factor, invert (SetupSolver.m:257-259)
   Level& lc = c->lev[c->L - 1];
@@ -952,7 +1064,8 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->prm = *p;
     if (!(p->poisson_ratio > -1.0 && p->poisson_ratio < 0.5) || !(p->youngs_modulus > 0) ||
         !(p->tol > 0) || p->max_it < 0 || !(p->jacobi_weight > 0) || p->num_levels < 0 ||
-        (p->precision != GMGS_PREC_FP64 && p->precision != GMGS_PREC_MIXED))
+        (p->precision != GMGS_PREC_FP64 && p->precision != GMGS_PREC_MIXED) ||
+        p->vcycle_sweeps < 0 || p->vcycle_sweeps > 2)
       throw Status(GMGS_ERR_ARGUMENT, "invalid solver parameter");
     c->device = p->device;
     GMGS_HIP(hipSetDevice(c->device));
@@ -967,6 +1080,9 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->simple_kernels = (p->flags & GMGS_FLAG_SIMPLE_KERNELS) != 0;
     c->autotune = (p->flags & GMGS_FLAG_AUTOTUNE) != 0;
     c->use_graphs = (p->flags & GMGS_FLAG_NO_GRAPHS) == 0;
+    c->sym = (p->flags & GMGS_FLAG_SYM_VCYCLE) != 0;
+    c->sweeps = p->vcycle_sweeps == 0 ? 2 : p->vcycle_sweeps;
+    c->no_jacobi = (p->flags & GMGS_FLAG_NO_JACOBI_STEP) != 0;
     c->nely = (int)ny;
     c->nelx = (int)nx;
     c->nelz = (int)nz;
@@ -1448,11 +1564,11 @@ int gmgs_vcycle(gmgs_ctx* c, const double* r, double* z) {
     launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dr.as<double>(), c->tmp.as<double>(), N, s);
     comp.alloc(sizeof(double) * 3 * nn);
     if (c->mixed) {
-      vcycle<float>(c, c->tmp.as<double>(), c->z[0].as<float>(), nullptr, nullptr, nullptr);
+      run_vcycle<float>(c, c->tmp.as<double>(), c->z[0].as<float>(), nullptr, nullptr, nullptr);
       launch_gather_compact<float>(l.mesh.node_pidx.as<long long>(), nn, c->z[0].as<float>(), N,
                                    comp.as<double>(), s);
     } else {
-      vcycle<double>(c, c->tmp.as<double>(), c->z[0].as<double>(), nullptr, nullptr, nullptr);
+      run_vcycle<double>(c, c->tmp.as<double>(), c->z[0].as<double>(), nullptr, nullptr, nullptr);
       launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, c->z[0].as<double>(), N,
                                     comp.as<double>(), s);
     }

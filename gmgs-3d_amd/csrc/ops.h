@@ -23,6 +23,7 @@ struct LevelOp {
   const T* invd;             // inverse diagonal, SoA 3N (zero off the solid nodes)
   const int* colz;           // per padded column (y + py x): solid-node z range lo | hi << 16, or null
   const T* corner;           // canonical row block of Ke (72): every block is a reflection of one
+  int zero_fixed;            // SMOOTH: fixed components of the output are 0 instead of the input's
 };
 
 struct PcgScalars {
@@ -44,7 +45,8 @@ struct PcgScalars {
   unsigned long long t0;
 };
 
-enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1 };
+// AX: out = A in; SMOOTH: out = in - A (omega in / d); RESID: out = in2 - A in
+enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2 };
 
 // ---- launches (all on stream s; 'done' may be null) ----
 template <typename T, typename TR>
@@ -64,12 +66,36 @@ void launch_prolong_smooth(const Grid& gf, const uint32_t* flags_f, const TileRe
                            T omega, TZ* x, bool fix, const TZ* zprev, double* partial,
                            const int* done, hipStream_t s);
 
+// out = a - out on the solid nodes
+template <typename T>
+void launch_rsub(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, const T* a,
+                 T* out, const int* done, hipStream_t s);
+
+// e = P xc on the fine level (prolongation only); fixed components 0 if 'mask'
+template <typename T>
+void launch_prolong(const Grid& gf, const uint32_t* flags_f, const TileRec* tiles, long long nt,
+                    const Grid& gc, const T* xc, T* e, bool mask, const int* done, hipStream_t s);
+
+// dst = src with the fixed components zeroed, on the solid nodes
+template <typename T>
+void launch_mask_fixed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                       const T* src, T* dst, hipStream_t s);
+
+// x = omega ((r0 + a + b + c) / d) + e (null a, b, c skipped), fixed components
+// zeroed if 'fix'; optional partials of r0.x and r0.zprev (FP64)
+template <typename T, typename TR, typename TZ>
+void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                    const T* invd, T omega, const TR* r0, const T* a, const T* b, const T* c,
+                    const T* e, TZ* x, bool fix, const TZ* zprev, double* partial, const int* done,
+                    hipStream_t s);
+
 template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                      const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s);
 
 // x = (x + omega r / d) + alpha p (the Jacobi increment of the iteration, from
-// r before it is replaced), r = r_a - alpha q, partials of r.r
+// r before it is replaced; none when invd is null), r = r_a - alpha q (r_a may
+// be r), partials of r.r
 void launch_update_xr(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                       double* x, const double* p, const double* ra, const double* q, double* r,
                       const double* invd, double omega, const PcgScalars* sc, double* partial,

@@ -153,9 +153,10 @@ __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __r
                + (double)in[2 * N + n] * (double)a2;
       }
     } else {
-      out[n] = (T)in[n] - a0;
-      out[N + n] = (T)in[N + n] - a1;
-      out[2 * N + n] = (T)in[2 * N + n] - a2;
+      const bool z = op.zero_fixed != 0;
+      out[n] = (z && (fl & (1u << (F_FIX_SHIFT + 0)))) ? (T)0 : (T)in[n] - a0;
+      out[N + n] = (z && (fl & (1u << (F_FIX_SHIFT + 1)))) ? (T)0 : (T)in[N + n] - a1;
+      out[2 * N + n] = (z && (fl & (1u << (F_FIX_SHIFT + 2)))) ? (T)0 : (T)in[2 * N + n] - a2;
       if (xacc) {
         xacc[n] += (TR)U(n, 0);
         xacc[N + n] += (TR)U(n, 1);
@@ -246,6 +247,111 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(
   }
 }
 
+// e = P xc on the fine level's solid nodes (the prolongation of
+// kernel/InterpolationDeviation.m:3-17 on the padded grids)
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong(Grid gf, const uint32_t* __restrict__ flags,
+                                                 const TileRec* __restrict__ tiles, long long nt,
+                                                 Grid gc, const T* __restrict__ xc, T* __restrict__ e,
+                                                 int mask, const int* __restrict__ done) {
+  if (done && *done) return;
+  GMGS_TILE_LOOP(gf, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+    const int yf = ln_.yp - 1, xf = ln_.xp - 1, zf = ln_.zp - 1;
+    const int ya = yf >> 1, yb = (yf + 1) >> 1, xa = xf >> 1, xb = (xf + 1) >> 1, za = zf >> 1,
+              zb = (zf + 1) >> 1;
+    const T wy = (yf & 1) ? (T)0.5 : (T)1, wx = (xf & 1) ? (T)0.5 : (T)1,
+            wz = (zf & 1) ? (T)0.5 : (T)1;
+    const long long Nc = gc.N, N = gf.N, n = ln_.n;
+    T p[3] = {0, 0, 0};
+    const int ys[2] = {ya, yb}, xs[2] = {xa, xb}, zs[2] = {za, zb};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int cy = ys[k & 1], cx = xs[(k >> 1) & 1], cz = zs[k >> 2];
+      const T w = (((k & 1) && !(yf & 1)) ? (T)0 : wy) * ((((k >> 1) & 1) && !(xf & 1)) ? (T)0 : wx)
+                  * (((k >> 2) && !(zf & 1)) ? (T)0 : wz);
+      const long long m = gc.pidx(cy, cx, cz);
+      p[0] += w * xc[m];
+      p[1] += w * xc[Nc + m];
+      p[2] += w * xc[2 * Nc + m];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e[c * N + n] = (mask && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) ? (T)0 : p[c];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mask_fixed(Grid g, const uint32_t* __restrict__ flags,
+                                                    const TileRec* __restrict__ tiles, long long nt,
+                                                    const T* __restrict__ src, T* __restrict__ dst) {
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      dst[c * N + ln_.n] = (ln_.fl & (1u << (F_FIX_SHIFT + c))) ? (T)0 : src[c * N + ln_.n];
+  }
+}
+
+// x = omega ((r0 + a + b + c) / d) + e on the solid nodes (null a, b, c
+// skipped); zero fixed components if 'fix'; partials of r0.x and r0.zprev
+template <typename T, typename TR, typename TZ>
+__global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restrict__ flags,
+                                                 const TileRec* __restrict__ tiles, long long nt,
+                                                 const T* __restrict__ invd, T omega,
+                                                 const TR* __restrict__ r0, const T* __restrict__ a,
+                                                 const T* __restrict__ b, const T* __restrict__ cc,
+                                                 const T* __restrict__ e, TZ* __restrict__ x, int fix,
+                                                 const TZ* __restrict__ zprev,
+                                                 double* __restrict__ partial,
+                                                 const int* __restrict__ done) {
+  if (done && *done) return;
+  double d0 = 0.0, d1 = 0.0;
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+    const long long n = ln_.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long m = c * N + n;
+      T sum = (T)r0[m];
+      if (a) sum += a[m];
+      if (b) sum += b[m];
+      if (cc) sum += cc[m];
+      T v = omega * invd[m] * sum + e[m];
+      if (fix && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) v = 0;
+      x[m] = (TZ)v;
+      if (partial) {
+        d0 += (double)r0[m] * (double)(TZ)v;
+        if (zprev) d1 += (double)r0[m] * (double)zprev[m];
+      }
+    }
+  }
+  if (partial) {
+    const double b0 = block_sum(d0);
+    const double b1 = block_sum(d1);
+    if (threadIdx.x == 0) {
+      partial[blockIdx.x] = b0;
+      partial[gridDim.x + blockIdx.x] = b1;
+    }
+  }
+}
+
+// out = a - out on the solid nodes
+template <typename T>
+__global__ __launch_bounds__(256) void k_rsub(Grid g, const uint32_t* __restrict__ flags,
+                                              const TileRec* __restrict__ tiles, long long nt,
+                                              const T* __restrict__ a, T* __restrict__ out,
+                                              const int* __restrict__ done) {
+  if (done && *done) return;
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c * N + ln_.n] = a[c * N + ln_.n] - out[c * N + ln_.n];
+  }
+}
+
 template <typename TZ>
 __global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __restrict__ flags,
                                                   const TileRec* __restrict__ tiles, long long nt,
@@ -264,13 +370,14 @@ __global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __rest
 }
 
 // x = (x + omega r / d) + alpha p with the iteration's Jacobi increment
-// taken from r before it is replaced; r = r_a - alpha q; partials of r.r
+// taken from r before it is replaced (no increment when invd is null);
+// r = r_a - alpha q (r_a may be r itself); partials of r.r
 __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __restrict__ flags,
                                                    const TileRec* __restrict__ tiles, long long nt,
                                                    double* __restrict__ x, const double* __restrict__ p,
-                                                   const double* __restrict__ ra,
-                                                   const double* __restrict__ q, double* __restrict__ r,
-                                                   const double* __restrict__ invd, double omega,
+                                                   const double* ra, const double* __restrict__ q,
+                                                   double* r, const double* __restrict__ invd,
+                                                   double omega,
                                                    const PcgScalars* __restrict__ sc,
                                                    double* __restrict__ partial,
                                                    const int* __restrict__ done) {
@@ -282,8 +389,12 @@ __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __res
     const long long N = g.N, n = ln_.n;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double u = omega * invd[c * N + n] * r[c * N + n];
-      x[c * N + n] = (x[c * N + n] + u) + alpha * p[c * N + n];
+      if (invd) {
+        const double u = omega * invd[c * N + n] * r[c * N + n];
+        x[c * N + n] = (x[c * N + n] + u) + alpha * p[c * N + n];
+      } else {
+        x[c * N + n] += alpha * p[c * N + n];
+      }
       const double rv = ra[c * N + n] - alpha * q[c * N + n];
       r[c * N + n] = rv;
       d += rv * rv;
@@ -576,6 +687,65 @@ void launch_prolong_smooth(const Grid& gf, const uint32_t* flags_f, const TileRe
                                                                partial, done);
   GMGS_LAUNCH_CHECK();
 }
+
+template <typename T>
+void launch_rsub(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, const T* a,
+                 T* out, const int* done, hipStream_t s) {
+  if (nt <= 0) return;
+  k_rsub<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, out, done);
+  GMGS_LAUNCH_CHECK();
+}
+template void launch_rsub<double>(const Grid&, const uint32_t*, const TileRec*, long long, const double*,
+                                  double*, const int*, hipStream_t);
+template void launch_rsub<float>(const Grid&, const uint32_t*, const TileRec*, long long, const float*,
+                                 float*, const int*, hipStream_t);
+
+template <typename T>
+void launch_prolong(const Grid& gf, const uint32_t* flags_f, const TileRec* tiles, long long nt,
+                    const Grid& gc, const T* xc, T* e, bool mask, const int* done, hipStream_t s) {
+  if (nt <= 0) return;
+  k_prolong<T><<<pointwise_blocks(nt), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, e, mask ? 1 : 0, done);
+  GMGS_LAUNCH_CHECK();
+}
+template void launch_prolong<double>(const Grid&, const uint32_t*, const TileRec*, long long, const Grid&,
+                                     const double*, double*, bool, const int*, hipStream_t);
+template void launch_prolong<float>(const Grid&, const uint32_t*, const TileRec*, long long, const Grid&,
+                                    const float*, float*, bool, const int*, hipStream_t);
+
+template <typename T>
+void launch_mask_fixed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                       const T* src, T* dst, hipStream_t s) {
+  if (nt <= 0) return;
+  k_mask_fixed<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, src, dst);
+  GMGS_LAUNCH_CHECK();
+}
+template void launch_mask_fixed<double>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                        const double*, double*, hipStream_t);
+template void launch_mask_fixed<float>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                       const float*, float*, hipStream_t);
+
+template <typename T, typename TR, typename TZ>
+void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                    const T* invd, T omega, const TR* r0, const T* a, const T* b, const T* c,
+                    const T* e, TZ* x, bool fix, const TZ* zprev, double* partial, const int* done,
+                    hipStream_t s) {
+  if (nt <= 0) return;
+  k_combine<T, TR, TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, omega, r0, a, b, c,
+                                                             e, x, fix ? 1 : 0, zprev, partial, done);
+  GMGS_LAUNCH_CHECK();
+}
+template void launch_combine<double, double, double>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                                     const double*, double, const double*, const double*,
+                                                     const double*, const double*, const double*, double*,
+                                                     bool, const double*, double*, const int*, hipStream_t);
+template void launch_combine<float, float, float>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                                  const float*, float, const float*, const float*,
+                                                  const float*, const float*, const float*, float*, bool,
+                                                  const float*, double*, const int*, hipStream_t);
+template void launch_combine<float, double, float>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                                   const float*, float, const double*, const float*,
+                                                   const float*, const float*, const float*, float*, bool,
+                                                   const float*, double*, const int*, hipStream_t);
 
 template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,

@@ -17,11 +17,12 @@ long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, D
 // as lo | hi << 16; an empty column holds lo = 0xffff, hi = 0
 void build_colrange(const Grid& g, const uint32_t* flags, DevBuf& colz, hipStream_t s);
 
-// same contract as launch_apply without the x += u accumulation; returns the
-// number of dot partials written (one per tile)
+// same contract as launch_apply without the x += u accumulation, plus the
+// RESID mode (out = in2 - A in); returns the number of dot partials written
+// (one per tile)
 template <typename T, typename TR>
 int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
-                    int mode, const TR* in, T omega, T* out, double* partial, const int* done,
-                    hipStream_t s);
+                    int mode, const TR* in, T omega, const T* in2, T* out, double* partial,
+                    const int* done, hipStream_t s);
 
 }  // namespace gmgs

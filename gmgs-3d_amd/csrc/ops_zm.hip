@@ -45,7 +45,8 @@ __device__ inline cptr<T> opaque(const T* p) {
 template <typename T, typename TR, int MODE, bool HAS_EOP, int TX>
 __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTile* __restrict__ tiles,
                                                       int zc, const TR* __restrict__ in, T omega,
-                                                      T* __restrict__ out, double* __restrict__ partial,
+                                                      const T* __restrict__ in2, T* __restrict__ out,
+                                                      double* __restrict__ partial,
                                                       const int* __restrict__ done) {
   constexpr int TY = 16, NT = TY * TX, SY = TY + 2, SX = TX + 2;
   constexpr int NSTAGE = 3 * SX * SY;
@@ -162,6 +163,14 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     }
     // planes zz-2, zz-1, zz are staged in buffers b2, b1, kb
     const int b1 = (kb + 3) & 3, b2 = (kb + 2) & 3;
+    // RESID: centre values of in2 at the output completed in this plane,
+    // loaded now and used after the stencil
+    T cen[3] = {0, 0, 0};
+    if (MODE == AP_RESID && lane_ok && (fM & F_ELEM_MASK)) {
+      const long long n = colidx + sz * (zz - 1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) cen[c] = in2[c * N + n];
+    }
     // Element corrections of output zz-1.  An owned node lacking incident
     // element p subtracts that element's universal row block, one touching a
     // stored element p swaps the universal block for the stored one.  The
@@ -315,8 +324,11 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
         if (MODE == AP_AX) {
           out[c * N + n] = a;
           if (partial) dsum += (double)sU[b1][c][lx + 1][ly + 1] * (double)a;
+        } else if (MODE == AP_RESID) {
+          out[c * N + n] = cen[c] - a;
         } else {
-          out[c * N + n] = sR[zo_ & 1][c][lx][ly] - a;
+          const bool fx = (fM & (1u << (F_FIX_SHIFT + c))) != 0;
+          out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : sR[zo_ & 1][c][lx][ly] - a;
         }
       }
     }
@@ -421,14 +433,14 @@ long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, D
 
 template <typename T, typename TR>
 int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
-                    int mode, const TR* in, T omega, T* out, double* partial, const int* done,
-                    hipStream_t s) {
+                    int mode, const TR* in, T omega, const T* in2, T* out, double* partial,
+                    const int* done, hipStream_t s) {
   if (ntiles <= 0) return 0;
   if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("grid too large for the z-marching operator");
   const bool eop = op.eop != nullptr;
 #define GMGS_ZM(MODE, E, TXV)                                                                        \
-  k_apply_zm<T, TR, MODE, E, TXV><<<(unsigned)ntiles, 16 * TXV, 0, s>>>(op, tiles, zc, in, omega, out, \
-                                                                        partial, done)
+  k_apply_zm<T, TR, MODE, E, TXV><<<(unsigned)ntiles, 16 * TXV, 0, s>>>(op, tiles, zc, in, omega, in2, \
+                                                                        out, partial, done)
 #define GMGS_ZM_TX(MODE, E)               \
   do {                                    \
     if (tx == 4) GMGS_ZM(MODE, E, 4);     \
@@ -438,6 +450,9 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   if (mode == AP_AX) {
     if (eop) GMGS_ZM_TX(AP_AX, true);
     else GMGS_ZM_TX(AP_AX, false);
+  } else if (mode == AP_RESID) {
+    if (eop) GMGS_ZM_TX(AP_RESID, true);
+    else GMGS_ZM_TX(AP_RESID, false);
   } else {
     if (eop) GMGS_ZM_TX(AP_SMOOTH, true);
     else GMGS_ZM_TX(AP_SMOOTH, false);
@@ -449,13 +464,13 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
 }
 
 template int launch_apply_zm<double, double>(const LevelOp<double>&, const ColTile*, long long, int, int,
-                                             int, const double*, double, double*, double*, const int*,
-                                             hipStream_t);
+                                             int, const double*, double, const double*, double*, double*,
+                                             const int*, hipStream_t);
 template int launch_apply_zm<float, float>(const LevelOp<float>&, const ColTile*, long long, int, int,
-                                           int, const float*, float, float*, double*, const int*,
-                                           hipStream_t);
+                                           int, const float*, float, const float*, float*, double*,
+                                           const int*, hipStream_t);
 template int launch_apply_zm<float, double>(const LevelOp<float>&, const ColTile*, long long, int, int,
-                                            int, const double*, float, float*, double*, const int*,
-                                            hipStream_t);
+                                            int, const double*, float, const float*, float*, double*,
+                                            const int*, hipStream_t);
 
 }  // namespace gmgs

@@ -32,6 +32,8 @@ FLAG_PR_BETA = 8
 FLAG_SIMPLE_KERNELS = 16
 FLAG_AUTOTUNE = 32
 FLAG_NO_GRAPHS = 64
+FLAG_SYM_VCYCLE = 128
+FLAG_NO_JACOBI_STEP = 256
 
 Q = dict(NUM_LEVELS=0, RES=1, NUM_ELEMENTS=2, NUM_NODES=3, ELE_MAP_BACK=4, ELE_MAP_FORWARD=5,
          NOD_MAP_BACK=6, NOD_MAP_FORWARD=7, ENODMAT=8, NUM_NOD2ELES=9, NODES_ON_BOUNDARY=10,
@@ -63,7 +65,8 @@ class Params(C.Structure):
     _fields_ = [("youngs_modulus", C.c_double), ("poisson_ratio", C.c_double), ("tol", C.c_double),
                 ("jacobi_weight", C.c_double), ("coarsest_resolution_control", C.c_double),
                 ("max_it", C.c_int32), ("num_levels", C.c_int32), ("precision", C.c_int32),
-                ("flags", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("flags", C.c_int32), ("device", C.c_int32), ("vcycle_sweeps", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class SolveInfo(C.Structure):

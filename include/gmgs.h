@@ -62,7 +62,12 @@ enum {
   GMGS_FLAG_SIMPLE_KERNELS = 16,  /* one-wave-per-tile element-sum operator everywhere
                                      (no z-marching stencil kernel); for cross-checks */
   GMGS_FLAG_AUTOTUNE = 32,        /* time column-tile shapes per level at set-up */
-  GMGS_FLAG_NO_GRAPHS = 64        /* launch PCG iterations one kernel at a time */
+  GMGS_FLAG_NO_GRAPHS = 64,       /* launch PCG iterations one kernel at a time */
+  GMGS_FLAG_SYM_VCYCLE = 128,     /* symmetric V(nu, nu) cycle: residual recomputed after the
+                                     coarse correction, nu = vcycle_sweeps (no reference
+                                     counterpart; kernel/Vcycle.m:27-33 reuses the stale one) */
+  GMGS_FLAG_NO_JACOBI_STEP = 256  /* omit the Jacobi step opening every PCG iteration
+                                     (CG_solver.m:18-19) */
 };
 
 typedef struct gmgs_params {
@@ -76,7 +81,8 @@ typedef struct gmgs_params {
   int32_t precision;                   /* GMGS_PREC_FP64 or GMGS_PREC_MIXED */
   int32_t flags;                       /* GMGS_FLAG_* */
   int32_t device;                      /* HIP device ordinal */
-  int32_t reserved[7];
+  int32_t vcycle_sweeps;               /* nu of GMGS_FLAG_SYM_VCYCLE: 1 or 2 (0 = 2) */
+  int32_t reserved[6];
 } gmgs_params;
 
 typedef struct gmgs_solve_info {

@@ -697,6 +697,61 @@ def vcycle(model: Model, r: np.ndarray) -> np.ndarray:
     return d
 
 
+def vcycle_sym(model: Model, r: np.ndarray, nu: int = 2) -> np.ndarray:
+    """Symmetric V(nu, nu) cycle on the free DOFs (no reference counterpart).
+
+    The levels, transfers, Jacobi weight and coarsest solve of
+    kernel/Vcycle.m:1-36, with three changes: each level works on its free
+    DOFs only (smoothing increments, smoothed residuals and the prolonged
+    correction are zero on the level's fixed DOFs), the post-smoothing starts
+    from the residual recomputed after the coarse correction (Vcycle.m:27-33
+    reuses the pre-smoothed one), and each level smooths nu times on the way
+    down and up.  Of the free-DOF restriction only the zero smoothing weight on
+    fixed DOFs changes the result: a coarse DOF is fixed whenever a fine fixed
+    DOF lies in its support (SetupSolver.m:247-254), so fixed-DOF residual
+    values only ever reach coarse fixed DOFs and P x_c vanishes on fine fixed
+    DOFs; the other two masks are kept as plain statements of the intent.
+    In the form the device evaluates it:
+    R_s = R_{s-1} - A (omega R_{s-1}/d), e = P x_c, Q_0 = R_nu - A e,
+    Q_s = Q_{s-1} - A (omega Q_{s-1}/d), x = omega (sum_{s<nu} R_s + Q_s)/d + e.
+    """
+    Lc = model.numLevels
+    rin = [None] * Lc
+    Rs = [None] * Lc
+    xs = [None] * Lc
+    rin[0] = r
+
+    def fixed(ii):
+        return model.levels[ii - 1].fixedDOFs - 1
+
+    def invd(ii):
+        w = model.omega / model.levels[ii - 1].diagK
+        w[fixed(ii)] = 0.0
+        return w
+
+    def smooth(rr, ii):
+        out = rr - atx(model, invd(ii) * rr, ii)
+        out[fixed(ii)] = 0.0
+        return out
+
+    for ii in range(1, Lc):
+        R = [rin[ii - 1]]
+        for _ in range(nu):
+            R.append(smooth(R[-1], ii))
+        Rs[ii - 1] = R
+        rin[ii] = restrict(model, R[nu], ii + 1)
+    xs[Lc - 1] = coarsest_solve(model, rin[Lc - 1])
+    for ii in range(Lc - 1, 0, -1):
+        R = Rs[ii - 1]
+        e = interpolate(model, xs[ii], ii + 1)
+        e[fixed(ii)] = 0.0
+        Q = [R[nu] - atx(model, e, ii)]
+        for _ in range(nu - 1):
+            Q.append(smooth(Q[-1], ii))
+        xs[ii - 1] = invd(ii) * (sum(R[:nu]) + sum(Q)) + e
+    return xs[0]
+
+
 @dataclasses.dataclass
 class CGResult:
     x: np.ndarray
@@ -707,9 +762,13 @@ class CGResult:
 
 
 def cg_solver(model: Model, b: np.ndarray, tol: float, max_it: int,
-              flexible: bool = False) -> CGResult:
+              flexible: bool = False, jacobi_step: bool = True, cycle=None) -> CGResult:
     """kernel/CG_solver.m:1-56.  ``flexible=True`` swaps in the Polak-Ribiere
-    beta used by the mixed-precision mode (no reference counterpart)."""
+    beta, ``jacobi_step=False`` drops the Jacobi step of CG_solver.m:18-19 and
+    ``cycle`` replaces the V-cycle (e.g. vcycle_sym); none of these has a
+    reference counterpart."""
+    if cycle is None:
+        cycle = vcycle
     n = b.size
     normB = np.linalg.norm(b)
     its = 0
@@ -723,8 +782,9 @@ def cg_solver(model: Model, b: np.ndarray, tol: float, max_it: int,
     resnorm = np.inf
     converged = False
     while its <= max_it:
-        x, r1 = jacobi(model, x, r1, 1)
-        z2 = vcycle(model, r1)
+        if jacobi_step:
+            x, r1 = jacobi(model, x, r1, 1)
+        z2 = cycle(model, r1)
         its += 1
         if its == 1:
             p2 = z2

@@ -268,3 +268,51 @@ def test_determinism_graphs_and_autotune(pkg):
         Ug, Ut = out[("graph", prec)][0], out[("tuned", prec)][0]
         assert np.linalg.norm(Ug - Ut) / np.linalg.norm(Ug) < 1e-9
         assert abs(out[("graph", prec)][1] - out[("tuned", prec)][1]) <= 1
+
+
+# ---- symmetric V(nu, nu) cycle without the per-iteration Jacobi step ---------
+
+def sym_solver(case, pkg, precision, nu=2):
+    m = case["m"]
+    s = pkg.Solver(case["vox"], case["bbox"], num_levels=m.numLevels, jacobi_weight=case["omega"],
+                   tol=1e-8, max_it=2000, precision=precision, vcycle_sweeps=nu,
+                   flags=pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP)
+    s.set_bc(case["fixed"], case["loads"])
+    return s
+
+
+@pytest.mark.parametrize("nu", [1, 2])
+def test_symmetric_vcycle_matches_oracle(case, oracle, pkg, nu):
+    m = case["m"]
+    s2 = sym_solver(case, pkg, pkg.GMGS_PREC_FP64, nu)
+    rng = np.random.default_rng(12)
+    r = rng.standard_normal(m.levels[0].numDOFs)
+    z, zr = s2.vcycle(r), oracle.vcycle_sym(m, r, nu)
+    log(f"  sym vcycle nu={nu} rel {rel(z, zr):.2e}")
+    assert rel(z, zr) < 1e-9
+    fx = m.levels[0].fixedDOFs - 1
+    a, b = rng.standard_normal(r.size), rng.standard_normal(r.size)
+    a[fx] = 0.0
+    b[fx] = 0.0
+    Ma, Mb = s2.vcycle(a), s2.vcycle(b)
+    assert abs(Ma @ b - a @ Mb) <= 1e-12 * abs(Ma @ b)
+    s2.close()
+
+
+def test_symmetric_cycle_solves(case, oracle, pkg):
+    m = case["m"]
+    s2 = sym_solver(case, pkg, pkg.GMGS_PREC_FP64)
+    U, info, _ = s2.solve(raise_on_nonconvergence=False)
+    res = oracle.cg_solver(m, m.F, 1e-8, 2000, jacobi_step=False,
+                           cycle=lambda mm, r: oracle.vcycle_sym(mm, r, 2))
+    e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
+    log(f"  sym fp64: gpu its {info.its} oracle its {res.its} err {e:.2e}")
+    assert abs(info.its - res.its) <= 1 and e < 1e-6
+    s2.close()
+    s3 = sym_solver(case, pkg, pkg.GMGS_PREC_MIXED)
+    U3, info3, _ = s3.solve(raise_on_nonconvergence=False)
+    ref = direct_solution(m, oracle)
+    e3 = np.linalg.norm(U3 - ref) / np.linalg.norm(ref)
+    log(f"  sym mixed: its {info3.its} true {info3.true_relres:.2e} err {e3:.2e}")
+    assert info3.converged and info3.true_relres < 1e-7 and e3 < 1e-5
+    s3.close()

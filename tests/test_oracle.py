@@ -417,3 +417,60 @@ def test_stress_patch(oracle, pkg):
                     A[0, 1] + A[1, 0]])
     sig = oracle.hooke(1.0, 0.3) @ eps
     assert np.abs(S - sig[None, :]).max() < 1e-13
+
+
+# ---- symmetric V(nu, nu) cycle (no reference counterpart) -------------------
+
+def _free_random(m, rng):
+    v = rng.standard_normal(m.levels[0].numDOFs)
+    v[m.levels[0].fixedDOFs - 1] = 0.0
+    return v
+
+
+@pytest.mark.parametrize("nu", [1, 2])
+def test_symmetric_vcycle_is_symmetric_and_reference_is_not(oracle, nu):
+    m = holey_brick(oracle)
+    rng = np.random.default_rng(5)
+    a, b = _free_random(m, rng), _free_random(m, rng)
+    Ma, Mb = oracle.vcycle_sym(m, a, nu), oracle.vcycle_sym(m, b, nu)
+    assert abs(Ma @ b - a @ Mb) <= 1e-12 * abs(Ma @ b)
+    assert a @ Ma > 0 and b @ Mb > 0
+    # kernel/Vcycle.m:27-33 post-smooths with the stale residual: not symmetric
+    Ra, Rb = oracle.vcycle(m, a), oracle.vcycle(m, b)
+    assert abs(Ra @ b - a @ Rb) > 1e-8 * abs(Ra @ b)
+
+
+@pytest.mark.parametrize("nu", [1, 2])
+def test_symmetric_vcycle_two_level_error_propagation(oracle, nu):
+    """I - M A = (I - S A)^nu (I - P A_c^-1 P^T A) (I - S A)^nu on the free DOFs,
+    with S = omega/diag, built from dense matrices of the level operators."""
+    m = holey_brick(oracle, L=2, shape=(4, 4, 6), seed=3)
+    L1, L2 = m.levels
+    F = L1.freeDOFs - 1
+    Fc = L2.freeDOFs - 1
+    n, nc = L1.numDOFs, L2.numDOFs
+
+    def unit(k, size):
+        v = np.zeros(size)
+        v[k] = 1.0
+        return v
+
+    A = np.stack([oracle.atx(m, unit(j, n), 1)[F] for j in F], axis=1)
+    P = np.stack([oracle.interpolate(m, unit(k, nc), 2)[F] for k in Fc], axis=1)
+    Ac_inv = np.stack([oracle.coarsest_solve(m, unit(k, nc))[Fc] for k in Fc], axis=1)
+    S = np.diag(m.omega / L1.diagK[F])
+    I = np.eye(F.size)
+    Sm = np.linalg.matrix_power(I - S @ A, nu)
+    E = Sm @ (I - P @ Ac_inv @ P.T @ A) @ Sm
+    M_formula = (I - E) @ np.linalg.inv(A)
+    M = np.stack([oracle.vcycle_sym(m, unit(j, n), nu)[F] for j in F], axis=1)
+    assert np.abs(M - M_formula).max() <= 1e-9 * np.abs(M_formula).max()
+
+
+def test_symmetric_cycle_cg_converges_faster(oracle):
+    m = holey_brick(oracle)
+    ref = oracle.cg_solver(m, m.F, 1e-10, 500)
+    sym = oracle.cg_solver(m, m.F, 1e-10, 500, jacobi_step=False,
+                           cycle=lambda mm, r: oracle.vcycle_sym(mm, r, 2))
+    assert sym.converged and sym.its < ref.its
+    assert np.linalg.norm(sym.x - ref.x) / np.linalg.norm(ref.x) < 1e-8
