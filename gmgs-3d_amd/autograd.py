@@ -30,6 +30,13 @@ def objective_and_gradient(solver, rho: np.ndarray, g: np.ndarray, w_f: float = 
     return J, dJdE * simp_dmodulus(rho), U
 
 
+def objective(solver, rho: np.ndarray, g: np.ndarray, w_f: float = 1.0) -> float:
+    """J = w_f f'u + g'u alone (no adjoint solve), e.g. for finite differences."""
+    solver.set_modulus(simp_modulus(rho))
+    U, _, _ = solver.solve()
+    return w_f * float(np.dot(solver.query("F"), U)) + float(np.dot(g, U))
+
+
 def make_function():
     import torch
 

@@ -471,8 +471,9 @@ void build_operators(gmgs_ctx* c) {
 }
 
 // ------------------------------------------------------------------ autotuner
-// per-level choice of the z-marching column tile (TX x ZC), timed on the
-// level's own grid with the V-cycle smoother; cached per (grid, precision)
+// per-level choice of the z-marching column tile (TX x ZC), or of the tile
+// kernel on coarse levels, timed on the level's own grid with the V-cycle
+// smoother; cached per (grid, precision)
 struct TuneKey {
   int ey, ex, ez, prec;
   bool operator<(const TuneKey& o) const {
@@ -486,6 +487,31 @@ std::map<TuneKey, std::pair<int, int>>& tune_cache() {
 std::mutex& tune_mutex() {
   static std::mutex m;
   return m;
+}
+
+template <typename T, typename TR>
+float time_v1(gmgs_ctx* c, int li, const TR* in, T* out) {
+  Level& l = c->lev[li];
+  hipStream_t s = c->stream;
+  LevelOp<T> op = make_op<T>(c, li);
+  hipEvent_t a, b;
+  GMGS_HIP(hipEventCreate(&a));
+  GMGS_HIP(hipEventCreate(&b));
+  for (int w = 0; w < 2; ++w)
+    launch_apply<T, TR>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr,
+                        nullptr, s);
+  GMGS_HIP(hipEventRecord(a, s));
+  const int reps = 5;
+  for (int w = 0; w < reps; ++w)
+    launch_apply<T, TR>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr,
+                        nullptr, s);
+  GMGS_HIP(hipEventRecord(b, s));
+  GMGS_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  GMGS_HIP(hipEventElapsedTime(&ms, a, b));
+  GMGS_HIP(hipEventDestroy(a));
+  GMGS_HIP(hipEventDestroy(b));
+  return ms / reps;
 }
 
 template <typename T, typename TR>
@@ -551,8 +577,19 @@ void autotune_levels(gmgs_ctx* c) {
         }
         if (ms < best_ms) { best_ms = ms; best = {cd[0], cd[1]}; }
       }
+      // coarse levels (small, many stored elements) may run faster on the tile
+      // kernel; recorded as tile (0, 0)
+      if (li > 0) {
+        const float ms = c->mixed ? time_v1<float, float>(c, li, l.r.as<float>(), l.rp.as<float>())
+                                  : time_v1<double, double>(c, li, l.r.as<double>(), l.rp.as<double>());
+        if (ms < best_ms) { best_ms = ms; best = {0, 0}; }
+      }
       std::lock_guard<std::mutex> lk(tune_mutex());
       tune_cache()[key] = best;
+    }
+    if (best.first == 0) {
+      l.zm = false;
+      continue;
     }
     if (best.first != l.col_tx || best.second != l.col_zc) {
       l.col_tx = best.first;
@@ -841,36 +878,53 @@ LoopResult pcg_run(gmgs_ctx* c) {
   GMGS_HIP(hipEventCreate(&e1));
   GMGS_HIP(hipEventRecord(e0, s));
   const int maxk = c->prm.max_it + 1;
-  int k = 0, chunk = 2, slot = 0, pending = -1;
+  int k = 0;   // iterations launched
   PcgScalars h{};
-  // keep one chunk in flight while the host reads the previous chunk's flag
-  while (true) {
-    if (k < maxk) {
-      const int n = std::min(chunk, maxk - k + (graph ? 1 : 0));
-      if (graph) {
-        for (int i = 0; i < n; i += 2, k += 2) GMGS_HIP(hipGraphLaunch(c->gexec, s));
-      } else {
-        for (int i = 0; i < n; ++i, ++k) {
-          if (c->mixed) pcg_iteration<float>(c, k);
-          else pcg_iteration<double>(c, k);
-        }
+  // queue n iterations (even with graphs: one replay is an iteration pair) and
+  // a read-back of the scalars into pinned slot sl
+  auto launch = [&](int n, int sl) -> bool {
+    if (k >= maxk) return false;
+    n = std::min(n, maxk - k + (graph ? 1 : 0));
+    if (graph) {
+      for (int i = 0; i < n; i += 2, k += 2) GMGS_HIP(hipGraphLaunch(c->gexec, s));
+    } else {
+      for (int i = 0; i < n; ++i, ++k) {
+        if (c->mixed) pcg_iteration<float>(c, k);
+        else pcg_iteration<double>(c, k);
       }
-      GMGS_HIP(hipMemcpyAsync(&c->pinned[slot], c->sc.as<void>(), sizeof(PcgScalars),
-                              hipMemcpyDeviceToHost, s));
-      GMGS_HIP(hipEventRecord(c->poll_ev[slot], s));
-      chunk = std::min(chunk * 2, 16);
     }
-    if (pending >= 0) {
-      GMGS_HIP(hipEventSynchronize(c->poll_ev[pending]));
-      if (c->pinned[pending].done) break;
+    GMGS_HIP(hipMemcpyAsync(&c->pinned[sl], c->sc.as<void>(), sizeof(PcgScalars), hipMemcpyDeviceToHost,
+                            s));
+    GMGS_HIP(hipEventRecord(c->poll_ev[sl], s));
+    return true;
+  };
+  // Two chunks are always queued: while the device runs the newer one the
+  // host reads the older one's scalars and sizes the next chunk from the
+  // residual reduction rate seen so far, so that only a couple of iterations
+  // run (and exit at once) past convergence.
+  bool qa = launch(2, 0), qb = launch(2, 1);
+  int a = 0;
+  double prev_rel = -1.0;
+  int prev_its = 0;
+  while (qa) {
+    GMGS_HIP(hipEventSynchronize(c->poll_ev[a]));
+    const PcgScalars ha = c->pinned[a];
+    if (ha.done) break;
+    int next = prev_rel > 0.0 ? 16 : 4;   // no rate yet: a short chunk; no progress: a long one
+    if (prev_rel > 0.0 && ha.its > prev_its && ha.relres > 0.0 && ha.relres < prev_rel) {
+      const double rate = std::pow(ha.relres / prev_rel, 1.0 / (ha.its - prev_its));
+      const double need = std::log(ha.tol / ha.relres) / std::log(rate);   // after iteration ha.its
+      const int queued = k - ha.its;                                        // launched, not yet read
+      const double want = std::ceil(need) - queued;
+      next = want < 2.0 ? 2 : (want > 16.0 ? 16 : (int)want);
+      next += next & 1;
     }
-    if (k >= maxk) {
-      if (pending == slot) break;
-      pending = slot;
-      continue;
-    }
-    pending = slot;
-    slot ^= 1;
+    prev_rel = ha.relres;
+    prev_its = ha.its;
+    const bool launched = launch(next, a);
+    a ^= 1;
+    qa = qb;
+    qb = launched;
   }
   GMGS_HIP(hipEventRecord(e1, s));
   GMGS_HIP(hipEventSynchronize(e1));

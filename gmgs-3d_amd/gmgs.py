@@ -366,14 +366,15 @@ def copy_probe(nbytes: int, reps: int = 10, device: int = 0) -> float:
     return ms.value
 
 
-def batch_solve(solvers):
+def batch_solve(solvers, copy=True):
     """Solve every solver's problem together (gmgs_batch_solve); returns
-    [(U, SolveInfo)] in order."""
+    [(U, SolveInfo)] in order (U is None with ``copy=False``: the solutions
+    stay on the device)."""
     lib = load_library()
     n = len(solvers)
     ctxs = (C.c_void_p * n)(*[s._ctx.value for s in solvers])
-    Us = [np.zeros(3 * s.num_nodes(1)) for s in solvers]
-    ptrs = (C.POINTER(C.c_double) * n)(*[_dptr(u) for u in Us])
+    Us = [np.zeros(3 * s.num_nodes(1)) if copy else None for s in solvers]
+    ptrs = (C.POINTER(C.c_double) * n)(*[_dptr(u) if copy else None for u in Us])
     infos = (SolveInfo * n)()
     st = lib.gmgs_batch_solve(ctxs, n, ptrs, infos)
     if st not in (0, 9):

@@ -97,3 +97,43 @@ def test_gradient_vs_central_fd(cant, pkg, w_f, use_g):
     log(f"  FD check w_f={w_f} g={use_g}: worst rel dev {worst:.2e}")
     assert worst < 1e-5
     s.set_modulus(ag.simp_modulus(rho))
+
+
+@pytest.mark.parametrize("w_f,use_g", [(1.0, False), (0.0, True)])
+def test_c5_full_size_gradient_vs_central_fd(pkg, w_f, use_g):
+    """The 128x64x64 cantilever (524,288 elements, rho ~ U[0.01, 1], SIMP
+    E = 1e-9 + rho^3 (1 - 1e-9)): adjoint gradient of compliance and of a random
+    g'u against central differences (step 1e-4) on 32 random elements."""
+    import torch
+    from importlib import import_module
+    ag = import_module("gmgs3d_amd.autograd")
+    p = pkg.workloads.cantilever_c5()
+    rho = p.meta["rho"].copy()
+    s = pkg.Solver(p.vox, p.bbox, num_levels=p.num_levels, tol=1e-13, max_it=5000,
+                   vcycle_sweeps=2,
+                   flags=(pkg.FLAG_EXACT_GALERKIN | pkg.FLAG_PR_BETA | pkg.FLAG_SYM_VCYCLE
+                          | pkg.FLAG_NO_JACOBI_STEP | pkg.FLAG_NO_INDEX_TABLES))
+    s.set_bc(p.fixed, p.loads)
+    s.set_modulus(ag.simp_modulus(rho))
+    n3 = 3 * s.num_nodes(1)
+    g = np.zeros(n3)
+    if use_g:
+        g = np.random.default_rng(5).standard_normal(n3)
+        g[s.query("FIXED_DOFS") - 1] = 0.0
+    Fn = ag.make_function()
+    r = torch.tensor(rho, dtype=torch.float64, requires_grad=True)
+    J = Fn.apply(r, s, g, w_f)
+    J.backward()
+    grad = r.grad.numpy()
+    idx = np.random.default_rng(7).choice(rho.size, size=32, replace=False)
+    h = 1e-4
+    worst = 0.0
+    for e in idx:
+        rp, rm = rho.copy(), rho.copy()
+        rp[e] += h
+        rm[e] -= h
+        Jp, Jm = ag.objective(s, rp, g, w_f), ag.objective(s, rm, g, w_f)
+        worst = max(worst, abs((Jp - Jm) / (2 * h) - grad[e]) / np.abs(grad).max())
+    log(f"  C5 FD check w_f={w_f} g={use_g}: {rho.size} elements, worst rel dev {worst:.2e}")
+    assert worst < 1e-5
+    s.close()
