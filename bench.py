@@ -97,7 +97,7 @@ def run_batch(args, pkg):
     probs = pkg.workloads.batch_c4(args.seed, 64)
     log(f"[bench] generated 64 problems in {time.perf_counter() - t0:.1f}s")
     flags = (pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE | pkg.FLAG_SYM_VCYCLE
-             | pkg.FLAG_NO_JACOBI_STEP)
+             | pkg.FLAG_NO_JACOBI_STEP | pkg.FLAG_CHEBYSHEV)
     t0 = time.perf_counter()
     solvers = []
     for p in probs:
@@ -144,7 +144,8 @@ def run_batch(args, pkg):
         "data": "synthetic (64 seeded uneven ellipsoid-union solids)",
         "config": {"workload": f"batch64s{args.seed}", "problems": 64, "dofs": total,
                    "dofs_min": int(min(dofs)), "dofs_max": int(max(dofs)), "rtol": 1e-6,
-                   "mixed": "FP32 symmetric V(2,2) cycle, no Jacobi step, FP64 flexible PCG",
+                   "mixed": ("FP32 symmetric V(2,2) cycle, Chebyshev sweep weights, no Jacobi step, "
+                             "FP64 flexible PCG"),
                    "sequential_ms": round(ms_seq, 3),
                    "speedup_vs_sequential": round(ms_seq / ms_batch, 3),
                    "batch_vs_individual_max_rel_dev": worst,
@@ -192,10 +193,12 @@ def main():
                   max_it=args.max_it)
     base_flags = pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE
     sym_flags = base_flags | pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP
+    cheb_flags = sym_flags | pkg.FLAG_CHEBYSHEV
 
-    # headline: FP32 symmetric V(2,2) cycle inside the FP64 flexible PCG
+    # headline: FP32 symmetric V(2,2) cycle with Chebyshev sweep weights inside
+    # the FP64 flexible PCG
     t0 = time.perf_counter()
-    s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, flags=sym_flags,
+    s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, flags=cheb_flags,
                    vcycle_sweeps=2, **common)
     s.set_bc(prob.fixed, prob.loads)
     setup_s = time.perf_counter() - t0
@@ -222,8 +225,10 @@ def main():
     err = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
     # the same symmetric cycle in FP64 (precision-only comparison) and the
     # reference cycle in FP32
-    _, ms64s, info64s = other("fp64-sym", pkg.GMGS_PREC_FP64, sym_flags)
+    _, ms64s, info64s = other("fp64-cheb", pkg.GMGS_PREC_FP64, cheb_flags)
     _, msmr, infomr = other("mixed-ref", pkg.GMGS_PREC_MIXED, base_flags)
+    # the symmetric cycle with the plain Jacobi weight in every sweep
+    _, msmj, infomj = other("mixed-sym-jacobi", pkg.GMGS_PREC_MIXED, sym_flags)
     log(f"[bench] mixed vs fp64 rel L2 {err:.2e}; speed-up {ms64 / ms_per_step:.3f} "
         f"(precision only {ms64s / ms_per_step:.3f})")
 
@@ -273,7 +278,8 @@ def main():
         "data": "synthetic (seeded ellipsoid-union voxel solid)",
         "config": {"workload": prob.name, "dofs": ndofs, "levels": s.num_levels,
                    "rtol": 1e-6,
-                   "mixed": "FP32 symmetric V(2,2) cycle, no Jacobi step, FP64 flexible PCG",
+                   "mixed": ("FP32 symmetric V(2,2) cycle, Chebyshev sweep weights, no Jacobi step, "
+                             "FP64 flexible PCG"),
                    "fp64": "reference algorithm: kernel/Vcycle.m V-cycle, Jacobi step, FP64 PCG",
                    "its_mixed": info.its, "its_fp64": info64.its,
                    "converged_mixed": bool(info.converged), "converged_fp64": bool(info64.converged),
@@ -281,9 +287,11 @@ def main():
                    "fp64_ms": round(ms64, 3),
                    "fp64_mdofs": round(ndofs / (ms64 * 1e-3) / 1e6, 3),
                    "speedup_vs_fp64": round(ms64 / ms_per_step, 3),
-                   "fp64_symmetric_ms": round(ms64s, 3), "its_fp64_symmetric": info64s.its,
+                   "fp64_same_cycle_ms": round(ms64s, 3), "its_fp64_same_cycle": info64s.its,
                    "speedup_precision_only": round(ms64s / ms_per_step, 3),
                    "mixed_reference_cycle_ms": round(msmr, 3), "its_mixed_reference_cycle": infomr.its,
+                   "mixed_jacobi_sweeps_ms": round(msmj, 3), "its_mixed_jacobi_sweeps": infomj.its,
+                   "smoother_lmax": [round(s.smoother(li)[0], 5) for li in range(1, s.num_levels)],
                    "mixed_vs_fp64_rel_l2": err,
                    "setup_s": round(setup_s, 3)},
         "roofline": {"kernel": "k_apply_zm<f64,AX> (level-1 matvec q = A p)", "bound": "hbm",

@@ -54,6 +54,10 @@ struct Level {
   DevBuf colz;                   // per padded column: solid-node z range
   long long ncol = 0;
   int col_tx = 8, col_zc = 16;
+  // symmetric cycle: weights of the nu smoothing sweeps, and the estimate of
+  // lambda_max(D^-1 A) on the free DOFs they came from (0 = plain Jacobi)
+  double om[2] = {0.0, 0.0};
+  double lmax = 0.0;
 };
 
 }  // namespace
@@ -77,6 +81,7 @@ struct gmgs_ctx {
   bool autotune = false;
   bool sym = false;              // symmetric V(nu, nu) cycle
   int sweeps = 2;
+  bool cheb = false;             // Chebyshev sweep weights in the symmetric cycle
   bool no_jacobi = false;        // no Jacobi step opening each PCG iteration
   // SIMP
   bool simp = false;
@@ -541,10 +546,11 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
 
 // dot partials any level-1 launch may write: the tile kernels (one per 4
 // tiles), the pointwise kernels, or the z-marching operator (one per tile)
+// partials of any reduction: the PCG kernels and an AX on any level
 long long partial_capacity(gmgs_ctx* c) {
-  const Level& l = L1(c);
-  long long n = std::max<long long>(apply_blocks(l.ntiles), c->nparts);
-  return std::max<long long>(n, l.ncol);
+  long long n = c->nparts;
+  for (const Level& l : c->lev) n = std::max<long long>(n, std::max<long long>(apply_blocks(l.ntiles), l.ncol));
+  return n;
 }
 
 void autotune_levels(gmgs_ctx* c) {
@@ -701,26 +707,30 @@ void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partia
 }
 
 // Symmetric V(nu, nu) cycle (no reference counterpart): nu pre-smoothing
-// sweeps R_s = R_{s-1} - A(omega R_{s-1}/d) from the level's input R_0, the
+// sweeps R_s = R_{s-1} - A(w_{s-1} R_{s-1}/d) from the level's input R_0, the
 // coarse correction e = P x_c, the residual recomputed as Q_0 = R_nu - A e
-// (A x_pre = R_0 - R_nu), nu post-smoothing sweeps Q_s from it, and
-// x = omega (sum_{s<nu} R_s + sum_{s<nu} Q_s)/d + e.  Unlike the stale
-// post-smoothing of kernel/Vcycle.m:27-33 this is a symmetric preconditioner.
+// (A x_pre = R_0 - R_nu), nu post-smoothing sweeps Q_s from it with the same
+// weights, and x = sum_{s<nu} w_s (R_s + Q_s)/d + e.  Unlike the stale
+// post-smoothing of kernel/Vcycle.m:27-33 this is a symmetric preconditioner:
+// the sweeps of a level are polynomials in D^-1 A, which commute, so the
+// post-smoother is the A-adjoint of the pre-smoother for any weights.  The
+// weights are the Jacobi weight, or the reciprocal roots of the degree-nu
+// Chebyshev polynomial on the level's eigenvalue bounds.
 template <typename T>
 void vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
   hipStream_t s = c->stream;
   const int L = c->L, nu = c->sweeps;
-  const T omega = (T)c->prm.jacobi_weight;
   for (int li = 0; li < L - 1; ++li) {
     Level& l = c->lev[li];
+    const T w0 = (T)l.om[0], w1 = (T)l.om[1];
     {
       KTime t(c, li == 0 ? 2 : -1);
-      if (li == 0) apply_level<T, double>(c, 0, AP_SMOOTH, rin, omega, vec<T>(l.rp), nullptr, nullptr, done,
+      if (li == 0) apply_level<T, double>(c, 0, AP_SMOOTH, rin, w0, vec<T>(l.rp), nullptr, nullptr, done,
                                         nullptr, true);
-      else apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.r), omega, vec<T>(l.rp), nullptr, nullptr, done,
+      else apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.r), w0, vec<T>(l.rp), nullptr, nullptr, done,
                                  nullptr, true);
       if (nu == 2)
-        apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.rp), omega, vec<T>(l.w1), nullptr, nullptr, done,
+        apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.rp), w1, vec<T>(l.w1), nullptr, nullptr, done,
                           nullptr, true);
     }
     T* Rnu = nu == 2 ? vec<T>(l.w1) : vec<T>(l.rp);
@@ -743,20 +753,21 @@ void vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* pa
     const uint32_t* fl = l.mesh.flags.as<uint32_t>();
     const TileRec* tl = l.tiles.as<TileRec>();
     const T* invd = make_op<T>(c, li, true).invd;
+    const T w0 = (T)l.om[0], w1 = (T)l.om[1];
     T* Rnu = nu == 2 ? vec<T>(l.w1) : vec<T>(l.rp);
     launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
     apply_level<T, T>(c, li, AP_RESID, vec<T>(l.e), (T)0, vec<T>(l.w2), nullptr, nullptr, done, Rnu, true);
     if (nu == 2)
-      apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.w2), omega, vec<T>(l.w1), nullptr, nullptr, done,
+      apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.w2), w0, vec<T>(l.w1), nullptr, nullptr, done,
                         nullptr, true);
     const T* a = nu == 2 ? vec<T>(l.rp) : nullptr;       // R_1
     const T* b = vec<T>(l.w2);                            // Q_0
     const T* q1 = nu == 2 ? vec<T>(l.w1) : nullptr;      // Q_1
     if (li == 0)
-      launch_combine<T, double, T>(g, fl, tl, l.ntiles, invd, omega, rin, a, b, q1, vec<T>(l.e), z, true,
+      launch_combine<T, double, T>(g, fl, tl, l.ntiles, invd, w0, w1, rin, a, b, q1, vec<T>(l.e), z, true,
                                    zprev, partial, done, s);
     else
-      launch_combine<T, T, T>(g, fl, tl, l.ntiles, invd, omega, vec<T>(l.r), a, b, q1, vec<T>(l.e),
+      launch_combine<T, T, T>(g, fl, tl, l.ntiles, invd, w0, w1, vec<T>(l.r), a, b, q1, vec<T>(l.e),
                               vec<T>(l.x), false, (const T*)nullptr, nullptr, done, s);
   }
 }
@@ -952,6 +963,120 @@ double dot_host(gmgs_ctx* c, const double* a, const double* b) {
   return sum;
 }
 
+// ------------------------------------------------------- smoother bounds
+// Chebyshev sweep weights of the symmetric cycle (no reference counterpart)
+constexpr int kLanczosSteps = 20;       // CG steps of the lambda_max estimate
+constexpr double kChebSafety = 1.1;     // upper bound = kChebSafety * estimate
+constexpr double kChebLower = 0.2;      // lower bound = kChebLower * upper bound
+
+// largest eigenvalue of the symmetric tridiagonal (diag d, off-diagonal e) by
+// bisection on Sturm sequence counts
+double tridiag_max_eig(const std::vector<double>& d, const std::vector<double>& e) {
+  const size_t n = d.size();
+  double lo = 1e300, hi = -1e300;
+  for (size_t i = 0; i < n; ++i) {
+    const double rad = (i ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - rad);
+    hi = std::max(hi, d[i] + rad);
+  }
+  // number of eigenvalues below x
+  auto below = [&](double x) {
+    size_t cnt = 0;
+    double q = 1.0;
+    for (size_t i = 0; i < n; ++i) {
+      q = d[i] - x - (i ? e[i - 1] * e[i - 1] / q : 0.0);
+      if (q == 0.0) q = -1e-300;
+      if (q < 0.0) ++cnt;
+    }
+    return cnt;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(hi)); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (below(mid) == n) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+// lambda_max(D^-1 A) on the free DOFs of level li (cycle precision): the top
+// Ritz value of kLanczosSteps Jacobi-preconditioned CG iterations from a fixed
+// pseudo-random start.  The CG coefficients give the Lanczos matrix:
+// T_kk = 1/alpha_k + beta_{k-1}/alpha_{k-1}, T_k,k+1 = sqrt(beta_k)/alpha_k.
+// Uses the level's cycle work vectors; 0 when the level has no free DOF.
+template <typename T>
+double estimate_lmax(gmgs_ctx* c, int li) {
+  hipStream_t s = c->stream;
+  Level& l = c->lev[li];
+  const Grid& g = l.mesh.g;
+  const uint32_t* fl = l.mesh.flags.as<uint32_t>();
+  const TileRec* tl = l.tiles.as<TileRec>();
+  const T* invd = make_op<T>(c, li, true).invd;
+  T* r = vec<T>(l.e);
+  T* z = vec<T>(l.rp);
+  T* p = vec<T>(l.w1);
+  T* q = vec<T>(l.w2);
+  double* part = c->part.as<double>();
+  auto total = [&](long long np) {
+    const std::vector<double> h = download<double>(c->part, (size_t)np, s);
+    double t = 0.0;
+    for (double v : h) t += v;
+    return t;
+  };
+  const long long npw = pointwise_blocks(l.ntiles);
+  launch_seed<T>(g, fl, tl, l.ntiles, r, s);
+  launch_jcg_step<T>(g, fl, tl, l.ntiles, invd, (const T*)nullptr, 0.0, r, z, part, s);
+  double rz = total(npw);
+  if (!(rz > 0.0)) return 0.0;
+  const double rz0 = rz;
+  launch_xpby<T>(g, fl, tl, l.ntiles, z, 0.0, p, s);
+  std::vector<double> al, be;
+  for (int k = 0; k < kLanczosSteps; ++k) {
+    const int np = apply_level<T, T>(c, li, AP_AX, p, (T)0, q, nullptr, part, nullptr, nullptr, true);
+    const double pq = total(np);
+    if (!(pq > 0.0)) break;
+    const double alpha = rz / pq;
+    launch_jcg_step<T>(g, fl, tl, l.ntiles, invd, q, alpha, r, z, part, s);
+    const double rzn = total(npw);
+    al.push_back(alpha);
+    be.push_back(rzn / rz);
+    if (!(rzn > 1e-24 * rz0)) break;
+    launch_xpby<T>(g, fl, tl, l.ntiles, z, rzn / rz, p, s);
+    rz = rzn;
+  }
+  if (al.empty()) throw Status(GMGS_ERR_BREAKDOWN, "smoother bound estimate: operator not positive");
+  const size_t n = al.size();
+  std::vector<double> d(n), e(n > 1 ? n - 1 : 0);
+  for (size_t i = 0; i < n; ++i) {
+    d[i] = 1.0 / al[i] + (i ? be[i - 1] / al[i - 1] : 0.0);
+    if (i + 1 < n) e[i] = std::sqrt(be[i]) / al[i];
+  }
+  return tridiag_max_eig(d, e);
+}
+
+// sweep weights of every smoothed level: the Jacobi weight, or with
+// Chebyshev smoothing w_i = 1/(theta + delta cos((2i-1) pi/(2 nu))), the
+// reciprocal roots of the degree-nu Chebyshev polynomial shifted to
+// [kChebLower b, b], b = kChebSafety lambda_max, theta and delta the centre
+// and half-width of that interval
+void smoother_weights(gmgs_ctx* c) {
+  const double pi = 3.14159265358979323846;
+  for (int li = 0; li < c->L - 1; ++li) {
+    Level& l = c->lev[li];
+    l.om[0] = l.om[1] = c->prm.jacobi_weight;
+    l.lmax = 0.0;
+    if (!(c->sym && c->cheb)) continue;
+    const double lm = c->mixed ? estimate_lmax<float>(c, li) : estimate_lmax<double>(c, li);
+    if (!(lm > 0.0)) continue;
+    l.lmax = lm;
+    const double b = kChebSafety * lm, a = kChebLower * b;
+    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a);
+    for (int i = 1; i <= c->sweeps; ++i)
+      l.om[i - 1] = 1.0 / (theta + delta * std::cos((2 * i - 1) * pi / (2 * c->sweeps)));
+    if (c->sweeps == 1) l.om[1] = l.om[0];
+  }
+  GMGS_HIP(hipStreamSynchronize(c->stream));
+}
+
 void set_bc_impl(gmgs_ctx* c, const int32_t* fixed, int64_t k, const double* loads, int64_t m) {
   hipStream_t s = c->stream;
   invalidate_graph(c);
@@ -1031,6 +1156,7 @@ void set_bc_impl(gmgs_ctx* c, const int32_t* fixed, int64_t k, const double* loa
   const long long nch = (c->nfp + 255) / 256;
   c->gv_part.alloc_zero(sizeof(double) * nch * c->nfp, s);
   GMGS_HIP(hipStreamSynchronize(s));
+  smoother_weights(c);
   c->bc = true;
   c->have_solution = false;
   if (c->bc_fixed.data() != fixed) {
@@ -1137,6 +1263,9 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->sym = (p->flags & GMGS_FLAG_SYM_VCYCLE) != 0;
     c->sweeps = p->vcycle_sweeps == 0 ? 2 : p->vcycle_sweeps;
     c->no_jacobi = (p->flags & GMGS_FLAG_NO_JACOBI_STEP) != 0;
+    c->cheb = (p->flags & GMGS_FLAG_CHEBYSHEV) != 0;
+    if (c->cheb && !c->sym)
+      throw Status(GMGS_ERR_ARGUMENT, "Chebyshev smoothing needs the symmetric cycle");
     c->nely = (int)ny;
     c->nelx = (int)nx;
     c->nelz = (int)nz;
@@ -1163,6 +1292,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     upload(c->stressM, c->M, 48 * 24, c->stream);
     build_operators(c.get());
     alloc_vectors(c.get());
+    for (Level& l : c->lev) l.om[0] = l.om[1] = p->jacobi_weight;
     if (c->autotune) autotune_levels(c.get());
     return GMGS_OK;
   });
@@ -1670,6 +1800,11 @@ int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count) {
       }
       case GMGS_Q_KE0: n = 576; break;
       case GMGS_Q_BBOX: n = 6; break;
+      case GMGS_Q_SMOOTHER:
+        if (level >= c->L) throw Status(GMGS_ERR_ARGUMENT, "the coarsest level is not smoothed");
+        if (!c->bc) throw Status(GMGS_ERR_STATE, "boundary conditions not set");
+        n = 3;
+        break;
       default: throw Status(GMGS_ERR_ARGUMENT, "unknown query key");
     }
     *count = n;
@@ -1757,6 +1892,7 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
       }
       case GMGS_Q_KE0: std::memcpy(f64, c->Ke0, sizeof(double) * 576); break;
       case GMGS_Q_BBOX: std::memcpy(f64, c->bbox, sizeof(double) * 6); break;
+      case GMGS_Q_SMOOTHER: f64[0] = l->lmax; f64[1] = l->om[0]; f64[2] = l->om[1]; break;
       case GMGS_Q_F:
         if (!c->bc || level != 1) throw Status(GMGS_ERR_STATE, "F_ exists on level 1 after gmgs_set_bc");
         std::memcpy(f64, c->F.data(), sizeof(double) * c->F.size());

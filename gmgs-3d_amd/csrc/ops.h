@@ -81,11 +81,11 @@ template <typename T>
 void launch_mask_fixed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                        const T* src, T* dst, hipStream_t s);
 
-// x = omega ((r0 + a + b + c) / d) + e (null a, b, c skipped), fixed components
-// zeroed if 'fix'; optional partials of r0.x and r0.zprev (FP64)
+// x = (w0 (r0 + b) + w1 (a + c)) / d + e (null a, b, c skipped), fixed
+// components zeroed if 'fix'; optional partials of r0.x and r0.zprev (FP64)
 template <typename T, typename TR, typename TZ>
 void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
-                    const T* invd, T omega, const TR* r0, const T* a, const T* b, const T* c,
+                    const T* invd, T w0, T w1, const TR* r0, const T* a, const T* b, const T* c,
                     const T* e, TZ* x, bool fix, const TZ* zprev, double* partial, const int* done,
                     hipStream_t s);
 
@@ -105,6 +105,21 @@ void launch_update_xr(const Grid& g, const uint32_t* flags, const TileRec* tiles
 template <typename T, typename TR>
 void launch_jacobi_acc(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                        const T* invd, const TR* r, T omega, TR* x, hipStream_t s);
+
+// Jacobi-preconditioned CG steps of the smoother-bound estimate:
+// v = fixed pseudo-random values in [-1, 1) on the free components of the solid nodes
+template <typename T>
+void launch_seed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, T* v,
+                 hipStream_t s);
+// r -= alpha q (q null: r kept), z = invd r; pointwise_blocks(nt) partials of r.z
+template <typename T>
+void launch_jcg_step(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                     const T* invd, const T* q, double alpha, T* r, T* z, double* partial,
+                     hipStream_t s);
+// p = z + beta p
+template <typename T>
+void launch_xpby(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, const T* z,
+                 double beta, T* p, hipStream_t s);
 
 // dot / norm partials over tiles (FP64)
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,

@@ -293,12 +293,12 @@ __global__ __launch_bounds__(256) void k_mask_fixed(Grid g, const uint32_t* __re
   }
 }
 
-// x = omega ((r0 + a + b + c) / d) + e on the solid nodes (null a, b, c
+// x = (w0 (r0 + b) + w1 (a + c)) / d + e on the solid nodes (null a, b, c
 // skipped); zero fixed components if 'fix'; partials of r0.x and r0.zprev
 template <typename T, typename TR, typename TZ>
 __global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restrict__ flags,
                                                  const TileRec* __restrict__ tiles, long long nt,
-                                                 const T* __restrict__ invd, T omega,
+                                                 const T* __restrict__ invd, T w0, T w1,
                                                  const TR* __restrict__ r0, const T* __restrict__ a,
                                                  const T* __restrict__ b, const T* __restrict__ cc,
                                                  const T* __restrict__ e, TZ* __restrict__ x, int fix,
@@ -314,11 +314,11 @@ __global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restr
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long m = c * N + n;
-      T sum = (T)r0[m];
-      if (a) sum += a[m];
-      if (b) sum += b[m];
-      if (cc) sum += cc[m];
-      T v = omega * invd[m] * sum + e[m];
+      T s0 = (T)r0[m], s1 = 0;
+      if (b) s0 += b[m];
+      if (a) s1 += a[m];
+      if (cc) s1 += cc[m];
+      T v = invd[m] * (w0 * s0 + w1 * s1) + e[m];
       if (fix && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) v = 0;
       x[m] = (TZ)v;
       if (partial) {
@@ -416,6 +416,74 @@ __global__ __launch_bounds__(256) void k_jacobi_acc(Grid g, const uint32_t* __re
     const long long n = ln_.n;
 #pragma unroll
     for (int c = 0; c < 3; ++c) x[c * N + n] += (TR)(omega * invd[c * N + n] * (T)r[c * N + n]);
+  }
+}
+
+// v = values in [-1, 1) from a fixed integer hash of (padded index, component)
+// on the free components of the solid nodes (the lowbias32 mixer of
+// C. Wellons' hash-prospector: xorshift-multiply rounds)
+template <typename T>
+__global__ __launch_bounds__(256) void k_seed(Grid g, const uint32_t* __restrict__ flags,
+                                              const TileRec* __restrict__ tiles, long long nt,
+                                              T* __restrict__ v) {
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uint32_t h = (uint32_t)(3 * ln_.n + c);
+      h ^= h >> 16;
+      h *= 0x7feb352du;
+      h ^= h >> 15;
+      h *= 0x846ca68bu;
+      h ^= h >> 16;
+      const T val = (T)((double)(h >> 8) * (1.0 / 8388608.0) - 1.0);
+      v[c * N + ln_.n] = (ln_.fl & (1u << (F_FIX_SHIFT + c))) ? (T)0 : val;
+    }
+  }
+}
+
+// r -= alpha q (q null: r unchanged), z = invd r; partials of r.z
+template <typename T>
+__global__ __launch_bounds__(256) void k_jcg_step(Grid g, const uint32_t* __restrict__ flags,
+                                                  const TileRec* __restrict__ tiles, long long nt,
+                                                  const T* __restrict__ invd, const T* __restrict__ q,
+                                                  double alpha, T* __restrict__ r, T* __restrict__ z,
+                                                  double* __restrict__ partial) {
+  double d = 0.0;
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long m = c * N + ln_.n;
+      T rv = r[m];
+      if (q) {
+        rv = (T)((double)rv - alpha * (double)q[m]);
+        r[m] = rv;
+      }
+      const T zv = invd[m] * rv;
+      z[m] = zv;
+      d += (double)rv * (double)zv;
+    }
+  }
+  const double b = block_sum(d);
+  if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// p = z + beta p (beta = 0: p = z)
+template <typename T>
+__global__ __launch_bounds__(256) void k_xpby(Grid g, const uint32_t* __restrict__ flags,
+                                              const TileRec* __restrict__ tiles, long long nt,
+                                              const T* __restrict__ z, double beta, T* __restrict__ p) {
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long m = c * N + ln_.n;
+      p[m] = beta == 0.0 ? z[m] : (T)((double)z[m] + beta * (double)p[m]);
+    }
   }
 }
 
@@ -726,24 +794,24 @@ template void launch_mask_fixed<float>(const Grid&, const uint32_t*, const TileR
 
 template <typename T, typename TR, typename TZ>
 void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
-                    const T* invd, T omega, const TR* r0, const T* a, const T* b, const T* c,
+                    const T* invd, T w0, T w1, const TR* r0, const T* a, const T* b, const T* c,
                     const T* e, TZ* x, bool fix, const TZ* zprev, double* partial, const int* done,
                     hipStream_t s) {
   if (nt <= 0) return;
-  k_combine<T, TR, TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, omega, r0, a, b, c,
+  k_combine<T, TR, TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, w0, w1, r0, a, b, c,
                                                              e, x, fix ? 1 : 0, zprev, partial, done);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_combine<double, double, double>(const Grid&, const uint32_t*, const TileRec*, long long,
-                                                     const double*, double, const double*, const double*,
+                                                     const double*, double, double, const double*, const double*,
                                                      const double*, const double*, const double*, double*,
                                                      bool, const double*, double*, const int*, hipStream_t);
 template void launch_combine<float, float, float>(const Grid&, const uint32_t*, const TileRec*, long long,
-                                                  const float*, float, const float*, const float*,
+                                                  const float*, float, float, const float*, const float*,
                                                   const float*, const float*, const float*, float*, bool,
                                                   const float*, double*, const int*, hipStream_t);
 template void launch_combine<float, double, float>(const Grid&, const uint32_t*, const TileRec*, long long,
-                                                   const float*, float, const double*, const float*,
+                                                   const float*, float, float, const double*, const float*,
                                                    const float*, const float*, const float*, float*, bool,
                                                    const float*, double*, const int*, hipStream_t);
 
@@ -777,6 +845,38 @@ template void launch_jacobi_acc<float, float>(const Grid&, const uint32_t*, cons
                                               const float*, const float*, float, float*, hipStream_t);
 template void launch_jacobi_acc<float, double>(const Grid&, const uint32_t*, const TileRec*, long long,
                                                const float*, const double*, float, double*, hipStream_t);
+
+template <typename T>
+void launch_seed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, T* v,
+                 hipStream_t s) {
+  if (nt <= 0) return;
+  k_seed<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, v);
+  GMGS_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_jcg_step(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                     const T* invd, const T* q, double alpha, T* r, T* z, double* partial,
+                     hipStream_t s) {
+  if (nt <= 0) return;
+  k_jcg_step<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, q, alpha, r, z, partial);
+  GMGS_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_xpby(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, const T* z,
+                 double beta, T* p, hipStream_t s) {
+  if (nt <= 0) return;
+  k_xpby<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, beta, p);
+  GMGS_LAUNCH_CHECK();
+}
+#define GMGS_INST_LANCZOS(T)                                                                          \
+  template void launch_seed<T>(const Grid&, const uint32_t*, const TileRec*, long long, T*, hipStream_t); \
+  template void launch_jcg_step<T>(const Grid&, const uint32_t*, const TileRec*, long long, const T*,   \
+                                   const T*, double, T*, T*, double*, hipStream_t);                     \
+  template void launch_xpby<T>(const Grid&, const uint32_t*, const TileRec*, long long, const T*, double, \
+                               T*, hipStream_t);
+GMGS_INST_LANCZOS(double)
+GMGS_INST_LANCZOS(float)
+#undef GMGS_INST_LANCZOS
 
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                 const double* a, const double* b, double* partial, hipStream_t s) {

@@ -103,17 +103,16 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     }
   }
   // raw loads only (no arithmetic on them until they are staged, two planes
-  // later): the input and, for the smoother, the inverse diagonal
+  // later): the input and, for the smoother, the inverse diagonal.  Index 0 of
+  // every padded array is a corner halo entry and holds 0, so a skipped
+  // position loads from there: the loads need no branches.
   auto fetch = [&](int zz, TR* vr, T* vd) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      vr[i] = 0;
-      if (SM) vd[i] = 0;
-      if (soff[i] >= 0 && (unsigned)(zz - (zr[i] & 0xffff)) <= (unsigned)(zr[i] >> 16)) {
-        const long long m = (long long)soff[i] + sz * zz;
-        vr[i] = in[m];
-        if (SM) vd[i] = op.invd[m];
-      }
+      const bool ok = soff[i] >= 0 && (unsigned)(zz - (zr[i] & 0xffff)) <= (unsigned)(zr[i] >> 16);
+      const long long m = ok ? (long long)soff[i] + sz * zz : 0;
+      vr[i] = in[m];
+      if (SM) vd[i] = op.invd[m];
     }
   };
 
@@ -129,8 +128,8 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   fetch(z0, rB, dB);
   // flags of the outputs this lane owns, loaded one plane before they are used
   auto flags_at = [&](const int z) -> uint32_t {
-    return (lane_ok && z >= z0 && z < z1 && z >= own_lo && z <= own_hi) ? op.flags[colidx + sz * z]
-                                                                          : 0u;
+    const bool ok = lane_ok && z >= z0 && z < z1 && z >= own_lo && z <= own_hi;
+    return op.flags[ok ? colidx + sz * z : 0];    // flags[0]: a halo node, 0
   };
   uint32_t fN = flags_at(z0);
 

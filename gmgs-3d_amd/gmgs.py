@@ -34,14 +34,15 @@ FLAG_AUTOTUNE = 32
 FLAG_NO_GRAPHS = 64
 FLAG_SYM_VCYCLE = 128
 FLAG_NO_JACOBI_STEP = 256
+FLAG_CHEBYSHEV = 512
 
 Q = dict(NUM_LEVELS=0, RES=1, NUM_ELEMENTS=2, NUM_NODES=3, ELE_MAP_BACK=4, ELE_MAP_FORWARD=5,
          NOD_MAP_BACK=6, NOD_MAP_FORWARD=7, ENODMAT=8, NUM_NOD2ELES=9, NODES_ON_BOUNDARY=10,
          ELEMENTS_ON_BOUNDARY=11, ELEMENT_UPWARD_MAP=12, TRANSFER_MAT=13, TRANSFER_MAT_COEFFI=14,
          SOLID_NODE_MAP_C2F=15, INTERMEDIATE_NUM_NODES=16, STORING_STATE=17, UNIQUE_ELEMENTS=18,
          KS=19, DIAGK=20, FIXED_DOFS=21, FREE_DOFS=22, KE0=23, BBOX=24, F=25, NUM_TILES=26,
-         NUM_FREE_COARSEST=27, TILE_CONFIG=28)
-_Q_DOUBLE = {Q["TRANSFER_MAT_COEFFI"], Q["KS"], Q["DIAGK"], Q["KE0"], Q["BBOX"], Q["F"]}
+         NUM_FREE_COARSEST=27, TILE_CONFIG=28, SMOOTHER=29)
+_Q_DOUBLE = {Q["TRANSFER_MAT_COEFFI"], Q["KS"], Q["DIAGK"], Q["KE0"], Q["BBOX"], Q["F"], Q["SMOOTHER"]}
 _Q_INT64 = {Q["NUM_LEVELS"], Q["RES"], Q["NUM_ELEMENTS"], Q["NUM_NODES"], Q["INTERMEDIATE_NUM_NODES"],
             Q["STORING_STATE"], Q["NUM_TILES"], Q["NUM_FREE_COARSEST"], Q["TILE_CONFIG"]}
 
@@ -223,6 +224,12 @@ class Solver:
     @property
     def num_levels(self) -> int:
         return self.query("NUM_LEVELS")
+
+    def smoother(self, level: int = 1):
+        """(lambda_max estimate of D^-1 A on the free DOFs or 0, w1, w2): the sweep
+        weights of the symmetric cycle on a smoothed level, after set_bc."""
+        lm, w1, w2 = self.query("SMOOTHER", level)
+        return float(lm), float(w1), float(w2)
 
     def num_nodes(self, level=1) -> int:
         return self.query("NUM_NODES", level)

@@ -66,8 +66,13 @@ enum {
   GMGS_FLAG_SYM_VCYCLE = 128,     /* symmetric V(nu, nu) cycle: residual recomputed after the
                                      coarse correction, nu = vcycle_sweeps (no reference
                                      counterpart; kernel/Vcycle.m:27-33 reuses the stale one) */
-  GMGS_FLAG_NO_JACOBI_STEP = 256  /* omit the Jacobi step opening every PCG iteration
+  GMGS_FLAG_NO_JACOBI_STEP = 256, /* omit the Jacobi step opening every PCG iteration
                                      (CG_solver.m:18-19) */
+  GMGS_FLAG_CHEBYSHEV = 512       /* with GMGS_FLAG_SYM_VCYCLE: the nu sweeps of a level use
+                                     the reciprocal roots of the degree-nu Chebyshev polynomial
+                                     on [0.22, 1.1] x lambda_max(D^-1 A), estimated per level by
+                                     20 Jacobi-CG (Lanczos) steps when the BCs are set (no
+                                     reference counterpart; JacobiSmoother.m uses one weight) */
 };
 
 typedef struct gmgs_params {
@@ -124,7 +129,9 @@ enum {
   GMGS_Q_F = 25,                    /* double[numDOFs] load vector F_ */
   GMGS_Q_NUM_TILES = 26,            /* int64 active 16x4x1 tiles of the level */
   GMGS_Q_NUM_FREE_COARSEST = 27,    /* int64 */
-  GMGS_Q_TILE_CONFIG = 28           /* int64[3]: z-marching on?, tile TX, z-chunk */
+  GMGS_Q_TILE_CONFIG = 28,          /* int64[3]: z-marching on?, tile TX, z-chunk */
+  GMGS_Q_SMOOTHER = 29              /* double[3] (levels < numLevels, after gmgs_set_bc):
+                                       lambda_max(D^-1 A) estimate (0 = none), sweep weights w1, w2 */
 };
 
 void gmgs_default_params(gmgs_params* p);

@@ -697,23 +697,25 @@ def vcycle(model: Model, r: np.ndarray) -> np.ndarray:
     return d
 
 
-def vcycle_sym(model: Model, r: np.ndarray, nu: int = 2) -> np.ndarray:
+def vcycle_sym(model: Model, r: np.ndarray, nu: int = 2, weights=None) -> np.ndarray:
     """Symmetric V(nu, nu) cycle on the free DOFs (no reference counterpart).
 
-    The levels, transfers, Jacobi weight and coarsest solve of
-    kernel/Vcycle.m:1-36, with three changes: each level works on its free
-    DOFs only (smoothing increments, smoothed residuals and the prolonged
-    correction are zero on the level's fixed DOFs), the post-smoothing starts
-    from the residual recomputed after the coarse correction (Vcycle.m:27-33
-    reuses the pre-smoothed one), and each level smooths nu times on the way
-    down and up.  Of the free-DOF restriction only the zero smoothing weight on
-    fixed DOFs changes the result: a coarse DOF is fixed whenever a fine fixed
-    DOF lies in its support (SetupSolver.m:247-254), so fixed-DOF residual
-    values only ever reach coarse fixed DOFs and P x_c vanishes on fine fixed
-    DOFs; the other two masks are kept as plain statements of the intent.
-    In the form the device evaluates it:
-    R_s = R_{s-1} - A (omega R_{s-1}/d), e = P x_c, Q_0 = R_nu - A e,
-    Q_s = Q_{s-1} - A (omega Q_{s-1}/d), x = omega (sum_{s<nu} R_s + Q_s)/d + e.
+    The levels, transfers and coarsest solve of kernel/Vcycle.m:1-36, with
+    three changes: each level works on its free DOFs only (smoothing
+    increments, smoothed residuals and the prolonged correction are zero on the
+    level's fixed DOFs), the post-smoothing starts from the residual
+    recomputed after the coarse correction (Vcycle.m:27-33 reuses the
+    pre-smoothed one), and each level smooths nu times on the way down and up.
+    Of the free-DOF restriction only the zero smoothing weight on fixed DOFs
+    changes the result: a coarse DOF is fixed whenever a fine fixed DOF lies in
+    its support (SetupSolver.m:247-254), so fixed-DOF residual values only ever
+    reach coarse fixed DOFs and P x_c vanishes on fine fixed DOFs; the other
+    two masks are kept as plain statements of the intent.
+    weights[ii] lists the nu sweep weights w_s of level ii (default: the
+    Jacobi weight of JacobiSmoother.m for every sweep).  In the form the device
+    evaluates it: R_s = R_{s-1} - A (w_{s-1} R_{s-1}/d), e = P x_c,
+    Q_0 = R_nu - A e, Q_s = Q_{s-1} - A (w_{s-1} Q_{s-1}/d),
+    x = sum_{s<nu} w_s (R_s + Q_s)/d + e.
     """
     Lc = model.numLevels
     rin = [None] * Lc
@@ -724,32 +726,93 @@ def vcycle_sym(model: Model, r: np.ndarray, nu: int = 2) -> np.ndarray:
     def fixed(ii):
         return model.levels[ii - 1].fixedDOFs - 1
 
-    def invd(ii):
-        w = model.omega / model.levels[ii - 1].diagK
+    def dinv(ii):
+        w = 1.0 / model.levels[ii - 1].diagK
         w[fixed(ii)] = 0.0
         return w
 
-    def smooth(rr, ii):
-        out = rr - atx(model, invd(ii) * rr, ii)
+    def wts(ii):
+        return list(weights[ii]) if weights is not None else [model.omega] * nu
+
+    def smooth(rr, ii, w):
+        out = rr - atx(model, w * dinv(ii) * rr, ii)
         out[fixed(ii)] = 0.0
         return out
 
     for ii in range(1, Lc):
+        w = wts(ii)
         R = [rin[ii - 1]]
-        for _ in range(nu):
-            R.append(smooth(R[-1], ii))
+        for s in range(nu):
+            R.append(smooth(R[-1], ii, w[s]))
         Rs[ii - 1] = R
         rin[ii] = restrict(model, R[nu], ii + 1)
     xs[Lc - 1] = coarsest_solve(model, rin[Lc - 1])
     for ii in range(Lc - 1, 0, -1):
+        w = wts(ii)
         R = Rs[ii - 1]
         e = interpolate(model, xs[ii], ii + 1)
         e[fixed(ii)] = 0.0
         Q = [R[nu] - atx(model, e, ii)]
-        for _ in range(nu - 1):
-            Q.append(smooth(Q[-1], ii))
-        xs[ii - 1] = invd(ii) * (sum(R[:nu]) + sum(Q)) + e
+        for s in range(nu - 1):
+            Q.append(smooth(Q[-1], ii, w[s]))
+        acc = sum(w[s] * (R[s] + Q[s]) for s in range(nu))
+        xs[ii - 1] = dinv(ii) * acc + e
     return xs[0]
+
+
+def chebyshev_weights(lmax: float, nu: int, lower: float = 0.2, safety: float = 1.1) -> list:
+    """Sweep weights of a degree-nu Chebyshev smoother (no reference counterpart).
+
+    nu Jacobi sweeps x += w_i r/d have the error propagator prod_i (I - w_i D^-1 A),
+    a polynomial in D^-1 A; with w_i the reciprocal roots of the Chebyshev
+    polynomial T_nu shifted to [lower b, b], b = safety lmax, it is
+    T_nu((theta - l)/delta) / T_nu(theta/delta) (theta, delta: centre and
+    half-width of the interval), the polynomial of least maximum on it.
+    """
+    b = safety * lmax
+    a = lower * b
+    theta, delta = 0.5 * (b + a), 0.5 * (b - a)
+    return [1.0 / (theta + delta * np.cos((2 * i - 1) * np.pi / (2 * nu))) for i in range(1, nu + 1)]
+
+
+def estimate_lmax(model: Model, ii: int, steps: int = 20, r0=None) -> float:
+    """Top Ritz value of D^-1 A on the free DOFs of level ii from 'steps' Jacobi-
+    preconditioned CG iterations (the CG coefficients give the Lanczos matrix:
+    T_kk = 1/alpha_k + beta_{k-1}/alpha_{k-1}, T_k,k+1 = sqrt(beta_k)/alpha_k).
+    The start vector r0 defaults to a seeded uniform one on the free DOFs."""
+    lv = model.levels[ii - 1]
+    f = lv.fixedDOFs - 1
+    dinv = 1.0 / lv.diagK
+    dinv[f] = 0.0
+    r = (np.random.default_rng(0).uniform(-1, 1, lv.numDOFs) if r0 is None else np.array(r0, float))
+    r[f] = 0.0
+    z = dinv * r
+    rz = r @ z
+    p = z.copy()
+    al, be = [], []
+    for _ in range(steps):
+        q = atx(model, p, ii)
+        q[f] = 0.0
+        pq = p @ q
+        if not pq > 0:
+            break
+        a = rz / pq
+        r = r - a * q
+        z = dinv * r
+        rzn = r @ z
+        al.append(a)
+        be.append(rzn / rz)
+        if not rzn > 1e-24 * rz:
+            break
+        p = z + (rzn / rz) * p
+        rz = rzn
+    n = len(al)
+    T = np.zeros((n, n))
+    for i in range(n):
+        T[i, i] = 1.0 / al[i] + (be[i - 1] / al[i - 1] if i else 0.0)
+        if i + 1 < n:
+            T[i, i + 1] = T[i + 1, i] = np.sqrt(be[i]) / al[i]
+    return float(np.linalg.eigvalsh(T)[-1])
 
 
 @dataclasses.dataclass

@@ -427,12 +427,18 @@ def _free_random(m, rng):
     return v
 
 
+def _cheb_weights(oracle, m, nu):
+    return {ii: oracle.chebyshev_weights(oracle.estimate_lmax(m, ii), nu) for ii in range(1, m.numLevels)}
+
+
+@pytest.mark.parametrize("cheb", [False, True])
 @pytest.mark.parametrize("nu", [1, 2])
-def test_symmetric_vcycle_is_symmetric_and_reference_is_not(oracle, nu):
+def test_symmetric_vcycle_is_symmetric_and_reference_is_not(oracle, nu, cheb):
     m = holey_brick(oracle)
     rng = np.random.default_rng(5)
     a, b = _free_random(m, rng), _free_random(m, rng)
-    Ma, Mb = oracle.vcycle_sym(m, a, nu), oracle.vcycle_sym(m, b, nu)
+    w = _cheb_weights(oracle, m, nu) if cheb else None
+    Ma, Mb = oracle.vcycle_sym(m, a, nu, w), oracle.vcycle_sym(m, b, nu, w)
     assert abs(Ma @ b - a @ Mb) <= 1e-12 * abs(Ma @ b)
     assert a @ Ma > 0 and b @ Mb > 0
     # kernel/Vcycle.m:27-33 post-smooths with the stale residual: not symmetric
@@ -440,10 +446,12 @@ def test_symmetric_vcycle_is_symmetric_and_reference_is_not(oracle, nu):
     assert abs(Ra @ b - a @ Rb) > 1e-8 * abs(Ra @ b)
 
 
+@pytest.mark.parametrize("cheb", [False, True])
 @pytest.mark.parametrize("nu", [1, 2])
-def test_symmetric_vcycle_two_level_error_propagation(oracle, nu):
-    """I - M A = (I - S A)^nu (I - P A_c^-1 P^T A) (I - S A)^nu on the free DOFs,
-    with S = omega/diag, built from dense matrices of the level operators."""
+def test_symmetric_vcycle_two_level_error_propagation(oracle, nu, cheb):
+    """I - M A = G (I - P A_c^-1 P^T A) G on the free DOFs, G = prod_s (I - w_s D^-1 A)
+    (w_s = omega, or the Chebyshev weights), built from dense matrices of the
+    level operators."""
     m = holey_brick(oracle, L=2, shape=(4, 4, 6), seed=3)
     L1, L2 = m.levels
     F = L1.freeDOFs - 1
@@ -458,12 +466,15 @@ def test_symmetric_vcycle_two_level_error_propagation(oracle, nu):
     A = np.stack([oracle.atx(m, unit(j, n), 1)[F] for j in F], axis=1)
     P = np.stack([oracle.interpolate(m, unit(k, nc), 2)[F] for k in Fc], axis=1)
     Ac_inv = np.stack([oracle.coarsest_solve(m, unit(k, nc))[Fc] for k in Fc], axis=1)
-    S = np.diag(m.omega / L1.diagK[F])
+    Dinv = np.diag(1.0 / L1.diagK[F])
     I = np.eye(F.size)
-    Sm = np.linalg.matrix_power(I - S @ A, nu)
+    w = _cheb_weights(oracle, m, nu) if cheb else {1: [m.omega] * nu}
+    Sm = I
+    for ws in w[1]:
+        Sm = Sm @ (I - ws * Dinv @ A)
     E = Sm @ (I - P @ Ac_inv @ P.T @ A) @ Sm
     M_formula = (I - E) @ np.linalg.inv(A)
-    M = np.stack([oracle.vcycle_sym(m, unit(j, n), nu)[F] for j in F], axis=1)
+    M = np.stack([oracle.vcycle_sym(m, unit(j, n), nu, w if cheb else None)[F] for j in F], axis=1)
     assert np.abs(M - M_formula).max() <= 1e-9 * np.abs(M_formula).max()
 
 
@@ -474,3 +485,55 @@ def test_symmetric_cycle_cg_converges_faster(oracle):
                            cycle=lambda mm, r: oracle.vcycle_sym(mm, r, 2))
     assert sym.converged and sym.its < ref.its
     assert np.linalg.norm(sym.x - ref.x) / np.linalg.norm(ref.x) < 1e-8
+
+
+# ---- Chebyshev sweep weights (no reference counterpart) ---------------------
+
+@pytest.mark.parametrize("nu", [1, 2, 3])
+def test_chebyshev_weights_give_the_shifted_chebyshev_polynomial(oracle, nu):
+    """prod_i (1 - w_i l) = T_nu((theta - l)/delta) / T_nu(theta/delta), with T_nu
+    from numpy's Chebyshev series, and |p| equioscillates at 1/T_nu(theta/delta)
+    on [lower b, b]."""
+    from numpy.polynomial import chebyshev as C
+    lmax, lower, safety = 2.7, 0.2, 1.1
+    w = oracle.chebyshev_weights(lmax, nu, lower, safety)
+    b = safety * lmax
+    a = lower * b
+    th, de = 0.5 * (b + a), 0.5 * (b - a)
+    Tn = [0.0] * nu + [1.0]
+    lam = np.linspace(0.0, 1.3 * b, 101)
+    p = np.prod([1.0 - wi * lam for wi in w], axis=0)
+    ref = C.chebval((th - lam) / de, Tn) / C.chebval(th / de, Tn)
+    assert np.abs(p - ref).max() < 1e-12
+    inside = np.linspace(a, b, 2001)
+    pin = np.abs(np.prod([1.0 - wi * inside for wi in w], axis=0))
+    assert abs(pin.max() - 1.0 / C.chebval(th / de, Tn)) < 1e-9
+
+
+def test_lanczos_estimate_is_a_lower_bound_close_to_lambda_max(oracle):
+    """The top Ritz value never exceeds lambda_max(D^-1 A) on the free DOFs
+    (dense eigenvalues of D^-1/2 A D^-1/2) and reaches it within 1% here."""
+    m = holey_brick(oracle, L=2, shape=(4, 4, 6), seed=3)
+    L1 = m.levels[0]
+    F = L1.freeDOFs - 1
+    n = L1.numDOFs
+    A = np.stack([oracle.atx(m, np.eye(1, n, j).ravel(), 1)[F] for j in F], axis=1)
+    s = 1.0 / np.sqrt(L1.diagK[F])
+    lam = np.linalg.eigvalsh(s[:, None] * A * s[None, :])[-1]
+    prev = 0.0
+    for steps in (2, 5, 10, 20):
+        est = oracle.estimate_lmax(m, 1, steps)
+        assert est <= lam * (1 + 1e-12) and est >= prev - 1e-12
+        prev = est
+    assert prev >= 0.99 * lam
+
+
+def test_chebyshev_cycle_cg_converges_faster(oracle):
+    m = holey_brick(oracle)
+    jac = oracle.cg_solver(m, m.F, 1e-10, 500, jacobi_step=False,
+                           cycle=lambda mm, r: oracle.vcycle_sym(mm, r, 2))
+    w = _cheb_weights(oracle, m, 2)
+    che = oracle.cg_solver(m, m.F, 1e-10, 500, jacobi_step=False,
+                           cycle=lambda mm, r: oracle.vcycle_sym(mm, r, 2, w))
+    assert che.converged and che.its < jac.its
+    assert np.linalg.norm(che.x - jac.x) / np.linalg.norm(jac.x) < 1e-8
