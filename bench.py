@@ -97,13 +97,13 @@ def run_batch(args, pkg):
     probs = pkg.workloads.batch_c4(args.seed, 64)
     log(f"[bench] generated 64 problems in {time.perf_counter() - t0:.1f}s")
     flags = (pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE | pkg.FLAG_SYM_VCYCLE
-             | pkg.FLAG_NO_JACOBI_STEP | pkg.FLAG_CHEBYSHEV)
+             | pkg.FLAG_NO_JACOBI_STEP | pkg.FLAG_CHEBYSHEV | pkg.FLAG_EXACT_GALERKIN)
     t0 = time.perf_counter()
     solvers = []
     for p in probs:
         s = pkg.Solver(p.vox, p.bbox, num_levels=p.num_levels, jacobi_weight=p.omega, tol=1e-6,
                        max_it=args.max_it, precision=pkg.GMGS_PREC_MIXED, flags=flags,
-                       vcycle_sweeps=2)
+                       vcycle_sweeps=2, vcycle_sweeps_coarse=4)
         s.set_bc(p.fixed, p.loads)
         solvers.append(s)
     setup_s = time.perf_counter() - t0
@@ -144,8 +144,8 @@ def run_batch(args, pkg):
         "data": "synthetic (64 seeded uneven ellipsoid-union solids)",
         "config": {"workload": f"batch64s{args.seed}", "problems": 64, "dofs": total,
                    "dofs_min": int(min(dofs)), "dofs_max": int(max(dofs)), "rtol": 1e-6,
-                   "mixed": ("FP32 symmetric V(2,2) cycle, Chebyshev sweep weights, no Jacobi step, "
-                             "FP64 flexible PCG"),
+                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 2 on level 1, 4 below), "
+                             "exact Galerkin coarse operators, no Jacobi step, FP64 flexible PCG"),
                    "sequential_ms": round(ms_seq, 3),
                    "speedup_vs_sequential": round(ms_seq / ms_batch, 3),
                    "batch_vs_individual_max_rel_dev": worst,
@@ -193,13 +193,14 @@ def main():
                   max_it=args.max_it)
     base_flags = pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE
     sym_flags = base_flags | pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP
-    cheb_flags = sym_flags | pkg.FLAG_CHEBYSHEV
+    cheb_flags = sym_flags | pkg.FLAG_CHEBYSHEV | pkg.FLAG_EXACT_GALERKIN
+    cheb_sweeps = dict(vcycle_sweeps=2, vcycle_sweeps_coarse=4)
 
-    # headline: FP32 symmetric V(2,2) cycle with Chebyshev sweep weights inside
-    # the FP64 flexible PCG
+    # headline: FP32 symmetric cycle with Chebyshev sweeps (degree 2 on level 1,
+    # 4 below) on exact Galerkin coarse operators, inside the FP64 flexible PCG
     t0 = time.perf_counter()
     s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, flags=cheb_flags,
-                   vcycle_sweeps=2, **common)
+                   **cheb_sweeps, **common)
     s.set_bc(prob.fixed, prob.loads)
     setup_s = time.perf_counter() - t0
     nn = s.num_nodes(1)
@@ -211,8 +212,9 @@ def main():
     ms_per_step, info = timed_solves(s, args.steps, "mixed")
     value = ndofs / (ms_per_step * 1e-3) / 1e6
 
-    def other(tag, precision, flags):
-        o = pkg.Solver(prob.vox, prob.bbox, precision=precision, flags=flags, vcycle_sweeps=2, **common)
+    def other(tag, precision, flags, sweeps=None):
+        o = pkg.Solver(prob.vox, prob.bbox, precision=precision, flags=flags,
+                       **(sweeps or dict(vcycle_sweeps=2)), **common)
         o.set_bc(prob.fixed, prob.loads)
         Uo, _, _ = o.solve(raise_on_nonconvergence=False)
         ms, inf = timed_solves(o, args.steps, tag)
@@ -225,9 +227,9 @@ def main():
     err = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
     # the same symmetric cycle in FP64 (precision-only comparison) and the
     # reference cycle in FP32
-    _, ms64s, info64s = other("fp64-cheb", pkg.GMGS_PREC_FP64, cheb_flags)
+    _, ms64s, info64s = other("fp64-cheb", pkg.GMGS_PREC_FP64, cheb_flags, cheb_sweeps)
     _, msmr, infomr = other("mixed-ref", pkg.GMGS_PREC_MIXED, base_flags)
-    # the symmetric cycle with the plain Jacobi weight in every sweep
+    # the symmetric V(2,2) cycle with the plain Jacobi weight in every sweep
     _, msmj, infomj = other("mixed-sym-jacobi", pkg.GMGS_PREC_MIXED, sym_flags)
     log(f"[bench] mixed vs fp64 rel L2 {err:.2e}; speed-up {ms64 / ms_per_step:.3f} "
         f"(precision only {ms64s / ms_per_step:.3f})")
@@ -278,8 +280,8 @@ def main():
         "data": "synthetic (seeded ellipsoid-union voxel solid)",
         "config": {"workload": prob.name, "dofs": ndofs, "levels": s.num_levels,
                    "rtol": 1e-6,
-                   "mixed": ("FP32 symmetric V(2,2) cycle, Chebyshev sweep weights, no Jacobi step, "
-                             "FP64 flexible PCG"),
+                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 2 on level 1, 4 below), "
+                             "exact Galerkin coarse operators, no Jacobi step, FP64 flexible PCG"),
                    "fp64": "reference algorithm: kernel/Vcycle.m V-cycle, Jacobi step, FP64 PCG",
                    "its_mixed": info.its, "its_fp64": info64.its,
                    "converged_mixed": bool(info.converged), "converged_fp64": bool(info64.converged),

@@ -46,7 +46,8 @@ struct Level {
   DevBuf uniq;                   // int32 compact ids of unique elements (state 2)
   long long nuniq = 0;
   DevBuf r, rp, x;               // V-cycle work vectors (precision of the cycle)
-  DevBuf w1, w2, e;              // symmetric cycle: second sweep, post-smoothing residuals, P x_c
+  DevBuf wk[2 * kMaxSweeps - 2]; // symmetric cycle: smoothed residuals (see Rbuf, Qbuf)
+  DevBuf e;                      // symmetric cycle: P x_c
   double scale = 1.0;            // Ke_l = Ke0 * scale
   // z-marching operator (levels with a universal element matrix)
   bool zm = false;
@@ -54,9 +55,10 @@ struct Level {
   DevBuf colz;                   // per padded column: solid-node z range
   long long ncol = 0;
   int col_tx = 8, col_zc = 16;
-  // symmetric cycle: weights of the nu smoothing sweeps, and the estimate of
+  // symmetric cycle: nu sweeps with weights om, and the estimate of
   // lambda_max(D^-1 A) on the free DOFs they came from (0 = plain Jacobi)
-  double om[2] = {0.0, 0.0};
+  int nu = 2;
+  double om[kMaxSweeps] = {0.0, 0.0, 0.0, 0.0};
   double lmax = 0.0;
 };
 
@@ -80,7 +82,7 @@ struct gmgs_ctx {
   bool simple_kernels = false;
   bool autotune = false;
   bool sym = false;              // symmetric V(nu, nu) cycle
-  int sweeps = 2;
+  int sweeps = 2, sweeps_coarse = 2;   // nu of level 1, of the coarser smoothed levels
   bool cheb = false;             // Chebyshev sweep weights in the symmetric cycle
   bool no_jacobi = false;        // no Jacobi step opening each PCG iteration
   // SIMP
@@ -623,8 +625,8 @@ void alloc_vectors(gmgs_ctx* c) {
       if (li < c->L - 1) l.rp.alloc_zero(es * n3, s);
     }
     if (c->sym && li < c->L - 1) {
-      l.w1.alloc_zero(es * n3, s);
-      l.w2.alloc_zero(es * n3, s);
+      const int nb = std::max(2 * l.nu - 2, 2);
+      for (int k = 0; k < nb; ++k) l.wk[k].alloc_zero(es * n3, s);
       l.e.alloc_zero(es * n3, s);
     }
   }
@@ -706,6 +708,18 @@ void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partia
   }
 }
 
+// buffers of the symmetric cycle on a level with nu sweeps: R_1 in rp, R_s
+// (s >= 2) in wk[s - 2], Q_0 in wk[nu - 1], Q_1 in the buffer of R_nu (free
+// once Q_0 exists), Q_s (s >= 2) in wk[nu + s - 2]
+template <typename T>
+T* Rbuf(Level& l, int s) { return s == 1 ? vec<T>(l.rp) : vec<T>(l.wk[s - 2]); }
+template <typename T>
+T* Qbuf(Level& l, int s) {
+  if (s == 0) return vec<T>(l.wk[l.nu - 1]);
+  if (s == 1) return Rbuf<T>(l, l.nu);
+  return vec<T>(l.wk[l.nu + s - 2]);
+}
+
 // Symmetric V(nu, nu) cycle (no reference counterpart): nu pre-smoothing
 // sweeps R_s = R_{s-1} - A(w_{s-1} R_{s-1}/d) from the level's input R_0, the
 // coarse correction e = P x_c, the residual recomputed as Q_0 = R_nu - A e
@@ -719,25 +733,24 @@ void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partia
 template <typename T>
 void vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
   hipStream_t s = c->stream;
-  const int L = c->L, nu = c->sweeps;
+  const int L = c->L;
   for (int li = 0; li < L - 1; ++li) {
     Level& l = c->lev[li];
-    const T w0 = (T)l.om[0], w1 = (T)l.om[1];
+    const int nu = l.nu;
     {
       KTime t(c, li == 0 ? 2 : -1);
-      if (li == 0) apply_level<T, double>(c, 0, AP_SMOOTH, rin, w0, vec<T>(l.rp), nullptr, nullptr, done,
-                                        nullptr, true);
-      else apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.r), w0, vec<T>(l.rp), nullptr, nullptr, done,
-                                 nullptr, true);
-      if (nu == 2)
-        apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.rp), w1, vec<T>(l.w1), nullptr, nullptr, done,
-                          nullptr, true);
+      if (li == 0) apply_level<T, double>(c, 0, AP_SMOOTH, rin, (T)l.om[0], Rbuf<T>(l, 1), nullptr, nullptr,
+                                        done, nullptr, true);
+      else apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.r), (T)l.om[0], Rbuf<T>(l, 1), nullptr, nullptr, done,
+                             nullptr, true);
+      for (int k = 1; k < nu; ++k)
+        apply_level<T, T>(c, li, AP_SMOOTH, Rbuf<T>(l, k), (T)l.om[k], Rbuf<T>(l, k + 1), nullptr, nullptr,
+                          done, nullptr, true);
     }
-    T* Rnu = nu == 2 ? vec<T>(l.w1) : vec<T>(l.rp);
     Level& nx = c->lev[li + 1];
     KTime t(c, li == 0 ? 3 : -1);
     launch_restrict<T>(nx.mesh.g, nx.mesh.flags.as<uint32_t>(), nx.tiles.as<TileRec>(), nx.ntiles,
-                       l.mesh.g, Rnu, vec<T>(nx.r), done, s);
+                       l.mesh.g, Rbuf<T>(l, nu), vec<T>(nx.r), done, s);
   }
   {
     Level& lc = c->lev[L - 1];
@@ -748,27 +761,31 @@ void vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* pa
   for (int li = L - 2; li >= 0; --li) {
     Level& l = c->lev[li];
     Level& cr = c->lev[li + 1];
+    const int nu = l.nu;
     KTime t(c, li == 0 ? 4 : -1);
     const Grid& g = l.mesh.g;
     const uint32_t* fl = l.mesh.flags.as<uint32_t>();
     const TileRec* tl = l.tiles.as<TileRec>();
     const T* invd = make_op<T>(c, li, true).invd;
-    const T w0 = (T)l.om[0], w1 = (T)l.om[1];
-    T* Rnu = nu == 2 ? vec<T>(l.w1) : vec<T>(l.rp);
     launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
-    apply_level<T, T>(c, li, AP_RESID, vec<T>(l.e), (T)0, vec<T>(l.w2), nullptr, nullptr, done, Rnu, true);
-    if (nu == 2)
-      apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.w2), w0, vec<T>(l.w1), nullptr, nullptr, done,
-                        nullptr, true);
-    const T* a = nu == 2 ? vec<T>(l.rp) : nullptr;       // R_1
-    const T* b = vec<T>(l.w2);                            // Q_0
-    const T* q1 = nu == 2 ? vec<T>(l.w1) : nullptr;      // Q_1
+    apply_level<T, T>(c, li, AP_RESID, vec<T>(l.e), (T)0, Qbuf<T>(l, 0), nullptr, nullptr, done, Rbuf<T>(l, nu),
+                      true);
+    for (int k = 1; k < nu; ++k)
+      apply_level<T, T>(c, li, AP_SMOOTH, Qbuf<T>(l, k - 1), (T)l.om[k - 1], Qbuf<T>(l, k), nullptr, nullptr,
+                        done, nullptr, true);
+    CycleSum<T> cs{};
+    cs.nu = nu;
+    for (int k = 0; k < nu; ++k) {
+      cs.w[k] = (T)l.om[k];
+      cs.R[k] = k ? Rbuf<T>(l, k) : nullptr;
+      cs.Q[k] = Qbuf<T>(l, k);
+    }
     if (li == 0)
-      launch_combine<T, double, T>(g, fl, tl, l.ntiles, invd, w0, w1, rin, a, b, q1, vec<T>(l.e), z, true,
-                                   zprev, partial, done, s);
+      launch_combine<T, double, T>(g, fl, tl, l.ntiles, invd, cs, rin, vec<T>(l.e), z, true, zprev, partial,
+                                   done, s);
     else
-      launch_combine<T, T, T>(g, fl, tl, l.ntiles, invd, w0, w1, vec<T>(l.r), a, b, q1, vec<T>(l.e),
-                              vec<T>(l.x), false, (const T*)nullptr, nullptr, done, s);
+      launch_combine<T, T, T>(g, fl, tl, l.ntiles, invd, cs, vec<T>(l.r), vec<T>(l.e), vec<T>(l.x), false,
+                              (const T*)nullptr, nullptr, done, s);
   }
 }
 
@@ -1013,8 +1030,8 @@ double estimate_lmax(gmgs_ctx* c, int li) {
   const T* invd = make_op<T>(c, li, true).invd;
   T* r = vec<T>(l.e);
   T* z = vec<T>(l.rp);
-  T* p = vec<T>(l.w1);
-  T* q = vec<T>(l.w2);
+  T* p = vec<T>(l.wk[0]);
+  T* q = vec<T>(l.wk[1]);
   double* part = c->part.as<double>();
   auto total = [&](long long np) {
     const std::vector<double> h = download<double>(c->part, (size_t)np, s);
@@ -1062,7 +1079,7 @@ void smoother_weights(gmgs_ctx* c) {
   const double pi = 3.14159265358979323846;
   for (int li = 0; li < c->L - 1; ++li) {
     Level& l = c->lev[li];
-    l.om[0] = l.om[1] = c->prm.jacobi_weight;
+    for (int k = 0; k < kMaxSweeps; ++k) l.om[k] = k < l.nu ? c->prm.jacobi_weight : 0.0;
     l.lmax = 0.0;
     if (!(c->sym && c->cheb)) continue;
     const double lm = c->mixed ? estimate_lmax<float>(c, li) : estimate_lmax<double>(c, li);
@@ -1070,9 +1087,8 @@ void smoother_weights(gmgs_ctx* c) {
     l.lmax = lm;
     const double b = kChebSafety * lm, a = kChebLower * b;
     const double theta = 0.5 * (b + a), delta = 0.5 * (b - a);
-    for (int i = 1; i <= c->sweeps; ++i)
-      l.om[i - 1] = 1.0 / (theta + delta * std::cos((2 * i - 1) * pi / (2 * c->sweeps)));
-    if (c->sweeps == 1) l.om[1] = l.om[0];
+    for (int i = 1; i <= l.nu; ++i)
+      l.om[i - 1] = 1.0 / (theta + delta * std::cos((2 * i - 1) * pi / (2 * l.nu)));
   }
   GMGS_HIP(hipStreamSynchronize(c->stream));
 }
@@ -1245,7 +1261,8 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     if (!(p->poisson_ratio > -1.0 && p->poisson_ratio < 0.5) || !(p->youngs_modulus > 0) ||
         !(p->tol > 0) || p->max_it < 0 || !(p->jacobi_weight > 0) || p->num_levels < 0 ||
         (p->precision != GMGS_PREC_FP64 && p->precision != GMGS_PREC_MIXED) ||
-        p->vcycle_sweeps < 0 || p->vcycle_sweeps > 2)
+        p->vcycle_sweeps < 0 || p->vcycle_sweeps > kMaxSweeps || p->vcycle_sweeps_coarse < 0 ||
+        p->vcycle_sweeps_coarse > kMaxSweeps)
       throw Status(GMGS_ERR_ARGUMENT, "invalid solver parameter");
     c->device = p->device;
     GMGS_HIP(hipSetDevice(c->device));
@@ -1262,6 +1279,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->use_graphs = (p->flags & GMGS_FLAG_NO_GRAPHS) == 0;
     c->sym = (p->flags & GMGS_FLAG_SYM_VCYCLE) != 0;
     c->sweeps = p->vcycle_sweeps == 0 ? 2 : p->vcycle_sweeps;
+    c->sweeps_coarse = p->vcycle_sweeps_coarse == 0 ? c->sweeps : p->vcycle_sweeps_coarse;
     c->no_jacobi = (p->flags & GMGS_FLAG_NO_JACOBI_STEP) != 0;
     c->cheb = (p->flags & GMGS_FLAG_CHEBYSHEV) != 0;
     if (c->cheb && !c->sym)
@@ -1280,6 +1298,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     for (int d = 0; d < 3; ++d)
       if (!(c->bbox[3 + d] > c->bbox[d])) throw Status(GMGS_ERR_ARGUMENT, "empty bounding box");
     build_mesh(c.get(), vox);
+    for (int li = 0; li < c->L; ++li) c->lev[li].nu = li == 0 ? c->sweeps : c->sweeps_coarse;
     double probe[8][3];
     probe_coords(c.get(), probe);
     element_stiffness(p->youngs_modulus, p->poisson_ratio, probe, c->Ke0);
@@ -1292,7 +1311,8 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     upload(c->stressM, c->M, 48 * 24, c->stream);
     build_operators(c.get());
     alloc_vectors(c.get());
-    for (Level& l : c->lev) l.om[0] = l.om[1] = p->jacobi_weight;
+    for (Level& l : c->lev)
+      for (int k = 0; k < kMaxSweeps; ++k) l.om[k] = k < l.nu ? p->jacobi_weight : 0.0;
     if (c->autotune) autotune_levels(c.get());
     return GMGS_OK;
   });
@@ -1803,7 +1823,7 @@ int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count) {
       case GMGS_Q_SMOOTHER:
         if (level >= c->L) throw Status(GMGS_ERR_ARGUMENT, "the coarsest level is not smoothed");
         if (!c->bc) throw Status(GMGS_ERR_STATE, "boundary conditions not set");
-        n = 3;
+        n = 1 + kMaxSweeps;
         break;
       default: throw Status(GMGS_ERR_ARGUMENT, "unknown query key");
     }
@@ -1892,7 +1912,10 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
       }
       case GMGS_Q_KE0: std::memcpy(f64, c->Ke0, sizeof(double) * 576); break;
       case GMGS_Q_BBOX: std::memcpy(f64, c->bbox, sizeof(double) * 6); break;
-      case GMGS_Q_SMOOTHER: f64[0] = l->lmax; f64[1] = l->om[0]; f64[2] = l->om[1]; break;
+      case GMGS_Q_SMOOTHER:
+        f64[0] = l->lmax;
+        for (int k = 0; k < kMaxSweeps; ++k) f64[1 + k] = l->om[k];
+        break;
       case GMGS_Q_F:
         if (!c->bc || level != 1) throw Status(GMGS_ERR_STATE, "F_ exists on level 1 after gmgs_set_bc");
         std::memcpy(f64, c->F.data(), sizeof(double) * c->F.size());

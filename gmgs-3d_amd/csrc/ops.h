@@ -81,13 +81,24 @@ template <typename T>
 void launch_mask_fixed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                        const T* src, T* dst, hipStream_t s);
 
-// x = (w0 (r0 + b) + w1 (a + c)) / d + e (null a, b, c skipped), fixed
-// components zeroed if 'fix'; optional partials of r0.x and r0.zprev (FP64)
+// terms of the symmetric cycle's sum on one level: R_s (1 <= s < nu) and Q_s
+// (0 <= s < nu) the smoothed residuals, w_s the sweep weights (R[0] unused:
+// R_0 is the level's input, passed on its own)
+constexpr int kMaxSweeps = 4;
+template <typename T>
+struct CycleSum {
+  const T* R[kMaxSweeps];
+  const T* Q[kMaxSweeps];
+  T w[kMaxSweeps];
+  int nu;
+};
+
+// x = (sum_s w_s (R_s + Q_s)) / d + e, R_0 = r0; fixed components zeroed if
+// 'fix'; optional partials of r0.x and r0.zprev (FP64)
 template <typename T, typename TR, typename TZ>
 void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
-                    const T* invd, T w0, T w1, const TR* r0, const T* a, const T* b, const T* c,
-                    const T* e, TZ* x, bool fix, const TZ* zprev, double* partial, const int* done,
-                    hipStream_t s);
+                    const T* invd, const CycleSum<T>& cs, const TR* r0, const T* e, TZ* x, bool fix,
+                    const TZ* zprev, double* partial, const int* done, hipStream_t s);
 
 template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,

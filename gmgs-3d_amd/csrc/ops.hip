@@ -293,15 +293,14 @@ __global__ __launch_bounds__(256) void k_mask_fixed(Grid g, const uint32_t* __re
   }
 }
 
-// x = (w0 (r0 + b) + w1 (a + c)) / d + e on the solid nodes (null a, b, c
-// skipped); zero fixed components if 'fix'; partials of r0.x and r0.zprev
+// x = (sum_s w_s (R_s + Q_s)) / d + e on the solid nodes, R_0 = r0; zero
+// fixed components if 'fix'; partials of r0.x and r0.zprev
 template <typename T, typename TR, typename TZ>
 __global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restrict__ flags,
                                                  const TileRec* __restrict__ tiles, long long nt,
-                                                 const T* __restrict__ invd, T w0, T w1,
-                                                 const TR* __restrict__ r0, const T* __restrict__ a,
-                                                 const T* __restrict__ b, const T* __restrict__ cc,
-                                                 const T* __restrict__ e, TZ* __restrict__ x, int fix,
+                                                 const T* __restrict__ invd, CycleSum<T> cs,
+                                                 const TR* __restrict__ r0, const T* __restrict__ e,
+                                                 TZ* __restrict__ x, int fix,
                                                  const TZ* __restrict__ zprev,
                                                  double* __restrict__ partial,
                                                  const int* __restrict__ done) {
@@ -314,11 +313,11 @@ __global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restr
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long m = c * N + n;
-      T s0 = (T)r0[m], s1 = 0;
-      if (b) s0 += b[m];
-      if (a) s1 += a[m];
-      if (cc) s1 += cc[m];
-      T v = invd[m] * (w0 * s0 + w1 * s1) + e[m];
+      T acc = cs.w[0] * ((T)r0[m] + cs.Q[0][m]);
+#pragma unroll
+      for (int k = 1; k < kMaxSweeps; ++k)
+        if (k < cs.nu) acc += cs.w[k] * (cs.R[k][m] + cs.Q[k][m]);
+      T v = invd[m] * acc + e[m];
       if (fix && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) v = 0;
       x[m] = (TZ)v;
       if (partial) {
@@ -794,26 +793,22 @@ template void launch_mask_fixed<float>(const Grid&, const uint32_t*, const TileR
 
 template <typename T, typename TR, typename TZ>
 void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
-                    const T* invd, T w0, T w1, const TR* r0, const T* a, const T* b, const T* c,
-                    const T* e, TZ* x, bool fix, const TZ* zprev, double* partial, const int* done,
-                    hipStream_t s) {
+                    const T* invd, const CycleSum<T>& cs, const TR* r0, const T* e, TZ* x, bool fix,
+                    const TZ* zprev, double* partial, const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_combine<T, TR, TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, w0, w1, r0, a, b, c,
-                                                             e, x, fix ? 1 : 0, zprev, partial, done);
+  if (cs.nu < 1 || cs.nu > kMaxSweeps) throw std::invalid_argument("combine: sweeps out of range");
+  k_combine<T, TR, TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, cs, r0, e, x,
+                                                             fix ? 1 : 0, zprev, partial, done);
   GMGS_LAUNCH_CHECK();
 }
-template void launch_combine<double, double, double>(const Grid&, const uint32_t*, const TileRec*, long long,
-                                                     const double*, double, double, const double*, const double*,
-                                                     const double*, const double*, const double*, double*,
-                                                     bool, const double*, double*, const int*, hipStream_t);
-template void launch_combine<float, float, float>(const Grid&, const uint32_t*, const TileRec*, long long,
-                                                  const float*, float, float, const float*, const float*,
-                                                  const float*, const float*, const float*, float*, bool,
-                                                  const float*, double*, const int*, hipStream_t);
-template void launch_combine<float, double, float>(const Grid&, const uint32_t*, const TileRec*, long long,
-                                                   const float*, float, float, const double*, const float*,
-                                                   const float*, const float*, const float*, float*, bool,
-                                                   const float*, double*, const int*, hipStream_t);
+#define GMGS_INST_COMBINE(T, TR, TZ)                                                                \
+  template void launch_combine<T, TR, TZ>(const Grid&, const uint32_t*, const TileRec*, long long,  \
+                                          const T*, const CycleSum<T>&, const TR*, const T*, TZ*,   \
+                                          bool, const TZ*, double*, const int*, hipStream_t);
+GMGS_INST_COMBINE(double, double, double)
+GMGS_INST_COMBINE(float, float, float)
+GMGS_INST_COMBINE(float, double, float)
+#undef GMGS_INST_COMBINE
 
 template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,

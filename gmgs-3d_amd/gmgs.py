@@ -67,7 +67,7 @@ class Params(C.Structure):
                 ("jacobi_weight", C.c_double), ("coarsest_resolution_control", C.c_double),
                 ("max_it", C.c_int32), ("num_levels", C.c_int32), ("precision", C.c_int32),
                 ("flags", C.c_int32), ("device", C.c_int32), ("vcycle_sweeps", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("vcycle_sweeps_coarse", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class SolveInfo(C.Structure):
@@ -226,10 +226,10 @@ class Solver:
         return self.query("NUM_LEVELS")
 
     def smoother(self, level: int = 1):
-        """(lambda_max estimate of D^-1 A on the free DOFs or 0, w1, w2): the sweep
-        weights of the symmetric cycle on a smoothed level, after set_bc."""
-        lm, w1, w2 = self.query("SMOOTHER", level)
-        return float(lm), float(w1), float(w2)
+        """(lambda_max estimate of D^-1 A on the free DOFs or 0, [w_1 .. w_nu]): the
+        sweep weights of the symmetric cycle on a smoothed level, after set_bc."""
+        q = self.query("SMOOTHER", level)
+        return float(q[0]), [float(w) for w in q[1:] if w != 0.0]
 
     def num_nodes(self, level=1) -> int:
         return self.query("NUM_NODES", level)

@@ -86,8 +86,9 @@ typedef struct gmgs_params {
   int32_t precision;                   /* GMGS_PREC_FP64 or GMGS_PREC_MIXED */
   int32_t flags;                       /* GMGS_FLAG_* */
   int32_t device;                      /* HIP device ordinal */
-  int32_t vcycle_sweeps;               /* nu of GMGS_FLAG_SYM_VCYCLE: 1 or 2 (0 = 2) */
-  int32_t reserved[6];
+  int32_t vcycle_sweeps;               /* nu of GMGS_FLAG_SYM_VCYCLE on level 1: 1..4 (0 = 2) */
+  int32_t vcycle_sweeps_coarse;        /* nu on levels 2..numLevels-1: 1..4 (0 = vcycle_sweeps) */
+  int32_t reserved[5];
 } gmgs_params;
 
 typedef struct gmgs_solve_info {
@@ -130,8 +131,9 @@ enum {
   GMGS_Q_NUM_TILES = 26,            /* int64 active 16x4x1 tiles of the level */
   GMGS_Q_NUM_FREE_COARSEST = 27,    /* int64 */
   GMGS_Q_TILE_CONFIG = 28,          /* int64[3]: z-marching on?, tile TX, z-chunk */
-  GMGS_Q_SMOOTHER = 29              /* double[3] (levels < numLevels, after gmgs_set_bc):
-                                       lambda_max(D^-1 A) estimate (0 = none), sweep weights w1, w2 */
+  GMGS_Q_SMOOTHER = 29              /* double[5] (levels < numLevels, after gmgs_set_bc):
+                                       lambda_max(D^-1 A) estimate (0 = none), the weights of the
+                                       level's nu sweeps, 0 beyond nu */
 };
 
 void gmgs_default_params(gmgs_params* p);

@@ -711,8 +711,9 @@ def vcycle_sym(model: Model, r: np.ndarray, nu: int = 2, weights=None) -> np.nda
     its support (SetupSolver.m:247-254), so fixed-DOF residual values only ever
     reach coarse fixed DOFs and P x_c vanishes on fine fixed DOFs; the other
     two masks are kept as plain statements of the intent.
-    weights[ii] lists the nu sweep weights w_s of level ii (default: the
-    Jacobi weight of JacobiSmoother.m for every sweep).  In the form the device
+    nu is an int or a dict of per-level counts; weights[ii], if given, lists
+    the sweep weights w_s of level ii and sets its count (default: the Jacobi
+    weight of JacobiSmoother.m for every sweep).  In the form the device
     evaluates it: R_s = R_{s-1} - A (w_{s-1} R_{s-1}/d), e = P x_c,
     Q_0 = R_nu - A e, Q_s = Q_{s-1} - A (w_{s-1} Q_{s-1}/d),
     x = sum_{s<nu} w_s (R_s + Q_s)/d + e.
@@ -732,7 +733,9 @@ def vcycle_sym(model: Model, r: np.ndarray, nu: int = 2, weights=None) -> np.nda
         return w
 
     def wts(ii):
-        return list(weights[ii]) if weights is not None else [model.omega] * nu
+        if weights is not None:
+            return list(weights[ii])
+        return [model.omega] * (nu[ii] if isinstance(nu, dict) else nu)
 
     def smooth(rr, ii, w):
         out = rr - atx(model, w * dinv(ii) * rr, ii)
@@ -742,20 +745,20 @@ def vcycle_sym(model: Model, r: np.ndarray, nu: int = 2, weights=None) -> np.nda
     for ii in range(1, Lc):
         w = wts(ii)
         R = [rin[ii - 1]]
-        for s in range(nu):
+        for s in range(len(w)):
             R.append(smooth(R[-1], ii, w[s]))
         Rs[ii - 1] = R
-        rin[ii] = restrict(model, R[nu], ii + 1)
+        rin[ii] = restrict(model, R[-1], ii + 1)
     xs[Lc - 1] = coarsest_solve(model, rin[Lc - 1])
     for ii in range(Lc - 1, 0, -1):
         w = wts(ii)
         R = Rs[ii - 1]
         e = interpolate(model, xs[ii], ii + 1)
         e[fixed(ii)] = 0.0
-        Q = [R[nu] - atx(model, e, ii)]
-        for s in range(nu - 1):
+        Q = [R[-1] - atx(model, e, ii)]
+        for s in range(len(w) - 1):
             Q.append(smooth(Q[-1], ii, w[s]))
-        acc = sum(w[s] * (R[s] + Q[s]) for s in range(nu))
+        acc = sum(w[s] * (R[s] + Q[s]) for s in range(len(w)))
         xs[ii - 1] = dinv(ii) * acc + e
     return xs[0]
 

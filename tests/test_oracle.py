@@ -432,7 +432,7 @@ def _cheb_weights(oracle, m, nu):
 
 
 @pytest.mark.parametrize("cheb", [False, True])
-@pytest.mark.parametrize("nu", [1, 2])
+@pytest.mark.parametrize("nu", [1, 2, 3])
 def test_symmetric_vcycle_is_symmetric_and_reference_is_not(oracle, nu, cheb):
     m = holey_brick(oracle)
     rng = np.random.default_rng(5)
@@ -447,7 +447,7 @@ def test_symmetric_vcycle_is_symmetric_and_reference_is_not(oracle, nu, cheb):
 
 
 @pytest.mark.parametrize("cheb", [False, True])
-@pytest.mark.parametrize("nu", [1, 2])
+@pytest.mark.parametrize("nu", [1, 2, 3])
 def test_symmetric_vcycle_two_level_error_propagation(oracle, nu, cheb):
     """I - M A = G (I - P A_c^-1 P^T A) G on the free DOFs, G = prod_s (I - w_s D^-1 A)
     (w_s = omega, or the Chebyshev weights), built from dense matrices of the
