@@ -54,6 +54,9 @@ struct Level {
   DevBuf coltiles;
   DevBuf colz;                   // per padded column: solid-node z range
   long long ncol = 0;
+  // assembled 27-point stencil (coarse levels small enough): built, and in use
+  DevBuf asc64, asc32;
+  bool as_built = false, as = false;
   int col_tx = 8, col_zc = 16;
   // symmetric cycle: nu sweeps with weights om, and the estimate of
   // lambda_max(D^-1 A) on the free DOFs they came from (0 = plain Jacobi)
@@ -84,6 +87,7 @@ struct gmgs_ctx {
   bool sym = false;              // symmetric V(nu, nu) cycle
   int sweeps = 2, sweeps_coarse = 2;   // nu of level 1, of the coarser smoothed levels
   bool cheb = false;             // Chebyshev sweep weights in the symmetric cycle
+  bool prefer_as = false;        // assembled stencil on every coarse level where it fits
   bool no_jacobi = false;        // no Jacobi step opening each PCG iteration
   // SIMP
   bool simp = false;
@@ -177,6 +181,7 @@ LevelOp<T> make_op(gmgs_ctx* c, int li, bool cyc = false) {
   Level& l = c->lev[li];
   LevelOp<T> op;
   op.zero_fixed = 0;
+  op.asc = nullptr;
   op.g = l.mesh.g;
   op.flags = l.mesh.flags.as<uint32_t>();
   op.colz = l.colz.empty() ? nullptr : l.colz.as<int>();
@@ -197,6 +202,10 @@ LevelOp<T> make_op(gmgs_ctx* c, int li, bool cyc = false) {
     op.corner = l.Kc32.empty() ? nullptr : l.Kc32.as<float>();
     op.invd = l.invd32.empty() ? nullptr : l.invd32.as<float>();
     op.emod = (li == 0 && c->simp) ? c->emod32.as<float>() : nullptr;
+  }
+  if (l.as) {
+    if constexpr (std::is_same<T, double>::value) op.asc = l.asc64.as<double>();
+    else op.asc = l.asc32.as<float>();
   }
   if (cyc && c->sym && !l.invdm.empty()) {
     op.invd = l.invdm.as<T>();
@@ -356,6 +365,9 @@ void upload_level_constants(gmgs_ctx* c, int li) {
   }
 }
 
+// largest FP64 assembled stencil of one level (bytes)
+constexpr double kAscBudgetBytes = 4.0e9;
+
 // coarse element matrices and storing states (SetupSolver.m:34-196)
 void build_operators(gmgs_ctx* c) {
   hipStream_t s = c->stream;
@@ -473,6 +485,26 @@ void build_operators(gmgs_ctx* c) {
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
       build_colrange(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.colz, s);
     }
+    // assembled stencil on the smoothed coarse levels that are small enough
+    // (assembled in FP64); used where the z-marching kernel is not
+    l.as = l.as_built = false;
+    l.asc64.release();
+    l.asc32.release();
+    const long long asc_n = (long long)kStencilCoefs * 64 * l.ntiles;
+    if (li >= 1 && li < L - 1 && !c->simple_kernels && l.ntiles > 0 &&
+        (double)asc_n * sizeof(double) <= kAscBudgetBytes &&
+        64.0 * (double)l.ntiles * sizeof(double) < 4294967296.0) {
+      l.asc64.alloc_zero(sizeof(double) * asc_n, s);
+      launch_assemble_stencil(make_op<double>(c, li), l.tiles.as<TileRec>(), l.ntiles, l.asc64.as<double>(), s);
+      if (c->mixed) {
+        to_f32(l.asc64, l.asc32, asc_n, s);
+        GMGS_HIP(hipStreamSynchronize(s));
+        l.asc64.release();
+      }
+      l.as_built = true;
+      l.as = !l.zm || c->prefer_as;
+      if (c->prefer_as) l.zm = false;
+    }
   }
   GMGS_HIP(hipStreamSynchronize(s));
 }
@@ -496,11 +528,15 @@ std::mutex& tune_mutex() {
   return m;
 }
 
+// the tile kernel on level li, element sums (as = false) or the assembled stencil
 template <typename T, typename TR>
-float time_v1(gmgs_ctx* c, int li, const TR* in, T* out) {
+float time_v1(gmgs_ctx* c, int li, const TR* in, T* out, bool as = false) {
   Level& l = c->lev[li];
   hipStream_t s = c->stream;
+  const bool keep = l.as;
+  l.as = as;
   LevelOp<T> op = make_op<T>(c, li);
+  l.as = keep;
   hipEvent_t a, b;
   GMGS_HIP(hipEventCreate(&a));
   GMGS_HIP(hipEventCreate(&b));
@@ -561,9 +597,12 @@ void autotune_levels(gmgs_ctx* c) {
   hipStream_t s = c->stream;
   for (int li = 0; li < c->L - 1; ++li) {
     Level& l = c->lev[li];
-    if (!l.zm) continue;
+    if (!l.zm && li == 0) continue;
+    if (li > 0 && c->prefer_as && l.as_built) continue;   // the assembled stencil was asked for
     const TuneKey key{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, c->mixed ? 1 : 0};
-    std::pair<int, int> best{l.col_tx, l.col_zc};
+    // (tx, zc) of the z-marching kernel; (0, 0) the tile kernel, (0, 1) the
+    // assembled stencil
+    std::pair<int, int> best{l.zm ? l.col_tx : 0, l.zm ? l.col_zc : (l.as ? 1 : 0)};
     bool cached = false;
     {
       std::lock_guard<std::mutex> lk(tune_mutex());
@@ -572,33 +611,40 @@ void autotune_levels(gmgs_ctx* c) {
     }
     if (!cached) {
       float best_ms = 1e30f;
-      for (const auto& cd : cand) {
-        DevBuf tl;
-        const long long nt = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
-        float ms;
-        if (li == 0) {
-          if (c->mixed) ms = time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<float>());
-          else ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<double>());
-        } else {
-          if (c->mixed) ms = time_zm<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(), l.rp.as<float>());
-          else ms = time_zm<double, double>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<double>(), l.rp.as<double>());
+      if (l.zm) {
+        for (const auto& cd : cand) {
+          DevBuf tl;
+          const long long nt = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
+          float ms;
+          if (li == 0) {
+            if (c->mixed) ms = time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<float>());
+            else ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<double>());
+          } else {
+            if (c->mixed) ms = time_zm<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(), l.rp.as<float>());
+            else ms = time_zm<double, double>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<double>(), l.rp.as<double>());
+          }
+          if (ms < best_ms) { best_ms = ms; best = {cd[0], cd[1]}; }
         }
-        if (ms < best_ms) { best_ms = ms; best = {cd[0], cd[1]}; }
       }
       // coarse levels (small, many stored elements) may run faster on the tile
-      // kernel; recorded as tile (0, 0)
+      // kernel, with element sums or the assembled stencil
       if (li > 0) {
-        const float ms = c->mixed ? time_v1<float, float>(c, li, l.r.as<float>(), l.rp.as<float>())
-                                  : time_v1<double, double>(c, li, l.r.as<double>(), l.rp.as<double>());
-        if (ms < best_ms) { best_ms = ms; best = {0, 0}; }
+        for (int as = 0; as <= (l.as_built ? 1 : 0); ++as) {
+          const float ms = c->mixed ? time_v1<float, float>(c, li, l.r.as<float>(), l.rp.as<float>(), as)
+                                    : time_v1<double, double>(c, li, l.r.as<double>(), l.rp.as<double>(), as);
+          if (ms < best_ms) { best_ms = ms; best = {0, as}; }
+        }
       }
       std::lock_guard<std::mutex> lk(tune_mutex());
       tune_cache()[key] = best;
     }
     if (best.first == 0) {
       l.zm = false;
+      l.as = best.second == 1 && l.as_built;
       continue;
     }
+    if (!l.zm) continue;   // a cached z-marching shape this level cannot use
+    l.as = false;
     if (best.first != l.col_tx || best.second != l.col_zc) {
       l.col_tx = best.first;
       l.col_zc = best.second;
@@ -1282,6 +1328,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->sweeps_coarse = p->vcycle_sweeps_coarse == 0 ? c->sweeps : p->vcycle_sweeps_coarse;
     c->no_jacobi = (p->flags & GMGS_FLAG_NO_JACOBI_STEP) != 0;
     c->cheb = (p->flags & GMGS_FLAG_CHEBYSHEV) != 0;
+    c->prefer_as = (p->flags & GMGS_FLAG_ASSEMBLED_COARSE) != 0;
     if (c->cheb && !c->sym)
       throw Status(GMGS_ERR_ARGUMENT, "Chebyshev smoothing needs the symmetric cycle");
     c->nely = (int)ny;
@@ -1860,7 +1907,7 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
       case GMGS_Q_NUM_TILES: i64[0] = l->ntiles; break;
       case GMGS_Q_NUM_FREE_COARSEST: i64[0] = c->nfree; break;
       case GMGS_Q_TILE_CONFIG:
-        i64[0] = l->zm ? 1 : 0; i64[1] = l->col_tx; i64[2] = l->col_zc; break;
+        i64[0] = l->zm ? 1 : (l->as ? 2 : 0); i64[1] = l->col_tx; i64[2] = l->col_zc; break;
       case GMGS_Q_ELE_MAP_BACK: copy_dev(l->mesh.eleMapBack, 4 * need); break;
       case GMGS_Q_ELE_MAP_FORWARD: copy_dev(l->mesh.eleMapForward, 4 * need); break;
       case GMGS_Q_NOD_MAP_BACK: copy_dev(l->mesh.nodMapBack, 4 * need); break;

@@ -22,6 +22,7 @@ struct LevelOp {
   const T* emod;             // per padded cell modulus factor (level-1 SIMP) or null
   const T* invd;             // inverse diagonal, SoA 3N (zero off the solid nodes)
   const int* colz;           // per padded column (y + py x): solid-node z range lo | hi << 16, or null
+  const T* asc;              // assembled 27-point stencil, [243][tile * 64 + lane], or null
   const T* corner;           // canonical row block of Ke (72): every block is a reflection of one
   int zero_fixed;            // SMOOTH: fixed components of the output are 0 instead of the input's
 };
@@ -49,6 +50,14 @@ struct PcgScalars {
 enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2 };
 
 // ---- launches (all on stream s; 'done' may be null) ----
+// assembled stencil of every solid node of a coarse level (FP64): coefficient
+// (o, c, d) at asc[(9 o + 3 c + d) * 64 ntiles + 64 t + lane], o = neighbour
+// (dy + 1) + 3 (dx + 1) + 9 (dz + 1) of the node in lane 'lane' of tile t;
+// asc must be zeroed
+void launch_assemble_stencil(const LevelOp<double>& op, const TileRec* tiles, long long ntiles,
+                             double* asc, hipStream_t s);
+constexpr int kStencilCoefs = 243;
+
 template <typename T, typename TR>
 void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode,
                   const TR* in, T omega, T* out, TR* xacc, double* partial, const int* done,

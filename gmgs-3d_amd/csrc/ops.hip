@@ -81,7 +81,7 @@ struct InputU {
   }
 };
 
-template <typename T, typename TR, int MODE, bool HAS_EOP, bool HAS_EMOD>
+template <typename T, typename TR, int MODE, bool HAS_EOP, bool HAS_EMOD, bool AS>
 __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __restrict__ tiles,
                                                long long ntiles, const TR* __restrict__ in, T omega,
                                                T* __restrict__ out, TR* __restrict__ xacc,
@@ -97,7 +97,31 @@ __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __r
     const long long sx = op.g.py, sz = (long long)op.g.py * op.g.px;
     InputU<T, TR, MODE> U{in, op.invd, omega, N};
     T a0 = 0, a1 = 0, a2 = 0;
-    if (!HAS_EMOD && (fl & F_FAST)) {
+    if (AS) {
+      // assembled stencil: coefficient q of this lane at asc[q * 64 ntiles + slot],
+      // read from a uniform base per q at one 32-bit byte offset per lane
+      // (64 ntiles sizeof(T) < 2^32, checked at launch)
+      const long long stride = ntiles * 64;
+      const unsigned boff =
+          (unsigned)((((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63)) * sizeof(T));
+      auto coef = [&](int q) -> T {
+        return *(const T*)((const char*)(op.asc + (long long)q * stride) + boff);
+      };
+      // one z-plane of neighbours per iteration (bounds the loads in flight)
+#pragma unroll 1
+      for (int kz = 0; kz < 3; ++kz) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+          const int dy = j % 3 - 1, dx = j / 3 - 1;
+          const long long m = n + dy + dx * sx + (kz - 1) * sz;
+          const T u0 = U(m, 0), u1 = U(m, 1), u2 = U(m, 2);
+          const int q = 9 * (9 * kz + j);
+          a0 += coef(q + 0) * u0 + coef(q + 1) * u1 + coef(q + 2) * u2;
+          a1 += coef(q + 3) * u0 + coef(q + 4) * u1 + coef(q + 5) * u2;
+          a2 += coef(q + 6) * u0 + coef(q + 7) * u1 + coef(q + 8) * u2;
+        }
+      }
+    } else if (!HAS_EMOD && (fl & F_FAST)) {
       const T* __restrict__ S = op.stencil;
 #pragma unroll
       for (int o = 0; o < 27; ++o) {
@@ -167,6 +191,40 @@ __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __r
   if (partial) {
     const double b = block_sum(dsum);
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
+  }
+}
+
+// the row of a solid node in the level operator as a 27-point stencil: the
+// same element sum as k_apply (universal or stored element matrices, SIMP
+// scale), accumulated per neighbour instead of multiplied by the input
+template <bool HAS_EOP, bool HAS_EMOD>
+__global__ __launch_bounds__(256) void k_assemble_stencil(LevelOp<double> op,
+                                                          const TileRec* __restrict__ tiles,
+                                                          long long ntiles, double* __restrict__ asc) {
+  const LaneNode ln_ = lane_node(op.g, op.flags, tiles, ntiles);
+  if (!ln_.active) return;
+  const long long stride = ntiles * 64;
+  const long long slot = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63);
+  const long long n = ln_.n;
+  const long long sx = op.g.py, sz = (long long)op.g.py * op.g.px;
+  for (int p = 0; p < 8; ++p) {
+    if (!((ln_.fl >> p) & 1u)) continue;
+    const int pyb = p & 1, pxb = (p >> 1) & 1, pzb = p >> 2;
+    const long long cell = n - (1 - pyb) - (1 - pxb) * sx - (1 - pzb) * sz;
+    const double* __restrict__ K = op.Ke;
+    if (HAS_EOP) {
+      const int e = op.eop[cell];
+      if (e > 0) K = op.Ks + (long long)(e - 1) * 576;
+    }
+    const double sc = HAS_EMOD ? op.emod[cell] : 1.0;
+    const int ln = node_local_in(p);
+    for (int m = 0; m < 8; ++m) {
+      const int dy = pyb - 1 + hex_ro(m), dx = pxb - 1 + hex_xo(m), dz = pzb - 1 + hex_zo(m);
+      const int o = (dy + 1) + 3 * (dx + 1) + 9 * (dz + 1);
+      for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d)
+          asc[(long long)(9 * o + 3 * c + d) * stride + slot] += sc * K[(3 * ln + c) * 24 + 3 * m + d];
+    }
   }
 }
 
@@ -293,14 +351,19 @@ __global__ __launch_bounds__(256) void k_mask_fixed(Grid g, const uint32_t* __re
   }
 }
 
-// x = (sum_s w_s (R_s + Q_s)) / d + e on the solid nodes, R_0 = r0; zero
-// fixed components if 'fix'; partials of r0.x and r0.zprev
-template <typename T, typename TR, typename TZ>
+// x = (sum_s w_s (R_s + Q_s)) / d + e on the solid nodes, R_0 = r0, s < NU;
+// zero fixed components if 'fix'; partials of r0.x and r0.zprev.  The terms
+// are separate restrict arguments so the loads of all three components can
+// be issued ahead of the stores.
+template <typename T, typename TR, typename TZ, int NU>
 __global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restrict__ flags,
                                                  const TileRec* __restrict__ tiles, long long nt,
-                                                 const T* __restrict__ invd, CycleSum<T> cs,
-                                                 const TR* __restrict__ r0, const T* __restrict__ e,
-                                                 TZ* __restrict__ x, int fix,
+                                                 const T* __restrict__ invd, T w0, T w1, T w2, T w3,
+                                                 const TR* __restrict__ r0, const T* __restrict__ q0,
+                                                 const T* __restrict__ r1, const T* __restrict__ q1,
+                                                 const T* __restrict__ r2, const T* __restrict__ q2,
+                                                 const T* __restrict__ r3, const T* __restrict__ q3,
+                                                 const T* __restrict__ e, TZ* __restrict__ x, int fix,
                                                  const TZ* __restrict__ zprev,
                                                  double* __restrict__ partial,
                                                  const int* __restrict__ done) {
@@ -313,10 +376,10 @@ __global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restr
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long m = c * N + n;
-      T acc = cs.w[0] * ((T)r0[m] + cs.Q[0][m]);
-#pragma unroll
-      for (int k = 1; k < kMaxSweeps; ++k)
-        if (k < cs.nu) acc += cs.w[k] * (cs.R[k][m] + cs.Q[k][m]);
+      T acc = w0 * ((T)r0[m] + q0[m]);
+      if (NU > 1) acc += w1 * (r1[m] + q1[m]);
+      if (NU > 2) acc += w2 * (r2[m] + q2[m]);
+      if (NU > 3) acc += w3 * (r3[m] + q3[m]);
       T v = invd[m] * acc + e[m];
       if (fix && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) v = 0;
       x[m] = (TZ)v;
@@ -720,18 +783,33 @@ void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, 
   if (ntiles <= 0) return;
   const unsigned nb = apply_blocks(ntiles);
   const bool eop = op.eop != nullptr, emod = op.emod != nullptr;
-#define GMGS_APPLY(MODE, E, M)                                                                 \
-  k_apply<T, TR, MODE, E, M><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, xacc, partial, done)
+  if (op.asc && (unsigned long long)ntiles * 64 * sizeof(T) >= (1ull << 32))
+    throw std::invalid_argument("assembled stencil: too many tiles for 32-bit lane offsets");
+#define GMGS_APPLY(MODE, E, M, A)                                                              \
+  k_apply<T, TR, MODE, E, M, A><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, xacc, partial, done)
   if (mode == AP_AX) {
-    if (eop) GMGS_APPLY(AP_AX, true, false);
-    else if (emod) GMGS_APPLY(AP_AX, false, true);
-    else GMGS_APPLY(AP_AX, false, false);
+    if (op.asc) GMGS_APPLY(AP_AX, false, false, true);
+    else if (eop) GMGS_APPLY(AP_AX, true, false, false);
+    else if (emod) GMGS_APPLY(AP_AX, false, true, false);
+    else GMGS_APPLY(AP_AX, false, false, false);
   } else {
-    if (eop) GMGS_APPLY(AP_SMOOTH, true, false);
-    else if (emod) GMGS_APPLY(AP_SMOOTH, false, true);
-    else GMGS_APPLY(AP_SMOOTH, false, false);
+    if (op.asc) GMGS_APPLY(AP_SMOOTH, false, false, true);
+    else if (eop) GMGS_APPLY(AP_SMOOTH, true, false, false);
+    else if (emod) GMGS_APPLY(AP_SMOOTH, false, true, false);
+    else GMGS_APPLY(AP_SMOOTH, false, false, false);
   }
 #undef GMGS_APPLY
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_assemble_stencil(const LevelOp<double>& op, const TileRec* tiles, long long ntiles,
+                             double* asc, hipStream_t s) {
+  if (ntiles <= 0) return;
+  const unsigned nb = apply_blocks(ntiles);
+  if (op.eop && op.emod) throw std::invalid_argument("assemble_stencil: eop and emod together");
+  if (op.eop) k_assemble_stencil<true, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, asc);
+  else if (op.emod) k_assemble_stencil<false, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, asc);
+  else k_assemble_stencil<false, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, asc);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -796,9 +874,20 @@ void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, 
                     const T* invd, const CycleSum<T>& cs, const TR* r0, const T* e, TZ* x, bool fix,
                     const TZ* zprev, double* partial, const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  if (cs.nu < 1 || cs.nu > kMaxSweeps) throw std::invalid_argument("combine: sweeps out of range");
-  k_combine<T, TR, TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, cs, r0, e, x,
-                                                             fix ? 1 : 0, zprev, partial, done);
+  const unsigned nb = pointwise_blocks(nt);
+  const int f = fix ? 1 : 0;
+#define GMGS_COMBINE(NU)                                                                            \
+  k_combine<T, TR, TZ, NU><<<nb, 256, 0, s>>>(g, flags, tiles, nt, invd, cs.w[0], cs.w[1], cs.w[2],  \
+                                              cs.w[3], r0, cs.Q[0], cs.R[1], cs.Q[1], cs.R[2],       \
+                                              cs.Q[2], cs.R[3], cs.Q[3], e, x, f, zprev, partial, done)
+  switch (cs.nu) {
+    case 1: GMGS_COMBINE(1); break;
+    case 2: GMGS_COMBINE(2); break;
+    case 3: GMGS_COMBINE(3); break;
+    case 4: GMGS_COMBINE(4); break;
+    default: throw std::invalid_argument("combine: sweeps out of range");
+  }
+#undef GMGS_COMBINE
   GMGS_LAUNCH_CHECK();
 }
 #define GMGS_INST_COMBINE(T, TR, TZ)                                                                \

@@ -68,11 +68,14 @@ enum {
                                      counterpart; kernel/Vcycle.m:27-33 reuses the stale one) */
   GMGS_FLAG_NO_JACOBI_STEP = 256, /* omit the Jacobi step opening every PCG iteration
                                      (CG_solver.m:18-19) */
-  GMGS_FLAG_CHEBYSHEV = 512       /* with GMGS_FLAG_SYM_VCYCLE: the nu sweeps of a level use
+  GMGS_FLAG_CHEBYSHEV = 512,      /* with GMGS_FLAG_SYM_VCYCLE: the nu sweeps of a level use
                                      the reciprocal roots of the degree-nu Chebyshev polynomial
                                      on [0.22, 1.1] x lambda_max(D^-1 A), estimated per level by
                                      20 Jacobi-CG (Lanczos) steps when the BCs are set (no
                                      reference counterpart; JacobiSmoother.m uses one weight) */
+  GMGS_FLAG_ASSEMBLED_COARSE = 1024 /* coarse smoothed levels apply an assembled 27-point
+                                     stencil (243 coefficients per node, set-up time) wherever it
+                                     fits, instead of the z-marching or element-sum kernels */
 };
 
 typedef struct gmgs_params {
@@ -130,7 +133,9 @@ enum {
   GMGS_Q_F = 25,                    /* double[numDOFs] load vector F_ */
   GMGS_Q_NUM_TILES = 26,            /* int64 active 16x4x1 tiles of the level */
   GMGS_Q_NUM_FREE_COARSEST = 27,    /* int64 */
-  GMGS_Q_TILE_CONFIG = 28,          /* int64[3]: z-marching on?, tile TX, z-chunk */
+  GMGS_Q_TILE_CONFIG = 28,          /* int64[3]: level kernel (0 tile kernel with element sums,
+                                       1 z-marching, 2 tile kernel with the assembled stencil),
+                                       z-marching tile TX, z-chunk */
   GMGS_Q_SMOOTHER = 29              /* double[5] (levels < numLevels, after gmgs_set_bc):
                                        lambda_max(D^-1 A) estimate (0 = none), the weights of the
                                        level's nu sweeps, 0 beyond nu */
