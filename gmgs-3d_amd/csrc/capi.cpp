@@ -54,8 +54,10 @@ struct Level {
   DevBuf coltiles;
   DevBuf colz;                   // per padded column: solid-node z range
   long long ncol = 0;
-  // assembled 27-point stencil (coarse levels small enough): built, and in use
-  DevBuf asc64, asc32;
+  // assembled 27-point stencil rows (coarse levels): one per solid node that
+  // is not F_FAST plus the shared universal row; built, and in use
+  DevBuf asc64, asc32, arow;
+  long long as_stride = 0;
   bool as_built = false, as = false;
   int col_tx = 8, col_zc = 16;
   // symmetric cycle: nu sweeps with weights om, and the estimate of
@@ -182,6 +184,8 @@ LevelOp<T> make_op(gmgs_ctx* c, int li, bool cyc = false) {
   LevelOp<T> op;
   op.zero_fixed = 0;
   op.asc = nullptr;
+  op.arow = l.arow.empty() ? nullptr : l.arow.as<int>();
+  op.as_stride = l.as_stride;
   op.g = l.mesh.g;
   op.flags = l.mesh.flags.as<uint32_t>();
   op.colz = l.colz.empty() ? nullptr : l.colz.as<int>();
@@ -485,22 +489,49 @@ void build_operators(gmgs_ctx* c) {
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
       build_colrange(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.colz, s);
     }
-    // assembled stencil on the smoothed coarse levels that are small enough
-    // (assembled in FP64); used where the z-marching kernel is not
+    // assembled stencil rows on the smoothed coarse levels (assembled in
+    // FP64): one per solid node that is not F_FAST, in tile-slot order, and
+    // the universal row last; used where the z-marching kernel is not
     l.as = l.as_built = false;
     l.asc64.release();
     l.asc32.release();
-    const long long asc_n = (long long)kStencilCoefs * 64 * l.ntiles;
-    if (li >= 1 && li < L - 1 && !c->simple_kernels && l.ntiles > 0 &&
-        (double)asc_n * sizeof(double) <= kAscBudgetBytes &&
-        64.0 * (double)l.ntiles * sizeof(double) < 4294967296.0) {
-      l.asc64.alloc_zero(sizeof(double) * asc_n, s);
-      launch_assemble_stencil(make_op<double>(c, li), l.tiles.as<TileRec>(), l.ntiles, l.asc64.as<double>(), s);
+    l.arow.release();
+    l.as_stride = 0;
+    long long rows = 0;
+    std::vector<int> ar;
+    if (li >= 1 && li < L - 1 && !c->simple_kernels && l.ntiles > 0) {
+      const Grid& g = l.mesh.g;
+      const std::vector<TileRec> th = download<TileRec>(l.tiles, (size_t)l.ntiles, s);
+      const std::vector<uint32_t> fh = download<uint32_t>(l.mesh.flags, (size_t)g.N, s);
+      ar.assign((size_t)l.ntiles * 64, -1);
+      for (long long t = 0; t < l.ntiles; ++t)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int yp = th[t].y0 + (lane & 15), xp = th[t].x0 + (lane >> 4);
+          if (xp >= g.px) continue;
+          const uint32_t f = fh[yp + (long long)g.py * (xp + (long long)g.px * th[t].z)];
+          if ((f & F_ELEM_MASK) && !(f & F_FAST)) ar[t * 64 + lane] = (int)rows++;
+        }
+    }
+    const long long stride = rows + 1;
+    const long long asc_n = (long long)kStencilCoefs * stride;
+    if (!ar.empty() && (double)asc_n * sizeof(double) <= kAscBudgetBytes &&
+        16.0 * (double)stride < 4294967296.0) {
+      for (int& v : ar)
+        if (v < 0) v = (int)(stride - 1);
+      upload(l.arow, ar.data(), ar.size(), s);
+      l.as_stride = stride;
+      DevBuf canon;
+      canon.alloc_zero(sizeof(double) * asc_n, s);
+      launch_assemble_stencil(make_op<double>(c, li), l.tiles.as<TileRec>(), l.ntiles, l.arow.as<int>(), stride,
+                              canon.as<double>(), s);
       if (c->mixed) {
-        to_f32(l.asc64, l.asc32, asc_n, s);
-        GMGS_HIP(hipStreamSynchronize(s));
-        l.asc64.release();
+        l.asc32.alloc_zero(sizeof(float) * stencil_group_elems(stride, sizeof(float)), s);
+        launch_relayout_stencil<float>(canon.as<double>(), stride, l.asc32.as<float>(), s);
+      } else {
+        l.asc64.alloc_zero(sizeof(double) * stencil_group_elems(stride, sizeof(double)), s);
+        launch_relayout_stencil<double>(canon.as<double>(), stride, l.asc64.as<double>(), s);
       }
+      GMGS_HIP(hipStreamSynchronize(s));
       l.as_built = true;
       l.as = !l.zm || c->prefer_as;
       if (c->prefer_as) l.zm = false;

@@ -22,7 +22,9 @@ struct LevelOp {
   const T* emod;             // per padded cell modulus factor (level-1 SIMP) or null
   const T* invd;             // inverse diagonal, SoA 3N (zero off the solid nodes)
   const int* colz;           // per padded column (y + py x): solid-node z range lo | hi << 16, or null
-  const T* asc;              // assembled 27-point stencil, [243][tile * 64 + lane], or null
+  const T* asc;              // assembled stencil rows in 16-byte groups [q / G][row][G], or null
+  const int* arow;           // per tile slot (64 tile + lane): the node's row of asc
+  long long as_stride;       // rows of asc (the last one: the universal stencil)
   const T* corner;           // canonical row block of Ke (72): every block is a reflection of one
   int zero_fixed;            // SMOOTH: fixed components of the output are 0 instead of the input's
 };
@@ -50,13 +52,26 @@ struct PcgScalars {
 enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2 };
 
 // ---- launches (all on stream s; 'done' may be null) ----
-// assembled stencil of every solid node of a coarse level (FP64): coefficient
-// (o, c, d) at asc[(9 o + 3 c + d) * 64 ntiles + 64 t + lane], o = neighbour
-// (dy + 1) + 3 (dx + 1) + 9 (dz + 1) of the node in lane 'lane' of tile t;
-// asc must be zeroed
+// assembled stencil rows of a coarse level (FP64): coefficient (o, c, d) of
+// row k at asc[(9 o + 3 c + d) * stride + k], o = neighbour (dy + 1) +
+// 3 (dx + 1) + 9 (dz + 1).  Row arow[64 t + lane] of every solid node that is
+// not F_FAST gets that node's element sum; row stride - 1 is the universal
+// stencil (op.stencil), shared by the F_FAST nodes.  asc must be zeroed.
 void launch_assemble_stencil(const LevelOp<double>& op, const TileRec* tiles, long long ntiles,
-                             double* asc, hipStream_t s);
+                             const int* arow, long long stride, double* asc, hipStream_t s);
 constexpr int kStencilCoefs = 243;
+// 16-byte groups of one z-plane's 81 coefficients of an assembled row
+__host__ __device__ constexpr int stencil_plane_groups(int elem_bytes) {
+  return (81 + 16 / elem_bytes - 1) / (16 / elem_bytes);
+}
+// the grouped layout [q' / G][row][G] (G = 16 / sizeof(T), z-planes padded to
+// whole groups) of the level operator's assembled rows, from the FP64 rows
+// [q][row]; dst zeroed, stencil_group_elems(rows, sizeof(T)) elements
+template <typename T>
+void launch_relayout_stencil(const double* src, long long rows, T* dst, hipStream_t s);
+inline long long stencil_group_elems(long long rows, int elem_bytes) {
+  return 3LL * stencil_plane_groups(elem_bytes) * (16 / elem_bytes) * rows;
+}
 
 template <typename T, typename TR>
 void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode,

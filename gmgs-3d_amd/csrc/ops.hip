@@ -81,7 +81,7 @@ struct InputU {
   }
 };
 
-template <typename T, typename TR, int MODE, bool HAS_EOP, bool HAS_EMOD, bool AS>
+template <typename T, typename TR, int MODE, bool HAS_EOP, bool HAS_EMOD>
 __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __restrict__ tiles,
                                                long long ntiles, const TR* __restrict__ in, T omega,
                                                T* __restrict__ out, TR* __restrict__ xacc,
@@ -97,31 +97,7 @@ __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __r
     const long long sx = op.g.py, sz = (long long)op.g.py * op.g.px;
     InputU<T, TR, MODE> U{in, op.invd, omega, N};
     T a0 = 0, a1 = 0, a2 = 0;
-    if (AS) {
-      // assembled stencil: coefficient q of this lane at asc[q * 64 ntiles + slot],
-      // read from a uniform base per q at one 32-bit byte offset per lane
-      // (64 ntiles sizeof(T) < 2^32, checked at launch)
-      const long long stride = ntiles * 64;
-      const unsigned boff =
-          (unsigned)((((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63)) * sizeof(T));
-      auto coef = [&](int q) -> T {
-        return *(const T*)((const char*)(op.asc + (long long)q * stride) + boff);
-      };
-      // one z-plane of neighbours per iteration (bounds the loads in flight)
-#pragma unroll 1
-      for (int kz = 0; kz < 3; ++kz) {
-#pragma unroll
-        for (int j = 0; j < 9; ++j) {
-          const int dy = j % 3 - 1, dx = j / 3 - 1;
-          const long long m = n + dy + dx * sx + (kz - 1) * sz;
-          const T u0 = U(m, 0), u1 = U(m, 1), u2 = U(m, 2);
-          const int q = 9 * (9 * kz + j);
-          a0 += coef(q + 0) * u0 + coef(q + 1) * u1 + coef(q + 2) * u2;
-          a1 += coef(q + 3) * u0 + coef(q + 4) * u1 + coef(q + 5) * u2;
-          a2 += coef(q + 6) * u0 + coef(q + 7) * u1 + coef(q + 8) * u2;
-        }
-      }
-    } else if (!HAS_EMOD && (fl & F_FAST)) {
+    if (!HAS_EMOD && (fl & F_FAST)) {
       const T* __restrict__ S = op.stencil;
 #pragma unroll
       for (int o = 0; o < 27; ++o) {
@@ -194,17 +170,127 @@ __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __r
   }
 }
 
+// Level operator from assembled stencil rows (coarse levels).  One wave per
+// 16 x 4 x 1 node tile, four tiles per block.  The tile's 18 x 6 x 3
+// neighbourhood of the input (u = omega r / d for the smoother) is staged in
+// LDS by the wave; each lane then multiplies its 27 neighbours by its row of
+// the operator, whose 243 coefficients are stored in 16-byte groups
+// ([q' / G][row][G], G = 16 / sizeof(T); each z-plane's 81 coefficients
+// padded to whole groups, q' = (q / 81) QP + q % 81) so a lane reads its row
+// with coalesced 16-byte loads, one z-plane at a time.  F_FAST lanes share the
+// universal row.
+template <typename T>
+struct alignas(16) CoefGroup {
+  T v[16 / sizeof(T)];
+};
+
+template <typename T, typename TR, int MODE>
+__global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* __restrict__ tiles,
+                                                  long long ntiles, const TR* __restrict__ in, T omega,
+                                                  T* __restrict__ out, double* __restrict__ partial,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  constexpr int SY = 18, SX = 6, NS = SY * SX * 3;   // staged nodes per tile
+  constexpr int G = 16 / (int)sizeof(T), NGP = stencil_plane_groups((int)sizeof(T));
+  __shared__ T sU[4][3][NS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long t = (long long)blockIdx.x * 4 + wave;
+  const Grid g = op.g;
+  const long long N = g.N, sx = g.py, sz = (long long)g.py * g.px;
+  double dsum = 0.0;
+  TileRec tr{0, 0, 0, 0};
+  if (t < ntiles) {
+    tr = tiles[t];
+    for (int e = lane; e < 3 * NS; e += 64) {
+      const int c = e / NS, r = e % NS, zz = r / (SY * SX), xx = (r / SY) % SX, yy = r % SY;
+      const int gy = tr.y0 - 1 + yy, gx = tr.x0 - 1 + xx, gz = tr.z - 1 + zz;
+      T v = 0;
+      if (gy >= 0 && gy < g.py && gx >= 0 && gx < g.px && gz >= 0 && gz < g.pz) {
+        const long long m = c * N + gy + sx * gx + sz * gz;
+        v = (T)in[m];
+        if (MODE == AP_SMOOTH) v = omega * op.invd[m] * v;
+      }
+      sU[wave][c][r] = v;
+    }
+  }
+  __syncthreads();
+  const int ly = lane & 15, lx = lane >> 4;
+  const int yp = tr.y0 + ly, xp = tr.x0 + lx;
+  uint32_t fl = 0;
+  long long n = 0;
+  if (t < ntiles && xp < g.px) {
+    n = yp + sx * xp + sz * tr.z;
+    fl = op.flags[n];
+  }
+  if (fl & F_ELEM_MASK) {
+    // group q0 of this lane's row at a uniform base (q0 rows groups) plus one
+    // 32-bit byte offset (16 rows < 2^32, checked at set-up)
+    const unsigned boff = (unsigned)op.arow[t * 64 + lane] * 16u;
+    const char* __restrict__ base = reinterpret_cast<const char*>(op.asc);
+    const long long gbytes = op.as_stride * 16;
+    T a[3] = {0, 0, 0};
+#pragma unroll 1
+    for (int kz = 0; kz < 3; ++kz) {
+      const T* __restrict__ sp = &sU[wave][0][SY * SX * kz + (ly + 1) + SY * (lx + 1)];
+#pragma unroll
+      for (int g0 = 0; g0 < NGP; ++g0) {
+        const CoefGroup<T> cg =
+            *reinterpret_cast<const CoefGroup<T>*>(base + (long long)(kz * NGP + g0) * gbytes + boff);
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+          const int r = g0 * G + k;                   // coefficient within the z-plane
+          if (r < 81) {
+            const int j = r / 9, cc = (r / 3) % 3, d = r % 3;
+            const int dy = j % 3 - 1, dx = j / 3 - 1;
+            a[cc] += cg.v[k] * sp[d * NS + dy + SY * dx];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const bool fx = (fl & (1u << (F_FIX_SHIFT + c))) != 0;
+      const T ac = fx ? (T)0 : a[c];          // AtX output zeroed on fixed DOFs (kernel/AtX.m:57)
+      if (MODE == AP_AX) {
+        out[c * N + n] = ac;
+        if (partial) dsum += (double)in[c * N + n] * (double)ac;
+      } else {
+        out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : (T)in[c * N + n] - ac;
+      }
+    }
+  }
+  if (partial) {
+    const double b = block_sum(dsum);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+  }
+}
+
+// canonical FP64 rows [q][row] -> the grouped layout [q / G][row][G] of
+// k_apply_as in T (coefficients 243.. of the last group zero)
+template <typename T>
+__global__ void k_relayout_stencil(const double* __restrict__ src, long long rows, T* __restrict__ dst) {
+  constexpr int G = 16 / (int)sizeof(T), QP = G * stencil_plane_groups((int)sizeof(T));
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)kStencilCoefs * rows) return;
+  const int q = (int)(i / rows);
+  const long long row = i % rows;
+  const int qp = (q / 81) * QP + q % 81;
+  dst[((long long)(qp / G) * rows + row) * G + qp % G] = (T)src[i];
+}
+
 // the row of a solid node in the level operator as a 27-point stencil: the
 // same element sum as k_apply (universal or stored element matrices, SIMP
 // scale), accumulated per neighbour instead of multiplied by the input
 template <bool HAS_EOP, bool HAS_EMOD>
 __global__ __launch_bounds__(256) void k_assemble_stencil(LevelOp<double> op,
                                                           const TileRec* __restrict__ tiles,
-                                                          long long ntiles, double* __restrict__ asc) {
+                                                          long long ntiles, const int* __restrict__ arow,
+                                                          long long stride, double* __restrict__ asc) {
+  if (blockIdx.x == 0 && threadIdx.x < kStencilCoefs)
+    asc[(long long)threadIdx.x * stride + stride - 1] = op.stencil[threadIdx.x];
   const LaneNode ln_ = lane_node(op.g, op.flags, tiles, ntiles);
-  if (!ln_.active) return;
-  const long long stride = ntiles * 64;
-  const long long slot = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63);
+  if (!ln_.active || (ln_.fl & F_FAST)) return;
+  const long long slot = arow[((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63)];
   const long long n = ln_.n;
   const long long sx = op.g.py, sz = (long long)op.g.py * op.g.px;
   for (int p = 0; p < 8; ++p) {
@@ -783,33 +869,47 @@ void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, 
   if (ntiles <= 0) return;
   const unsigned nb = apply_blocks(ntiles);
   const bool eop = op.eop != nullptr, emod = op.emod != nullptr;
-  if (op.asc && (unsigned long long)ntiles * 64 * sizeof(T) >= (1ull << 32))
-    throw std::invalid_argument("assembled stencil: too many tiles for 32-bit lane offsets");
-#define GMGS_APPLY(MODE, E, M, A)                                                              \
-  k_apply<T, TR, MODE, E, M, A><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, xacc, partial, done)
+  if (op.asc && !xacc && mode != AP_RESID) {
+    if (mode == AP_AX)
+      k_apply_as<T, TR, AP_AX><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, partial, done);
+    else
+      k_apply_as<T, TR, AP_SMOOTH><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, partial, done);
+    GMGS_LAUNCH_CHECK();
+    return;
+  }
+#define GMGS_APPLY(MODE, E, M)                                                                 \
+  k_apply<T, TR, MODE, E, M><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, xacc, partial, done)
   if (mode == AP_AX) {
-    if (op.asc) GMGS_APPLY(AP_AX, false, false, true);
-    else if (eop) GMGS_APPLY(AP_AX, true, false, false);
-    else if (emod) GMGS_APPLY(AP_AX, false, true, false);
-    else GMGS_APPLY(AP_AX, false, false, false);
+    if (eop) GMGS_APPLY(AP_AX, true, false);
+    else if (emod) GMGS_APPLY(AP_AX, false, true);
+    else GMGS_APPLY(AP_AX, false, false);
   } else {
-    if (op.asc) GMGS_APPLY(AP_SMOOTH, false, false, true);
-    else if (eop) GMGS_APPLY(AP_SMOOTH, true, false, false);
-    else if (emod) GMGS_APPLY(AP_SMOOTH, false, true, false);
-    else GMGS_APPLY(AP_SMOOTH, false, false, false);
+    if (eop) GMGS_APPLY(AP_SMOOTH, true, false);
+    else if (emod) GMGS_APPLY(AP_SMOOTH, false, true);
+    else GMGS_APPLY(AP_SMOOTH, false, false);
   }
 #undef GMGS_APPLY
   GMGS_LAUNCH_CHECK();
 }
 
+template <typename T>
+void launch_relayout_stencil(const double* src, long long rows, T* dst, hipStream_t s) {
+  const long long n = (long long)kStencilCoefs * rows;
+  if (n <= 0) return;
+  k_relayout_stencil<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, rows, dst);
+  GMGS_LAUNCH_CHECK();
+}
+template void launch_relayout_stencil<double>(const double*, long long, double*, hipStream_t);
+template void launch_relayout_stencil<float>(const double*, long long, float*, hipStream_t);
+
 void launch_assemble_stencil(const LevelOp<double>& op, const TileRec* tiles, long long ntiles,
-                             double* asc, hipStream_t s) {
+                             const int* arow, long long stride, double* asc, hipStream_t s) {
   if (ntiles <= 0) return;
   const unsigned nb = apply_blocks(ntiles);
   if (op.eop && op.emod) throw std::invalid_argument("assemble_stencil: eop and emod together");
-  if (op.eop) k_assemble_stencil<true, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, asc);
-  else if (op.emod) k_assemble_stencil<false, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, asc);
-  else k_assemble_stencil<false, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, asc);
+  if (op.eop) k_assemble_stencil<true, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, arow, stride, asc);
+  else if (op.emod) k_assemble_stencil<false, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, arow, stride, asc);
+  else k_assemble_stencil<false, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, arow, stride, asc);
   GMGS_LAUNCH_CHECK();
 }
 
