@@ -732,9 +732,10 @@ void alloc_vectors(gmgs_ctx* c) {
 template <typename T>
 T* vec(DevBuf& b) { return b.as<T>(); }
 
-// z = V(rin) (kernel/Vcycle.m:1-36); rin is the level-1 FP64 residual
+// z = V(rin) (kernel/Vcycle.m:1-36); rin is the level-1 FP64 residual;
+// returns the number n of r'z partials (r'zprev at [n, 2 n))
 template <typename T>
-void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
+int vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
   hipStream_t s = c->stream;
   const int L = c->L;
   const T omega = (T)c->prm.jacobi_weight;
@@ -783,6 +784,7 @@ void vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partia
                                         l.ntiles, cr.mesh.g, vec<T>(cr.x), invd, rin, vec<T>(l.rp),
                                         omega, z, true, zprev, partial, done, s);
   }
+  return (int)pointwise_blocks(L1(c).ntiles);
 }
 
 // buffers of the symmetric cycle on a level with nu sweeps: R_1 in rp, R_s
@@ -808,9 +810,10 @@ T* Qbuf(Level& l, int s) {
 // weights are the Jacobi weight, or the reciprocal roots of the degree-nu
 // Chebyshev polynomial on the level's eigenvalue bounds.
 template <typename T>
-void vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
+int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
   hipStream_t s = c->stream;
   const int L = c->L;
+  int nz = (int)pointwise_blocks(L1(c).ntiles);
   for (int li = 0; li < L - 1; ++li) {
     Level& l = c->lev[li];
     const int nu = l.nu;
@@ -847,6 +850,14 @@ void vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* pa
     launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
     apply_level<T, T>(c, li, AP_RESID, vec<T>(l.e), (T)0, Qbuf<T>(l, 0), nullptr, nullptr, done, Rbuf<T>(l, nu),
                       true);
+    if (li == 0 && nu == 2 && l.zm) {
+      // the last sweep writes the cycle's sum itself (no Q_1, no separate sum)
+      const CycleOut<T> co{rin, Rbuf<T>(l, 1), vec<T>(l.e), zprev, z, (T)l.om[1]};
+      nz = launch_apply_zm<T, T>(make_op<T>(c, 0, true), l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc,
+                                 AP_SMOOTH_SUM, Qbuf<T>(l, 0), (T)l.om[0], nullptr, nullptr, partial, done, s,
+                                 &co);
+      continue;
+    }
     for (int k = 1; k < nu; ++k)
       apply_level<T, T>(c, li, AP_SMOOTH, Qbuf<T>(l, k - 1), (T)l.om[k - 1], Qbuf<T>(l, k), nullptr, nullptr,
                         done, nullptr, true);
@@ -864,12 +875,14 @@ void vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* pa
       launch_combine<T, T, T>(g, fl, tl, l.ntiles, invd, cs, vec<T>(l.r), vec<T>(l.e), vec<T>(l.x), false,
                               (const T*)nullptr, nullptr, done, s);
   }
+  return nz;
 }
 
+// z = M^-1 rin; returns the number n of r'z partials (r'zprev at [n, 2 n))
 template <typename T>
-void run_vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
-  if (c->sym) vcycle_sym<T>(c, rin, z, zprev, partial, done);
-  else vcycle<T>(c, rin, z, zprev, partial, done);
+int run_vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
+  if (c->sym) return vcycle_sym<T>(c, rin, z, zprev, partial, done);
+  return vcycle<T>(c, rin, z, zprev, partial, done);
 }
 
 // ------------------------------------------------------------------ PCG
@@ -898,8 +911,8 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   // z = V(r) with r'z and r'z_prev partials (CG_solver.m:21)
   T* z = c->z[k & 1].as<T>();
   const T* zprev = c->flexible ? c->z[(k + 1) & 1].as<T>() : nullptr;
-  run_vcycle<T>(c, ra, z, zprev, part, done);
-  launch_finalize(1, part, np, sc, nullptr, nullptr, done, s);
+  const int nz = run_vcycle<T>(c, ra, z, zprev, part, done);
+  launch_finalize(1, part, nz, sc, nullptr, nullptr, done, s);
   // p = z + beta p (CG_solver.m:23-28)
   launch_update_p<T>(g, fl, tl, l.ntiles, z, c->p.as<double>(), sc, done, s);
   // q = A p, p'q (CG_solver.m:29-31)

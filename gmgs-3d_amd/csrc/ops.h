@@ -48,8 +48,23 @@ struct PcgScalars {
   unsigned long long t0;
 };
 
-// AX: out = A in; SMOOTH: out = in - A (omega in / d); RESID: out = in2 - A in
-enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2 };
+// AX: out = A in; SMOOTH: out = in - A (omega in / d); RESID: out = in2 - A in;
+// SMOOTH_SUM: the smoothing sweep Q_1 = Q_0 - A (omega Q_0 / d) from in = Q_0
+// that ends the symmetric V(2, 2) cycle on level 1, writing its sum instead
+// of Q_1 (CycleOut)
+enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3 };
+
+// outputs of AP_SMOOTH_SUM: z = (omega (r0 + Q_0) + w1 (R1 + Q_1)) / d + e,
+// fixed components 0, and partials of r0.z and r0.zprev (zprev may be null)
+template <typename T>
+struct CycleOut {
+  const double* r0;
+  const T* R1;
+  const T* e;
+  const T* zprev;
+  T* z;
+  T w1;
+};
 
 // ---- launches (all on stream s; 'done' may be null) ----
 // assembled stencil rows of a coarse level (FP64): coefficient (o, c, d) of

@@ -18,11 +18,12 @@ long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, D
 void build_colrange(const Grid& g, const uint32_t* flags, DevBuf& colz, hipStream_t s);
 
 // same contract as launch_apply without the x += u accumulation, plus the
-// RESID mode (out = in2 - A in); returns the number of dot partials written
-// (one per tile)
+// RESID mode (out = in2 - A in) and, on level 1 (no stored elements), the
+// SMOOTH_SUM mode (co: its outputs; partials r0.z at [0, n) and r0.zprev at
+// [n, 2 n)); returns the number n of dot partials written (one per tile)
 template <typename T, typename TR>
 int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
                     int mode, const TR* in, T omega, const T* in2, T* out, double* partial,
-                    const int* done, hipStream_t s);
+                    const int* done, hipStream_t s, const CycleOut<T>* co = nullptr);
 
 }  // namespace gmgs

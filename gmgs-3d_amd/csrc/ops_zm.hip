@@ -47,11 +47,12 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
                                                       int zc, const TR* __restrict__ in, T omega,
                                                       const T* __restrict__ in2, T* __restrict__ out,
                                                       double* __restrict__ partial,
-                                                      const int* __restrict__ done) {
+                                                      const int* __restrict__ done, CycleOut<T> co) {
   constexpr int TY = 16, NT = TY * TX, SY = TY + 2, SX = TX + 2;
   constexpr int NSTAGE = 3 * SX * SY;
   constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
-  constexpr bool SM = MODE == AP_SMOOTH;
+  constexpr bool SUM = MODE == AP_SMOOTH_SUM;
+  constexpr bool SM = MODE == AP_SMOOTH || SUM;
   if (done && *done) return;
   // four plane buffers: plane k (counted from the tile's first) is staged in
   // buffer k % 4 and read while computing planes k, k + 1 and k + 2 (stencil,
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
 
   T aM[3] = {0, 0, 0}, a0[3] = {0, 0, 0}, aP[3] = {0, 0, 0};   // outputs zz-1, zz, zz+1
   uint32_t fM = 0, f0 = 0, fP = 0;
-  double dsum = 0.0;
+  double dsum = 0.0, dsum2 = 0.0;
   int kb = 0;                                       // buffer of the current plane
   // two planes of loads in flight: plane zz is staged from one register set
   // while the other holds plane zz + 1; z0 < z1 for every tile
@@ -325,6 +326,21 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
           if (partial) dsum += (double)sU[b1][c][lx + 1][ly + 1] * (double)a;
         } else if (MODE == AP_RESID) {
           out[c * N + n] = cen[c] - a;
+        } else if (SUM) {
+          // the other terms of the cycle's sum, loaded here to keep them out
+          // of the stencil's registers
+          const long long m = c * N + n;
+          const double r0 = co.r0[m];
+          const bool fx = (fM & (1u << (F_FIX_SHIFT + c))) != 0;
+          const T q0 = sR[zo_ & 1][c][lx][ly];
+          const T q1 = q0 - a;
+          T v = op.invd[m] * (omega * ((T)r0 + q0) + co.w1 * (co.R1[m] + q1)) + co.e[m];
+          if (fx) v = 0;
+          co.z[m] = v;
+          if (partial) {
+            dsum += r0 * (double)v;
+            if (co.zprev) dsum2 += r0 * (double)co.zprev[m];
+          }
         } else {
           const bool fx = (fM & (1u << (F_FIX_SHIFT + c))) != 0;
           out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : sR[zo_ & 1][c][lx][ly] - a;
@@ -348,16 +364,26 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   }
   if (partial) {
     // block reduction (fixed order)
-    double v = dsum;
+    double v = dsum, v2 = dsum2;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    __shared__ double sh[NT / 64];
-    if ((tid & 63) == 0) sh[tid >> 6] = v;
+    for (int off = 32; off > 0; off >>= 1) {
+      v += __shfl_xor(v, off);
+      if (SUM) v2 += __shfl_xor(v2, off);
+    }
+    __shared__ double sh[SUM ? 2 : 1][NT / 64];
+    if ((tid & 63) == 0) {
+      sh[0][tid >> 6] = v;
+      if (SUM) sh[SUM ? 1 : 0][tid >> 6] = v2;
+    }
     __syncthreads();
     if (tid == 0) {
-      double r = 0.0;
-      for (int w = 0; w < NT / 64; ++w) r += sh[w];
+      double r = 0.0, r2 = 0.0;
+      for (int w = 0; w < NT / 64; ++w) {
+        r += sh[0][w];
+        if (SUM) r2 += sh[SUM ? 1 : 0][w];
+      }
       partial[blockIdx.x] = r;
+      if (SUM) partial[gridDim.x + blockIdx.x] = r2;
     }
   }
 }
@@ -433,13 +459,16 @@ long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, D
 template <typename T, typename TR>
 int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
                     int mode, const TR* in, T omega, const T* in2, T* out, double* partial,
-                    const int* done, hipStream_t s) {
+                    const int* done, hipStream_t s, const CycleOut<T>* co) {
   if (ntiles <= 0) return 0;
   if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("grid too large for the z-marching operator");
   const bool eop = op.eop != nullptr;
+  const CycleOut<T> cov = co ? *co : CycleOut<T>{nullptr, nullptr, nullptr, nullptr, nullptr, (T)0};
+  if (mode == AP_SMOOTH_SUM && (!co || eop || !op.invd))
+    throw std::invalid_argument("SMOOTH_SUM: level 1 only, with its outputs");
 #define GMGS_ZM(MODE, E, TXV)                                                                        \
   k_apply_zm<T, TR, MODE, E, TXV><<<(unsigned)ntiles, 16 * TXV, 0, s>>>(op, tiles, zc, in, omega, in2, \
-                                                                        out, partial, done)
+                                                                        out, partial, done, cov)
 #define GMGS_ZM_TX(MODE, E)               \
   do {                                    \
     if (tx == 4) GMGS_ZM(MODE, E, 4);     \
@@ -452,6 +481,9 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   } else if (mode == AP_RESID) {
     if (eop) GMGS_ZM_TX(AP_RESID, true);
     else GMGS_ZM_TX(AP_RESID, false);
+  } else if (mode == AP_SMOOTH_SUM) {
+    if constexpr (std::is_same<T, TR>::value) GMGS_ZM_TX(AP_SMOOTH_SUM, false);
+    else throw std::invalid_argument("SMOOTH_SUM: input in the cycle precision");
   } else {
     if (eop) GMGS_ZM_TX(AP_SMOOTH, true);
     else GMGS_ZM_TX(AP_SMOOTH, false);
@@ -464,12 +496,12 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
 
 template int launch_apply_zm<double, double>(const LevelOp<double>&, const ColTile*, long long, int, int,
                                              int, const double*, double, const double*, double*, double*,
-                                             const int*, hipStream_t);
+                                             const int*, hipStream_t, const CycleOut<double>*);
 template int launch_apply_zm<float, float>(const LevelOp<float>&, const ColTile*, long long, int, int,
                                            int, const float*, float, const float*, float*, double*,
-                                           const int*, hipStream_t);
+                                           const int*, hipStream_t, const CycleOut<float>*);
 template int launch_apply_zm<float, double>(const LevelOp<float>&, const ColTile*, long long, int, int,
                                             int, const double*, float, const float*, float*, double*,
-                                            const int*, hipStream_t);
+                                            const int*, hipStream_t, const CycleOut<float>*);
 
 }  // namespace gmgs
