@@ -167,6 +167,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-it", type=int, default=3000)
     ap.add_argument("--matvec-reps", type=int, default=20)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the comparison solvers (counter-collection runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -221,18 +223,31 @@ def main():
         o.close()
         return Uo, ms, inf
 
-    # the reference algorithm in FP64 (kernel/Vcycle.m cycle, Jacobi step,
-    # Fletcher-Reeves): the baseline of the speed-up
-    U64, ms64, info64 = other("fp64", pkg.GMGS_PREC_FP64, base_flags)
-    err = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
-    # the same symmetric cycle in FP64 (precision-only comparison) and the
-    # reference cycle in FP32
-    _, ms64s, info64s = other("fp64-cheb", pkg.GMGS_PREC_FP64, cheb_flags, cheb_sweeps)
-    _, msmr, infomr = other("mixed-ref", pkg.GMGS_PREC_MIXED, base_flags)
-    # the symmetric V(2,2) cycle with the plain Jacobi weight in every sweep
-    _, msmj, infomj = other("mixed-sym-jacobi", pkg.GMGS_PREC_MIXED, sym_flags)
-    log(f"[bench] mixed vs fp64 rel L2 {err:.2e}; speed-up {ms64 / ms_per_step:.3f} "
-        f"(precision only {ms64s / ms_per_step:.3f})")
+    if args.headline_only:
+        cmp = None
+    else:
+        # the reference algorithm in FP64 (kernel/Vcycle.m cycle, Jacobi step,
+        # Fletcher-Reeves): the baseline of the speed-up
+        U64, ms64, info64 = other("fp64", pkg.GMGS_PREC_FP64, base_flags)
+        err = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
+        # the same cycle in FP64 (precision-only comparison) and the reference
+        # cycle in FP32
+        _, ms64s, info64s = other("fp64-cheb", pkg.GMGS_PREC_FP64, cheb_flags, cheb_sweeps)
+        _, msmr, infomr = other("mixed-ref", pkg.GMGS_PREC_MIXED, base_flags)
+        # the symmetric V(2,2) cycle with the plain Jacobi weight in every sweep
+        _, msmj, infomj = other("mixed-sym-jacobi", pkg.GMGS_PREC_MIXED, sym_flags)
+        log(f"[bench] mixed vs fp64 rel L2 {err:.2e}; speed-up {ms64 / ms_per_step:.3f} "
+            f"(precision only {ms64s / ms_per_step:.3f})")
+        cmp = {"its_fp64": info64.its, "converged_fp64": bool(info64.converged),
+               "true_relres_fp64": info64.true_relres,
+               "fp64_ms": round(ms64, 3),
+               "fp64_mdofs": round(ndofs / (ms64 * 1e-3) / 1e6, 3),
+               "speedup_vs_fp64": round(ms64 / ms_per_step, 3),
+               "fp64_same_cycle_ms": round(ms64s, 3), "its_fp64_same_cycle": info64s.its,
+               "speedup_precision_only": round(ms64s / ms_per_step, 3),
+               "mixed_reference_cycle_ms": round(msmr, 3), "its_mixed_reference_cycle": infomr.its,
+               "mixed_jacobi_sweeps_ms": round(msmj, 3), "its_mixed_jacobi_sweeps": infomj.its,
+               "mixed_vs_fp64_rel_l2": err}
 
     # the level-1 FP64 operators, back-to-back launches between one event pair
     mv_ms = s.time_operator("matvec64", args.matvec_reps)
@@ -255,7 +270,7 @@ def main():
     s.kernel_timing(False)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and cmp is not None:
         log(f"[bench] cpu baseline: oracle set-up + {args.cpu_iters} timed iterations")
         per_dof, sdofs, dt, sname = cpu_baseline(args.seed, args.cpu_iters)
         t_cpu = per_dof * ndofs * info64.its
@@ -283,18 +298,10 @@ def main():
                    "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 2 on level 1, 4 below), "
                              "exact Galerkin coarse operators, no Jacobi step, FP64 flexible PCG"),
                    "fp64": "reference algorithm: kernel/Vcycle.m V-cycle, Jacobi step, FP64 PCG",
-                   "its_mixed": info.its, "its_fp64": info64.its,
-                   "converged_mixed": bool(info.converged), "converged_fp64": bool(info64.converged),
-                   "true_relres_mixed": info.true_relres, "true_relres_fp64": info64.true_relres,
-                   "fp64_ms": round(ms64, 3),
-                   "fp64_mdofs": round(ndofs / (ms64 * 1e-3) / 1e6, 3),
-                   "speedup_vs_fp64": round(ms64 / ms_per_step, 3),
-                   "fp64_same_cycle_ms": round(ms64s, 3), "its_fp64_same_cycle": info64s.its,
-                   "speedup_precision_only": round(ms64s / ms_per_step, 3),
-                   "mixed_reference_cycle_ms": round(msmr, 3), "its_mixed_reference_cycle": infomr.its,
-                   "mixed_jacobi_sweeps_ms": round(msmj, 3), "its_mixed_jacobi_sweeps": infomj.its,
+                   "its_mixed": info.its, "converged_mixed": bool(info.converged),
+                   "true_relres_mixed": info.true_relres,
+                   **(cmp or {"comparisons": "skipped (--headline-only)"}),
                    "smoother_lmax": [round(s.smoother(li)[0], 5) for li in range(1, s.num_levels)],
-                   "mixed_vs_fp64_rel_l2": err,
                    "setup_s": round(setup_s, 3)},
         "roofline": {"kernel": "k_apply_zm<f64,AX> (level-1 matvec q = A p)", "bound": "hbm",
                      "achieved": round(achieved, 1),

@@ -179,6 +179,37 @@ def test_assembled_coarse_operators(case, oracle, pkg):
     s.close()
 
 
+def _mixed_cycle(pkg, vox, bbox, fixed, loads, L, omega, extra):
+    s = pkg.Solver(vox, bbox, num_levels=L, jacobi_weight=omega, tol=1e-6, precision=pkg.GMGS_PREC_MIXED,
+                   vcycle_sweeps=2, vcycle_sweeps_coarse=4,
+                   flags=pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP | pkg.FLAG_EXACT_GALERKIN | extra)
+    s.set_bc(fixed, loads)
+    r = np.random.default_rng(31).standard_normal(3 * s.num_nodes(1))
+    z = s.vcycle(r)
+    s.close()
+    return z
+
+
+def test_assembled_coarse_mixed_cycle(case, pkg):
+    """FP32 cycle with assembled coarse rows against the element-sum kernels.
+    The rows are stored whole: a row stored as a difference from the universal
+    stencil cancels in FP32 on sparsely filled coarse elements."""
+    a = (pkg, case["vox"], case["bbox"], case["fixed"], case["loads"], case["m"].numLevels, case["omega"])
+    za = _mixed_cycle(*a, pkg.FLAG_ASSEMBLED_COARSE)
+    zs = _mixed_cycle(*a, pkg.FLAG_SIMPLE_KERNELS)
+    log(f"  mixed cycle assembled vs element sums: rel {rel(za, zs):.2e}")
+    assert rel(za, zs) < 1e-4
+
+
+def test_assembled_coarse_mixed_cycle_six_levels(pkg):
+    p = pkg.workloads.ellipsoid_problem(96, 88, 160, 6, seed=3)
+    a = (pkg, p.vox, p.bbox, p.fixed, p.loads, p.num_levels, p.omega)
+    za = _mixed_cycle(*a, pkg.FLAG_ASSEMBLED_COARSE)
+    zs = _mixed_cycle(*a, pkg.FLAG_SIMPLE_KERNELS)
+    log(f"  6-level mixed cycle assembled vs element sums: rel {rel(za, zs):.2e}")
+    assert rel(za, zs) < 1e-4
+
+
 def test_solve_fp64_matches_oracle(case, oracle):
     m, s = case["m"], case["s"]
     tol = 1e-8
