@@ -60,6 +60,11 @@ struct Level {
   long long as_stride = 0;
   bool as_built = false, as = false;
   int col_tx = 8, col_zc = 16;
+  // level 1 of a mixed context: the FP64 matvec's own column tiles (the
+  // cycle's FP32 sweeps are tuned on coltiles)
+  DevBuf coltiles64;
+  long long ncol64 = 0;
+  int col_tx64 = 0, col_zc64 = 0;
   // symmetric cycle: nu sweeps with weights om, and the estimate of
   // lambda_max(D^-1 A) on the free DOFs they came from (0 = plain Jacobi)
   int nu = 2;
@@ -234,8 +239,11 @@ int apply_level(gmgs_ctx* c, int li, int mode, const TR* in, T omega, T* out, TR
     return 0;
   }
   if (l.zm) {
-    const int np = launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc,
-                                          mode, in, omega, in2, out, partial, done, c->stream);
+    const bool own64 = std::is_same<T, double>::value && mode == AP_AX && !l.coltiles64.empty();
+    const int np = own64 ? launch_apply_zm<T, TR>(op, l.coltiles64.as<ColTile>(), l.ncol64, l.col_tx64, l.col_zc64,
+                                                  mode, in, omega, in2, out, partial, done, c->stream)
+                         : launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc,
+                                                  mode, in, omega, in2, out, partial, done, c->stream);
     if (xacc)
       launch_jacobi_acc<T, TR>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles,
                                op.invd, in, omega, xacc, c->stream);
@@ -590,7 +598,7 @@ float time_v1(gmgs_ctx* c, int li, const TR* in, T* out, bool as = false) {
 
 template <typename T, typename TR>
 float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, int zc, const TR* in,
-              T* out) {
+              T* out, int mode = AP_SMOOTH) {
   Level& l = c->lev[li];
   hipStream_t s = c->stream;
   LevelOp<T> op = make_op<T>(c, li);
@@ -598,11 +606,11 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
   GMGS_HIP(hipEventCreate(&a));
   GMGS_HIP(hipEventCreate(&b));
   for (int w = 0; w < 2; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, nullptr, out, nullptr, nullptr, s);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, nullptr, out, nullptr, nullptr, s);
   GMGS_HIP(hipEventRecord(a, s));
   const int reps = 5;
   for (int w = 0; w < reps; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, nullptr, out, nullptr, nullptr, s);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, nullptr, out, nullptr, nullptr, s);
   GMGS_HIP(hipEventRecord(b, s));
   GMGS_HIP(hipEventSynchronize(b));
   float ms = 0;
@@ -618,7 +626,8 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
 // partials of any reduction: the PCG kernels and an AX on any level
 long long partial_capacity(gmgs_ctx* c) {
   long long n = c->nparts;
-  for (const Level& l : c->lev) n = std::max<long long>(n, std::max<long long>(apply_blocks(l.ntiles), l.ncol));
+  for (const Level& l : c->lev)
+    n = std::max<long long>(n, std::max<long long>(apply_blocks(l.ntiles), std::max(l.ncol, l.ncol64)));
   return n;
 }
 
@@ -680,6 +689,36 @@ void autotune_levels(gmgs_ctx* c) {
       l.col_tx = best.first;
       l.col_zc = best.second;
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
+    }
+  }
+  // level 1 of a mixed context: the FP64 matvec of the PCG on tiles of its own
+  Level& l1 = L1(c);
+  if (c->mixed && l1.zm) {
+    const TuneKey key{l1.mesh.g.ey, l1.mesh.g.ex, l1.mesh.g.ez, 2};
+    std::pair<int, int> best{l1.col_tx, l1.col_zc};
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      auto it = tune_cache().find(key);
+      if (it != tune_cache().end()) { best = it->second; cached = true; }
+    }
+    if (!cached) {
+      float best_ms = 1e30f;
+      for (const auto& cd : cand) {
+        DevBuf tl;
+        const long long nt = build_coltiles(l1.mesh.g, l1.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
+        const float ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
+                                                 c->q.as<double>(), AP_AX);
+        if (ms < best_ms) { best_ms = ms; best = {cd[0], cd[1]}; }
+      }
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      tune_cache()[key] = best;
+    }
+    if (best.first != l1.col_tx || best.second != l1.col_zc) {
+      l1.col_tx64 = best.first;
+      l1.col_zc64 = best.second;
+      l1.ncol64 = build_coltiles(l1.mesh.g, l1.mesh.flags.as<uint32_t>(), best.first, best.second,
+                                 l1.coltiles64, s);
     }
   }
   const long long np_max = partial_capacity(c);
@@ -914,7 +953,10 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   const int nz = run_vcycle<T>(c, ra, z, zprev, part, done);
   launch_finalize(1, part, nz, sc, nullptr, nullptr, done, s);
   // p = z + beta p (CG_solver.m:23-28)
-  launch_update_p<T>(g, fl, tl, l.ntiles, z, c->p.as<double>(), sc, done, s);
+  // without the Jacobi step, x += alpha p of the previous iteration rides in
+  // this pass over p (read once per iteration); the last one follows the loop
+  launch_update_p<T>(g, fl, tl, l.ntiles, z, c->p.as<double>(), sc, done, s,
+                     c->no_jacobi ? c->x.as<double>() : nullptr);
   // q = A p, p'q (CG_solver.m:29-31)
   int npq;
   {
@@ -924,7 +966,8 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   }
   launch_finalize(2, part, npq, sc, nullptr, nullptr, done, s);
   // x += alpha p; r = r - alpha q; ||r|| (CG_solver.m:32-39)
-  launch_update_xr(g, fl, tl, l.ntiles, c->x.as<double>(), c->p.as<double>(), ra, c->q.as<double>(),
+  launch_update_xr(g, fl, tl, l.ntiles, c->no_jacobi ? nullptr : c->x.as<double>(), c->p.as<double>(), ra,
+                   c->q.as<double>(),
                    c->r.as<double>(), c->no_jacobi ? nullptr : l.invd64.as<double>(),
                    c->prm.jacobi_weight, sc, part, done, s);
   launch_finalize(3, part, np, sc, c->hist_rel.as<double>(), c->hist_t.as<unsigned long long>(),
@@ -962,6 +1005,15 @@ struct LoopResult {
   int its = 0, converged = 0;
   double relres = 0, seconds = 0;
 };
+
+// the x update of the last iteration, deferred into an update of p that the
+// finished loop no longer runs (see pcg_iteration)
+void finish_deferred_x(gmgs_ctx* c, hipStream_t s) {
+  if (!c->no_jacobi) return;
+  Level& l = L1(c);
+  launch_add_alpha_p(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->x.as<double>(),
+                     c->p.as<double>(), c->sc.as<PcgScalars>(), s);
+}
 
 void invalidate_graph(gmgs_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1044,6 +1096,7 @@ LoopResult pcg_run(gmgs_ctx* c) {
     qa = qb;
     qb = launched;
   }
+  finish_deferred_x(c, s);
   GMGS_HIP(hipEventRecord(e1, s));
   GMGS_HIP(hipEventSynchronize(e1));
   GMGS_HIP(hipMemcpy(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost));
@@ -1608,6 +1661,7 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
       if (all) break;
       chunk = std::min(chunk * 2, 16);
     }
+    for (int i = 0; i < n; ++i) finish_deferred_x(cs[i], s0);
     GMGS_HIP(hipEventRecord(t1, s0));
     GMGS_HIP(hipEventSynchronize(t1));
     float ms = 0;

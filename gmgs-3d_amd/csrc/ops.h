@@ -139,13 +139,20 @@ void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, 
                     const T* invd, const CycleSum<T>& cs, const TR* r0, const T* e, TZ* x, bool fix,
                     const TZ* zprev, double* partial, const int* done, hipStream_t s);
 
+// p = z + beta p; with x, first x += alpha p (the previous iteration's x
+// update, deferred so that p is read once per iteration)
 template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
-                     const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s);
+                     const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s,
+                     double* x = nullptr);
+
+// x += alpha p, unconditionally (the last deferred x update, after the PCG loop)
+void launch_add_alpha_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, double* x,
+                        const double* p, const PcgScalars* sc, hipStream_t s);
 
 // x = (x + omega r / d) + alpha p (the Jacobi increment of the iteration, from
-// r before it is replaced; none when invd is null), r = r_a - alpha q (r_a may
-// be r), partials of r.r
+// r before it is replaced; none when invd is null; no x update when x is
+// null), r = r_a - alpha q (r_a may be r), partials of r.r
 void launch_update_xr(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                       double* x, const double* p, const double* ra, const double* q, double* r,
                       const double* invd, double omega, const PcgScalars* sc, double* partial,

@@ -504,22 +504,43 @@ template <typename TZ>
 __global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __restrict__ flags,
                                                   const TileRec* __restrict__ tiles, long long nt,
                                                   const TZ* __restrict__ z, double* __restrict__ p,
+                                                  double* __restrict__ x,
                                                   const PcgScalars* __restrict__ sc,
                                                   const int* __restrict__ done) {
   if (done && *done) return;
-  const double beta = sc->beta;
+  const double beta = sc->beta, alpha = sc->alpha;
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
     const long long n = ln_.n;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) p[c * N + n] = (double)z[c * N + n] + beta * p[c * N + n];
+    for (int c = 0; c < 3; ++c) {
+      const double po = p[c * N + n];
+      if (x) x[c * N + n] += alpha * po;   // the previous iteration's x update
+      p[c * N + n] = (double)z[c * N + n] + beta * po;
+    }
+  }
+}
+
+// x += alpha p (the last iteration's deferred x update; runs after the loop)
+__global__ __launch_bounds__(256) void k_add_alpha_p(Grid g, const uint32_t* __restrict__ flags,
+                                                     const TileRec* __restrict__ tiles, long long nt,
+                                                     double* __restrict__ x, const double* __restrict__ p,
+                                                     const PcgScalars* __restrict__ sc) {
+  const double alpha = sc->alpha;
+  const long long N = g.N;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+    const long long n = ln_.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c * N + n] += alpha * p[c * N + n];
   }
 }
 
 // x = (x + omega r / d) + alpha p with the iteration's Jacobi increment
-// taken from r before it is replaced (no increment when invd is null);
-// r = r_a - alpha q (r_a may be r itself); partials of r.r
+// taken from r before it is replaced (no increment when invd is null; no x
+// update at all when x is null); r = r_a - alpha q (r_a may be r itself);
+// partials of r.r
 __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __restrict__ flags,
                                                    const TileRec* __restrict__ tiles, long long nt,
                                                    double* __restrict__ x, const double* __restrict__ p,
@@ -537,10 +558,10 @@ __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __res
     const long long N = g.N, n = ln_.n;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      if (invd) {
+      if (x && invd) {
         const double u = omega * invd[c * N + n] * r[c * N + n];
         x[c * N + n] = (x[c * N + n] + u) + alpha * p[c * N + n];
-      } else {
+      } else if (x) {
         x[c * N + n] += alpha * p[c * N + n];
       }
       const double rv = ra[c * N + n] - alpha * q[c * N + n];
@@ -1001,8 +1022,16 @@ GMGS_INST_COMBINE(float, double, float)
 
 template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
-                     const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s) {
-  k_update_p<TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, p, sc, done);
+                     const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s,
+                     double* x) {
+  k_update_p<TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, p, x, sc, done);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_add_alpha_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, double* x,
+                        const double* p, const PcgScalars* sc, hipStream_t s) {
+  if (nt <= 0) return;
+  k_add_alpha_p<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, sc);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1139,10 +1168,10 @@ template void launch_prolong_smooth<float, double, float>(
     const double*, const float*, float, float*, bool, const float*, double*, const int*, hipStream_t);
 template void launch_update_p<double>(const Grid&, const uint32_t*, const TileRec*, long long,
                                       const double*, double*, const PcgScalars*, const int*,
-                                      hipStream_t);
+                                      hipStream_t, double*);
 template void launch_update_p<float>(const Grid&, const uint32_t*, const TileRec*, long long,
                                      const float*, double*, const PcgScalars*, const int*,
-                                     hipStream_t);
+                                     hipStream_t, double*);
 template void launch_convert<double, float>(const double*, float*, long long, hipStream_t);
 template void launch_convert<float, double>(const float*, double*, long long, hipStream_t);
 template void launch_fill<double>(double*, double, long long, hipStream_t);
