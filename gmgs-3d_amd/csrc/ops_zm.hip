@@ -79,16 +79,17 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     own_hi = r >> 16;
   }
 
-  // this thread's share of a staged plane: global offset (without z) or -1
-  // for a position outside the grid or on a column without solid nodes, and
-  // that column's solid z range (lo in the low 16 bits, hi - lo above)
+  // this thread's share of a staged plane: global offset (without z) and
+  // that column's solid z range (lo in the low 16 bits, hi - lo above); a
+  // position outside the grid or on a column without solid nodes gets the
+  // range lo = 0xffff, hi - lo = 0, which no plane index matches
   int soff[PER];
   int zr[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = tid + i * NT;
-    soff[i] = -1;
-    zr[i] = 0;
+    soff[i] = 0;
+    zr[i] = 0xffff;
     if (e < NSTAGE) {
       const int c = e / (SX * SY), rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
       const int gy = t.y0 - 1 + yy, gx = t.x0 - 1 + xx;
@@ -107,11 +108,13 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   // later): the input and, for the smoother, the inverse diagonal.  Index 0 of
   // every padded array is a corner halo entry and holds 0, so a skipped
   // position loads from there: the loads need no branches.
+  // The index is formed in 32 bits (3 N < 2^31) and selected, not branched on.
   auto fetch = [&](int zz, TR* vr, T* vd) {
+    const int szz = (int)sz * zz;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const bool ok = soff[i] >= 0 && (unsigned)(zz - (zr[i] & 0xffff)) <= (unsigned)(zr[i] >> 16);
-      const long long m = ok ? (long long)soff[i] + sz * zz : 0;
+      const bool ok = (unsigned)(zz - (zr[i] & 0xffff)) <= (unsigned)(zr[i] >> 16);
+      const unsigned m = ok ? (unsigned)(soff[i] + szz) : 0u;
       vr[i] = in[m];
       if (SM) vd[i] = op.invd[m];
     }
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   // flags of the outputs this lane owns, loaded one plane before they are used
   auto flags_at = [&](const int z) -> uint32_t {
     const bool ok = lane_ok && z >= z0 && z < z1 && z >= own_lo && z <= own_hi;
-    return op.flags[ok ? colidx + sz * z : 0];    // flags[0]: a halo node, 0
+    return op.flags[ok ? (unsigned)((int)colidx + (int)sz * z) : 0u];    // flags[0]: a halo node, 0
   };
   uint32_t fN = flags_at(z0);
 
