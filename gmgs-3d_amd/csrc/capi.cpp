@@ -199,7 +199,7 @@ LevelOp<T> make_op(gmgs_ctx* c, int li, bool cyc = false) {
     op.Ks = l.Ks64.empty() ? nullptr : l.Ks64.as<double>();
     op.Ke = l.Ke64.as<double>();
     op.stencil = l.S64.as<double>();
-    op.cls = l.C64.empty() ? nullptr : l.C64.as<double>();
+    op.cls = (l.C64.empty() || !l.classes_ok) ? nullptr : l.C64.as<double>();
     op.corner = l.Kc64.empty() ? nullptr : l.Kc64.as<double>();
     op.invd = l.invd64.empty() ? nullptr : l.invd64.as<double>();
     op.emod = (li == 0 && c->simp) ? c->emod64.as<double>() : nullptr;
@@ -207,7 +207,7 @@ LevelOp<T> make_op(gmgs_ctx* c, int li, bool cyc = false) {
     op.Ks = l.Ks32.empty() ? nullptr : l.Ks32.as<float>();
     op.Ke = l.Ke32.as<float>();
     op.stencil = l.S32.as<float>();
-    op.cls = l.C32.empty() ? nullptr : l.C32.as<float>();
+    op.cls = (l.C32.empty() || !l.classes_ok) ? nullptr : l.C32.as<float>();
     op.corner = l.Kc32.empty() ? nullptr : l.Kc32.as<float>();
     op.invd = l.invd32.empty() ? nullptr : l.invd32.as<float>();
     op.emod = (li == 0 && c->simp) ? c->emod32.as<float>() : nullptr;
@@ -523,7 +523,7 @@ void build_operators(gmgs_ctx* c) {
     const long long stride = rows + 1;
     const long long asc_n = (long long)kStencilCoefs * stride;
     if (!ar.empty() && (double)asc_n * sizeof(double) <= kAscBudgetBytes &&
-        16.0 * (double)stride < 4294967296.0) {
+        16.0 * (double)stride < 4294967296.0 && 3 * l.mesh.g.N < (1LL << 31)) {
       for (int& v : ar)
         if (v < 0) v = (int)(stride - 1);
       upload(l.arow, ar.data(), ar.size(), s);
@@ -977,26 +977,25 @@ void pcg_iteration(gmgs_ctx* c, int k) {
 void pcg_init(gmgs_ctx* c, double tol) {
   hipStream_t s = c->stream;
   Level& l = L1(c);
-  const long long n3 = 3 * l.mesh.g.N;
-  launch_fill<double>(c->x.as<double>(), 0.0, n3, s);
-  launch_fill<double>(c->p.as<double>(), 0.0, n3, s);
-  if (c->mixed) {
-    launch_fill<float>(c->z[0].as<float>(), 0.0f, n3, s);
-    launch_fill<float>(c->z[1].as<float>(), 0.0f, n3, s);
-  } else {
-    launch_fill<double>(c->z[0].as<double>(), 0.0, n3, s);
-    launch_fill<double>(c->z[1].as<double>(), 0.0, n3, s);
-  }
-  // r1 = b - AtX(0) = b (CG_solver.m:12)
-  GMGS_HIP(hipMemcpyAsync(c->r.as<double>(), c->b.as<double>(), sizeof(double) * n3,
-                          hipMemcpyDeviceToDevice, s));
   PcgScalars h{};
   h.tol = tol;
   h.max_it = c->prm.max_it;
   h.flexible = c->flexible ? 1 : 0;
   GMGS_HIP(hipMemcpyAsync(c->sc.as<void>(), &h, sizeof(h), hipMemcpyHostToDevice, s));
-  launch_dot(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->b.as<double>(),
-             c->b.as<double>(), c->part.as<double>(), s);
+  // x = p = z = 0 and r1 = b - AtX(0) = b (CG_solver.m:11-12) on the solid
+  // nodes, with the partials of b.b; every other entry of these vectors is a
+  // void or halo entry, zero since allocation and never written
+  const Grid& g = l.mesh.g;
+  const uint32_t* fl = l.mesh.flags.as<uint32_t>();
+  const TileRec* tl = l.tiles.as<TileRec>();
+  if (c->mixed)
+    launch_pcg_start<float>(g, fl, tl, l.ntiles, c->b.as<double>(), c->x.as<double>(), c->p.as<double>(),
+                            c->r.as<double>(), c->z[0].as<float>(), c->z[1].as<float>(),
+                            c->part.as<double>(), s);
+  else
+    launch_pcg_start<double>(g, fl, tl, l.ntiles, c->b.as<double>(), c->x.as<double>(), c->p.as<double>(),
+                             c->r.as<double>(), c->z[0].as<double>(), c->z[1].as<double>(),
+                             c->part.as<double>(), s);
   launch_finalize(0, c->part.as<double>(), (int)c->nparts, c->sc.as<PcgScalars>(), nullptr, nullptr,
                   nullptr, s);
 }
@@ -1535,10 +1534,17 @@ static int finish_solve(gmgs_ctx* c, const LoopResult& r, gmgs_solve_info* info,
   LevelOp<double> op = make_op<double>(c, 0);
   launch_apply<double, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_AX, c->x.as<double>(), 0.0,
                                c->tmp.as<double>(), nullptr, nullptr, nullptr, s);
-  launch_sub(c->b.as<double>(), c->tmp.as<double>(), c->tmp.as<double>(), 3 * l.mesh.g.N, s);
-  const double bb = dot_host(c, c->b.as<double>(), c->b.as<double>());
-  const double rr = dot_host(c, c->tmp.as<double>(), c->tmp.as<double>());
-  const double tr = std::sqrt(rr) / std::sqrt(bb);
+  launch_resid_sq(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->b.as<double>(),
+                  c->tmp.as<double>(), c->part.as<double>(), s);
+  std::vector<double> parts((size_t)c->nparts);
+  PcgScalars h{};
+  GMGS_HIP(hipMemcpyAsync(parts.data(), c->part.as<void>(), sizeof(double) * parts.size(),
+                          hipMemcpyDeviceToHost, s));
+  GMGS_HIP(hipMemcpyAsync(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost, s));
+  GMGS_HIP(hipStreamSynchronize(s));
+  double rr = 0.0;
+  for (double v : parts) rr += v;
+  const double tr = std::sqrt(rr) / h.normB;   // normB = |b| (pcg_init)
   const long long nh = r.converged ? r.its - 1 : r.its;
   if (info) {
     info->its = r.its;
@@ -1551,9 +1557,6 @@ static int finish_solve(gmgs_ctx* c, const LoopResult& r, gmgs_solve_info* info,
   if (hist && hist_cap > 0 && nh > 0) {
     std::vector<double> hr = download<double>(c->hist_rel, (size_t)nh, s);
     std::vector<unsigned long long> ht = download<unsigned long long>(c->hist_t, (size_t)nh, s);
-    PcgScalars h{};
-    GMGS_HIP(hipMemcpyAsync(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost, s));
-    GMGS_HIP(hipStreamSynchronize(s));
     for (long long i = 0; i < nh && i < hist_cap; ++i) {
       hist[3 * i + 0] = (double)(i + 1);
       hist[3 * i + 1] = (double)(ht[i] - h.t0) / (c->clock_khz * 1e3);

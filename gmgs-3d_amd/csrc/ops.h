@@ -182,6 +182,16 @@ void launch_xpby(const Grid& g, const uint32_t* flags, const TileRec* tiles, lon
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                 const double* a, const double* b, double* partial, hipStream_t s);
 
+// start of a solve from x0 = 0: x = p = z0 = z1 = 0, r = b, partials of b.b
+template <typename TZ>
+void launch_pcg_start(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                      const double* b, double* x, double* p, double* r, TZ* z0, TZ* z1, double* partial,
+                      hipStream_t s);
+
+// partials of |b - q|^2 over tiles
+void launch_resid_sq(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                     const double* b, const double* q, double* partial, hipStream_t s);
+
 // scalar steps of the PCG iteration; 'which': 0 = init normB, 1 = beta, 2 = alpha, 3 = residual
 void launch_finalize(int which, const double* partial, int nparts, PcgScalars* sc,
                      double* hist_relres, unsigned long long* hist_t, const int* done, hipStream_t s);

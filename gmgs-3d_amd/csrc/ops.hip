@@ -179,6 +179,19 @@ __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __r
 // padded to whole groups, q' = (q / 81) QP + q % 81) so a lane reads its row
 // with coalesced 16-byte loads, one z-plane at a time.  F_FAST lanes share the
 // universal row.
+// Coefficients read through a constant-address-space pointer kept opaque to
+// loop-invariant code motion: scalar loads into SGPR operands where they are
+// used, instead of values hoisted into vector registers.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
+template <typename T>
+__device__ inline cptr<T> opaque(const T* p) {
+  cptr<T> q = (cptr<T>)p;
+  asm volatile("" : "+s"(q));
+  return q;
+}
+
 template <typename T>
 struct alignas(16) CoefGroup {
   T v[16 / sizeof(T)];
@@ -201,16 +214,77 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
   TileRec tr{0, 0, 0, 0};
   if (t < ntiles) {
     tr = tiles[t];
-    for (int e = lane; e < 3 * NS; e += 64) {
-      const int c = e / NS, r = e % NS, zz = r / (SY * SX), xx = (r / SY) % SX, yy = r % SY;
-      const int gy = tr.y0 - 1 + yy, gx = tr.x0 - 1 + xx, gz = tr.z - 1 + zz;
-      T v = 0;
-      if (gy >= 0 && gy < g.py && gx >= 0 && gx < g.px && gz >= 0 && gz < g.pz) {
-        const long long m = c * N + gy + sx * gx + sz * gz;
-        v = (T)in[m];
-        if (MODE == AP_SMOOTH) v = omega * op.invd[m] * v;
+    // Staging, all loads issued before the first is used.  Index 0 of every
+    // padded array is a corner halo entry holding 0: a position outside the
+    // grid loads from there (32-bit indices, 3 N < 2^31 checked at set-up).
+    // Tiles start at y, x >= 0 and hold solid nodes (z >= 1).
+    // Rows (c, zz, xx) of 18 y-values: lane (yy = lane % 16, w = lane / 16)
+    // loads y-values 0..15 of the rows with xx = w and xx = 4 + w (w < 2) of
+    // every (c, zz), and lane l < 54 the y-values 16 and 17 of row l.
+    const int gy0 = tr.y0 - 1, gx0 = tr.x0 - 1, gz0 = tr.z - 1;
+    const int yy = lane & 15, w = lane >> 4;
+    const int isx = (int)sx, isz = (int)sz, iN = (int)N;
+    const bool y_ok = gy0 + yy >= 0 && gy0 + yy < g.py;
+    const bool xa_ok = gx0 + w >= 0 && gx0 + w < g.px, xb_ok = w < 2 && gx0 + 4 + w < g.px;
+    const int ba = gy0 + yy + isx * (gx0 + w), bb = ba + 4 * isx;
+    const int le = lane < 54 ? lane : 0;                   // edge row of this lane
+    const int ec = le / 18, ezz = (le % 18) / 6, exx = le % 6;
+    const bool e_ok = lane < 54 && gx0 + exx >= 0 && gx0 + exx < g.px && gz0 + ezz < g.pz;
+    const int eb = gy0 + 16 + isx * (gx0 + exx) + isz * (gz0 + ezz) + iN * ec;
+    const bool e16 = e_ok && gy0 + 16 < g.py, e17 = e_ok && gy0 + 17 < g.py;
+    TR va[9], vb[9], ve[2];
+    T da[MODE == AP_SMOOTH ? 9 : 1], db[MODE == AP_SMOOTH ? 9 : 1], de[MODE == AP_SMOOTH ? 2 : 1];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int c = k / 3, zz = k % 3;
+      const bool z_ok = gz0 + zz < g.pz;                   // gz0 >= 0: solid nodes have z >= 1
+      const int u = iN * c + isz * (gz0 + zz);
+      const unsigned ma = (y_ok && xa_ok && z_ok) ? (unsigned)(ba + u) : 0u;
+      const unsigned mb = (y_ok && xb_ok && z_ok) ? (unsigned)(bb + u) : 0u;
+      va[k] = in[ma];
+      vb[k] = in[mb];
+      if (MODE == AP_SMOOTH) {
+        da[k] = op.invd[ma];
+        db[k] = op.invd[mb];
       }
-      sU[wave][c][r] = v;
+    }
+    {
+      const unsigned m16 = e16 ? (unsigned)eb : 0u, m17 = e17 ? (unsigned)(eb + 1) : 0u;
+      ve[0] = in[m16];
+      ve[1] = in[m17];
+      if (MODE == AP_SMOOTH) {
+        de[0] = op.invd[m16];
+        de[1] = op.invd[m17];
+      }
+    }
+    // every loaded value is taken here, after the last load is issued: the
+    // compiler may not sink a load into the branch that stores it (which
+    // would put a wait between the loads)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      asm volatile("" : "+v"(va[k]), "+v"(vb[k]));
+      if (MODE == AP_SMOOTH) asm volatile("" : "+v"(da[k]), "+v"(db[k]));
+    }
+    asm volatile("" : "+v"(ve[0]), "+v"(ve[1]));
+    if (MODE == AP_SMOOTH) asm volatile("" : "+v"(de[0]), "+v"(de[1]));
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int c = k / 3, zz = k % 3;
+      T a = (T)va[k], b = (T)vb[k];
+      if (MODE == AP_SMOOTH) {
+        a = omega * da[k] * a;
+        b = omega * db[k] * b;
+      }
+      sU[wave][c][SY * SX * zz + SY * w + yy] = a;
+      if (w < 2) sU[wave][c][SY * SX * zz + SY * (4 + w) + yy] = b;
+    }
+    if (lane < 54) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        T v = (T)ve[k];
+        if (MODE == AP_SMOOTH) v = omega * de[k] * v;
+        sU[wave][ec][SY * SX * ezz + SY * exx + 16 + k] = v;
+      }
     }
   }
   __syncthreads();
@@ -223,26 +297,76 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
     fl = op.flags[n];
   }
   if (fl & F_ELEM_MASK) {
-    // group q0 of this lane's row at a uniform base (q0 rows groups) plus one
-    // 32-bit byte offset (16 rows < 2^32, checked at set-up)
-    const unsigned boff = (unsigned)op.arow[t * 64 + lane] * 16u;
-    const char* __restrict__ base = reinterpret_cast<const char*>(op.asc);
-    const long long gbytes = op.as_stride * 16;
     T a[3] = {0, 0, 0};
+    // F_FAST nodes (all eight elements present and universal): the universal
+    // stencil from its 30 reflection-class coefficients, read as scalar
+    // operands, over the three staged planes (input plane z - 1 + zz, so
+    // oz = zz - 1 is the input's z offset from the output)
+    const bool ucls = op.cls != nullptr;
+    if (ucls && (fl & F_FAST)) {
+      const cptr<T> C = opaque(op.cls);
+      T cf[30];
+#pragma unroll
+      for (int i = 0; i < 30; ++i) cf[i] = C[i];
+#pragma unroll
+      for (int zz = 0; zz < 3; ++zz) {
+        const int oz = zz - 1, q = oz != 0 ? 1 : 0;
+        const T* __restrict__ pl = &sU[wave][0][SY * SX * zz];
+        // neighbour (dy, dx) = (a - 1, b - 1), component c
+#define GMGS_AU(a_, b_, c_) pl[(c_) * NS + SY * (lx + (b_)) + (ly + (a_))]
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const T C0 = GMGS_AU(1, 1, c);
+          const T Cy = GMGS_AU(0, 1, c) + GMGS_AU(2, 1, c);
+          const T Cx = GMGS_AU(1, 0, c) + GMGS_AU(1, 2, c);
+          const T Cxy = (GMGS_AU(0, 0, c) + GMGS_AU(2, 2, c)) + (GMGS_AU(0, 2, c) + GMGS_AU(2, 0, c));
+          a[c] += cf[8 * c + 4 * q] * C0 + cf[8 * c + 1 + 4 * q] * Cy + cf[8 * c + 2 + 4 * q] * Cx
+                  + cf[8 * c + 3 + 4 * q] * Cxy;
+        }
+        const T Dxy_x = (GMGS_AU(2, 2, 0) + GMGS_AU(0, 0, 0)) - (GMGS_AU(2, 0, 0) + GMGS_AU(0, 2, 0));
+        const T Dxy_y = (GMGS_AU(2, 2, 1) + GMGS_AU(0, 0, 1)) - (GMGS_AU(2, 0, 1) + GMGS_AU(0, 2, 1));
+        a[0] += cf[24 + q] * Dxy_y;
+        a[1] += cf[24 + q] * Dxy_x;
+        if (oz != 0) {
+          const T Dx0_x = GMGS_AU(1, 2, 0) - GMGS_AU(1, 0, 0);
+          const T Dx1_x = (GMGS_AU(0, 2, 0) - GMGS_AU(0, 0, 0)) + (GMGS_AU(2, 2, 0) - GMGS_AU(2, 0, 0));
+          const T Dx0_z = GMGS_AU(1, 2, 2) - GMGS_AU(1, 0, 2);
+          const T Dx1_z = (GMGS_AU(0, 2, 2) - GMGS_AU(0, 0, 2)) + (GMGS_AU(2, 2, 2) - GMGS_AU(2, 0, 2));
+          const T Dy0_y = GMGS_AU(2, 1, 1) - GMGS_AU(0, 1, 1);
+          const T Dy1_y = (GMGS_AU(2, 0, 1) - GMGS_AU(0, 0, 1)) + (GMGS_AU(2, 2, 1) - GMGS_AU(0, 2, 1));
+          const T Dy0_z = GMGS_AU(2, 1, 2) - GMGS_AU(0, 1, 2);
+          const T Dy1_z = (GMGS_AU(2, 0, 2) - GMGS_AU(0, 0, 2)) + (GMGS_AU(2, 2, 2) - GMGS_AU(0, 2, 2));
+          const T sgn = (T)oz;
+          a[0] += sgn * (cf[26] * Dx0_z + cf[27] * Dx1_z);
+          a[2] += sgn * (cf[26] * Dx0_x + cf[27] * Dx1_x);
+          a[1] += sgn * (cf[28] * Dy0_z + cf[29] * Dy1_z);
+          a[2] += sgn * (cf[28] * Dy0_y + cf[29] * Dy1_y);
+        }
+#undef GMGS_AU
+      }
+    } else {
+      // the node's assembled row (F_FAST nodes: the universal row): group q0
+      // at a uniform base (q0 rows groups) plus one 32-bit byte offset (16
+      // rows < 2^32, checked at set-up); a wave whose nodes are all F_FAST
+      // skips these loads
+      const unsigned boff = (unsigned)op.arow[t * 64 + lane] * 16u;
+      const char* __restrict__ base = reinterpret_cast<const char*>(op.asc);
+      const long long gbytes = op.as_stride * 16;
 #pragma unroll 1
-    for (int kz = 0; kz < 3; ++kz) {
-      const T* __restrict__ sp = &sU[wave][0][SY * SX * kz + (ly + 1) + SY * (lx + 1)];
+      for (int kz = 0; kz < 3; ++kz) {
+        const T* __restrict__ sp = &sU[wave][0][SY * SX * kz + (ly + 1) + SY * (lx + 1)];
 #pragma unroll
-      for (int g0 = 0; g0 < NGP; ++g0) {
-        const CoefGroup<T> cg =
-            *reinterpret_cast<const CoefGroup<T>*>(base + (long long)(kz * NGP + g0) * gbytes + boff);
+        for (int g0 = 0; g0 < NGP; ++g0) {
+          const CoefGroup<T> cg =
+              *reinterpret_cast<const CoefGroup<T>*>(base + (long long)(kz * NGP + g0) * gbytes + boff);
 #pragma unroll
-        for (int k = 0; k < G; ++k) {
-          const int r = g0 * G + k;                   // coefficient within the z-plane
-          if (r < 81) {
-            const int j = r / 9, cc = (r / 3) % 3, d = r % 3;
-            const int dy = j % 3 - 1, dx = j / 3 - 1;
-            a[cc] += cg.v[k] * sp[d * NS + dy + SY * dx];
+          for (int k = 0; k < G; ++k) {
+            const int r = g0 * G + k;                   // coefficient within the z-plane
+            if (r < 81) {
+              const int j = r / 9, cc = (r / 3) % 3, d = r % 3;
+              const int dy = j % 3 - 1, dx = j / 3 - 1;
+              a[cc] += cg.v[k] * sp[d * NS + dy + SY * dx];
+            }
           }
         }
       }
@@ -672,6 +796,56 @@ __global__ __launch_bounds__(256) void k_dot(Grid g, const uint32_t* __restrict_
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// start of a solve (CG_solver.m:11-13 with x0 = 0): x = p = 0, z0 = z1 = 0,
+// r = b, partials of b.b; solid nodes only (void entries are never written
+// and stay zero)
+template <typename TZ>
+__global__ __launch_bounds__(256) void k_pcg_start(Grid g, const uint32_t* __restrict__ flags,
+                                                   const TileRec* __restrict__ tiles, long long nt,
+                                                   const double* __restrict__ b, double* __restrict__ x,
+                                                   double* __restrict__ p, double* __restrict__ r,
+                                                   TZ* __restrict__ z0, TZ* __restrict__ z1,
+                                                   double* __restrict__ partial) {
+  double d = 0.0;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+    const long long N = g.N, n = ln_.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long m = c * N + n;
+      const double v = b[m];
+      x[m] = 0.0;
+      p[m] = 0.0;
+      r[m] = v;
+      z0[m] = (TZ)0;
+      z1[m] = (TZ)0;
+      d += v * v;
+    }
+  }
+  const double s = block_sum(d);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// partials of |b - q|^2 (the true residual of x when q = A x)
+__global__ __launch_bounds__(256) void k_resid_sq(Grid g, const uint32_t* __restrict__ flags,
+                                                  const TileRec* __restrict__ tiles, long long nt,
+                                                  const double* __restrict__ b,
+                                                  const double* __restrict__ q,
+                                                  double* __restrict__ partial) {
+  double d = 0.0;
+  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
+    if (!ln_.active) continue;
+    const long long N = g.N, n = ln_.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double e = b[c * N + n] - q[c * N + n];
+      d += e * e;
+    }
+  }
+  const double s = block_sum(d);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // one block of 256 threads; fixed-order sums
 __device__ inline double reduce_parts(const double* __restrict__ part, int n) {
   double v = 0.0;
@@ -1094,6 +1268,26 @@ GMGS_INST_LANCZOS(float)
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                 const double* a, const double* b, double* partial, hipStream_t s) {
   k_dot<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, b, partial);
+  GMGS_LAUNCH_CHECK();
+}
+
+template <typename TZ>
+void launch_pcg_start(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                      const double* b, double* x, double* p, double* r, TZ* z0, TZ* z1, double* partial,
+                      hipStream_t s) {
+  k_pcg_start<TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, b, x, p, r, z0, z1, partial);
+  GMGS_LAUNCH_CHECK();
+}
+template void launch_pcg_start<double>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                       const double*, double*, double*, double*, double*, double*,
+                                       double*, hipStream_t);
+template void launch_pcg_start<float>(const Grid&, const uint32_t*, const TileRec*, long long,
+                                      const double*, double*, double*, double*, float*, float*, double*,
+                                      hipStream_t);
+
+void launch_resid_sq(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
+                     const double* b, const double* q, double* partial, hipStream_t s) {
+  k_resid_sq<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, b, q, partial);
   GMGS_LAUNCH_CHECK();
 }
 
