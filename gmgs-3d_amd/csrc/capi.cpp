@@ -59,6 +59,10 @@ struct Level {
   DevBuf asc64, asc32, arow;
   long long as_stride = 0;
   bool as_built = false, as = false;
+  // split operator: the z-marching kernel on the F_FAST nodes and the
+  // assembled rows on the others (nfnode: padded index of row r's node)
+  DevBuf nfnode;
+  bool split_ok = false, split = false;
   int col_tx = 8, col_zc = 16;
   // level 1 of a mixed context: the FP64 matvec's own column tiles (the
   // cycle's FP32 sweeps are tuned on coltiles)
@@ -95,6 +99,7 @@ struct gmgs_ctx {
   int sweeps = 2, sweeps_coarse = 2;   // nu of level 1, of the coarser smoothed levels
   bool cheb = false;             // Chebyshev sweep weights in the symmetric cycle
   bool prefer_as = false;        // assembled stencil on every coarse level where it fits
+  bool prefer_split = false;     // split operator wherever it exists, the assembled stencil elsewhere
   bool no_jacobi = false;        // no Jacobi step opening each PCG iteration
   // SIMP
   bool simp = false;
@@ -212,7 +217,7 @@ LevelOp<T> make_op(gmgs_ctx* c, int li, bool cyc = false) {
     op.invd = l.invd32.empty() ? nullptr : l.invd32.as<float>();
     op.emod = (li == 0 && c->simp) ? c->emod32.as<float>() : nullptr;
   }
-  if (l.as) {
+  if (l.as || l.split) {
     if constexpr (std::is_same<T, double>::value) op.asc = l.asc64.as<double>();
     else op.asc = l.asc32.as<float>();
   }
@@ -231,6 +236,16 @@ int apply_level(gmgs_ctx* c, int li, int mode, const TR* in, T omega, T* out, TR
                 double* partial, const int* done, const T* in2 = nullptr, bool cyc = false) {
   Level& l = c->lev[li];
   LevelOp<T> op = make_op<T>(c, li, cyc);
+  if (l.split) {
+    int np = launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc, mode, in, omega,
+                                    in2, out, partial, done, c->stream, nullptr, true);
+    np += launch_apply_rows<T, TR>(op, l.nfnode.as<int>(), l.as_stride - 1, mode, in, omega, in2, out,
+                                   partial ? partial + np : nullptr, done, c->stream);
+    if (xacc)
+      launch_jacobi_acc<T, TR>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles,
+                               op.invd, in, omega, xacc, c->stream);
+    return np;
+  }
   if (mode == AP_RESID && !l.zm) {
     launch_apply<T, TR>(op, l.tiles.as<TileRec>(), l.ntiles, AP_AX, in, omega, out, nullptr, nullptr,
                         done, c->stream);
@@ -501,12 +516,14 @@ void build_operators(gmgs_ctx* c) {
     // FP64): one per solid node that is not F_FAST, in tile-slot order, and
     // the universal row last; used where the z-marching kernel is not
     l.as = l.as_built = false;
+    l.split = l.split_ok = false;
+    l.nfnode.release();
     l.asc64.release();
     l.asc32.release();
     l.arow.release();
     l.as_stride = 0;
     long long rows = 0;
-    std::vector<int> ar;
+    std::vector<int> ar, nodes;
     if (li >= 1 && li < L - 1 && !c->simple_kernels && l.ntiles > 0) {
       const Grid& g = l.mesh.g;
       const std::vector<TileRec> th = download<TileRec>(l.tiles, (size_t)l.ntiles, s);
@@ -517,7 +534,10 @@ void build_operators(gmgs_ctx* c) {
           const int yp = th[t].y0 + (lane & 15), xp = th[t].x0 + (lane >> 4);
           if (xp >= g.px) continue;
           const uint32_t f = fh[yp + (long long)g.py * (xp + (long long)g.px * th[t].z)];
-          if ((f & F_ELEM_MASK) && !(f & F_FAST)) ar[t * 64 + lane] = (int)rows++;
+          if ((f & F_ELEM_MASK) && !(f & F_FAST)) {
+            ar[t * 64 + lane] = (int)rows++;
+            nodes.push_back((int)(yp + (long long)g.py * (xp + (long long)g.px * th[t].z)));
+          }
         }
     }
     const long long stride = rows + 1;
@@ -539,10 +559,14 @@ void build_operators(gmgs_ctx* c) {
         l.asc64.alloc_zero(sizeof(double) * stencil_group_elems(stride, sizeof(double)), s);
         launch_relayout_stencil<double>(canon.as<double>(), stride, l.asc64.as<double>(), s);
       }
+      if (!nodes.empty()) upload(l.nfnode, nodes.data(), nodes.size(), s);
       GMGS_HIP(hipStreamSynchronize(s));
       l.as_built = true;
-      l.as = !l.zm || c->prefer_as;
-      if (c->prefer_as) l.zm = false;
+      // the split operator needs the z-marching kernel's tiles
+      l.split_ok = l.zm && !nodes.empty();
+      l.split = l.split_ok && c->prefer_split;
+      l.as = !l.split && (!l.zm || c->prefer_as || c->prefer_split);
+      if (c->prefer_as || c->prefer_split) l.zm = false;
     }
   }
   GMGS_HIP(hipStreamSynchronize(s));
@@ -621,13 +645,48 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
   return ms / reps;
 }
 
+// the split operator on (tx, zc) column tiles: z-marching kernel on the
+// F_FAST nodes, then the assembled rows
+template <typename T, typename TR>
+float time_split(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, int zc, const TR* in,
+                 T* out) {
+  Level& l = c->lev[li];
+  hipStream_t s = c->stream;
+  const bool keep = l.split;
+  l.split = true;
+  LevelOp<T> op = make_op<T>(c, li);
+  l.split = keep;
+  auto run = [&]() {
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, nullptr, out, nullptr, nullptr, s,
+                           nullptr, true);
+    launch_apply_rows<T, TR>(op, l.nfnode.as<int>(), l.as_stride - 1, AP_SMOOTH, in, (T)0.5, nullptr, out,
+                             nullptr, nullptr, s);
+  };
+  hipEvent_t a, b;
+  GMGS_HIP(hipEventCreate(&a));
+  GMGS_HIP(hipEventCreate(&b));
+  for (int w = 0; w < 2; ++w) run();
+  GMGS_HIP(hipEventRecord(a, s));
+  const int reps = 5;
+  for (int w = 0; w < reps; ++w) run();
+  GMGS_HIP(hipEventRecord(b, s));
+  GMGS_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  GMGS_HIP(hipEventElapsedTime(&ms, a, b));
+  GMGS_HIP(hipEventDestroy(a));
+  GMGS_HIP(hipEventDestroy(b));
+  return ms / reps;
+}
+
 // dot partials any level-1 launch may write: the tile kernels (one per 4
 // tiles), the pointwise kernels, or the z-marching operator (one per tile)
 // partials of any reduction: the PCG kernels and an AX on any level
 long long partial_capacity(gmgs_ctx* c) {
   long long n = c->nparts;
-  for (const Level& l : c->lev)
+  for (const Level& l : c->lev) {
     n = std::max<long long>(n, std::max<long long>(apply_blocks(l.ntiles), std::max(l.ncol, l.ncol64)));
+    if (l.split_ok) n = std::max<long long>(n, l.ncol + apply_rows_blocks(l.as_stride - 1));
+  }
   return n;
 }
 
@@ -638,10 +697,11 @@ void autotune_levels(gmgs_ctx* c) {
   for (int li = 0; li < c->L - 1; ++li) {
     Level& l = c->lev[li];
     if (!l.zm && li == 0) continue;
-    if (li > 0 && c->prefer_as && l.as_built) continue;   // the assembled stencil was asked for
+    // the assembled stencil or the split operator was asked for
+    if (li > 0 && (c->prefer_as || c->prefer_split) && l.as_built) continue;
     const TuneKey key{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, c->mixed ? 1 : 0};
-    // (tx, zc) of the z-marching kernel; (0, 0) the tile kernel, (0, 1) the
-    // assembled stencil
+    // (tx, zc) of the z-marching kernel; (-tx, zc) the split operator on those
+    // tiles; (0, 0) the tile kernel, (0, 1) the assembled stencil
     std::pair<int, int> best{l.zm ? l.col_tx : 0, l.zm ? l.col_zc : (l.as ? 1 : 0)};
     bool cached = false;
     {
@@ -664,6 +724,13 @@ void autotune_levels(gmgs_ctx* c) {
             else ms = time_zm<double, double>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<double>(), l.rp.as<double>());
           }
           if (ms < best_ms) { best_ms = ms; best = {cd[0], cd[1]}; }
+          if (li > 0 && l.split_ok) {
+            ms = c->mixed ? time_split<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(),
+                                                     l.rp.as<float>())
+                          : time_split<double, double>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1],
+                                                       l.r.as<double>(), l.rp.as<double>());
+            if (ms < best_ms) { best_ms = ms; best = {-cd[0], cd[1]}; }
+          }
         }
       }
       // coarse levels (small, many stored elements) may run faster on the tile
@@ -679,14 +746,18 @@ void autotune_levels(gmgs_ctx* c) {
       tune_cache()[key] = best;
     }
     if (best.first == 0) {
-      l.zm = false;
+      l.zm = l.split = false;
       l.as = best.second == 1 && l.as_built;
       continue;
     }
     if (!l.zm) continue;   // a cached z-marching shape this level cannot use
+    if (best.first < 0 && !l.split_ok) continue;
     l.as = false;
-    if (best.first != l.col_tx || best.second != l.col_zc) {
-      l.col_tx = best.first;
+    l.split = best.first < 0;
+    if (l.split) l.zm = false;
+    const int tx = std::abs(best.first);
+    if (tx != l.col_tx || best.second != l.col_zc) {
+      l.col_tx = tx;
       l.col_zc = best.second;
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
     }
@@ -1425,6 +1496,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->no_jacobi = (p->flags & GMGS_FLAG_NO_JACOBI_STEP) != 0;
     c->cheb = (p->flags & GMGS_FLAG_CHEBYSHEV) != 0;
     c->prefer_as = (p->flags & GMGS_FLAG_ASSEMBLED_COARSE) != 0;
+    c->prefer_split = (p->flags & GMGS_FLAG_SPLIT_COARSE) != 0;
     if (c->cheb && !c->sym)
       throw Status(GMGS_ERR_ARGUMENT, "Chebyshev smoothing needs the symmetric cycle");
     c->nely = (int)ny;
@@ -2008,7 +2080,7 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
       case GMGS_Q_NUM_TILES: i64[0] = l->ntiles; break;
       case GMGS_Q_NUM_FREE_COARSEST: i64[0] = c->nfree; break;
       case GMGS_Q_TILE_CONFIG:
-        i64[0] = l->zm ? 1 : (l->as ? 2 : 0); i64[1] = l->col_tx; i64[2] = l->col_zc; break;
+        i64[0] = l->split ? 3 : (l->zm ? 1 : (l->as ? 2 : 0)); i64[1] = l->col_tx; i64[2] = l->col_zc; break;
       case GMGS_Q_ELE_MAP_BACK: copy_dev(l->mesh.eleMapBack, 4 * need); break;
       case GMGS_Q_ELE_MAP_FORWARD: copy_dev(l->mesh.eleMapForward, 4 * need); break;
       case GMGS_Q_NOD_MAP_BACK: copy_dev(l->mesh.nodMapBack, 4 * need); break;

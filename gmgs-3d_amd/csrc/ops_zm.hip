@@ -42,7 +42,10 @@ __device__ inline cptr<T> opaque(const T* p) {
   return q;
 }
 
-template <typename T, typename TR, int MODE, bool HAS_EOP, int TX>
+// E: 0 = universal elements only (corrections for missing elements), 1 =
+// also stored elements, 2 = F_FAST nodes only (no corrections; the other
+// solid nodes are left to the assembled-row kernel)
+template <typename T, typename TR, int MODE, int E, int TX>
 __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTile* __restrict__ tiles,
                                                       int zc, const TR* __restrict__ in, T omega,
                                                       const T* __restrict__ in2, T* __restrict__ out,
@@ -53,6 +56,9 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
   constexpr bool SUM = MODE == AP_SMOOTH_SUM;
   constexpr bool SM = MODE == AP_SMOOTH || SUM;
+  constexpr bool HAS_EOP = E == 1, FAST_ONLY = E == 2;
+  // output nodes of this kernel
+  constexpr uint32_t OUT_MASK = FAST_ONLY ? F_FAST : F_ELEM_MASK;
   if (done && *done) return;
   // four plane buffers: plane k (counted from the tile's first) is staged in
   // buffer k % 4 and read while computing planes k, k + 1 and k + 2 (stencil,
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     // RESID: centre values of in2 at the output completed in this plane,
     // loaded now and used after the stencil
     T cen[3] = {0, 0, 0};
-    if (MODE == AP_RESID && lane_ok && (fM & F_ELEM_MASK)) {
+    if (MODE == AP_RESID && lane_ok && (fM & OUT_MASK)) {
       const long long n = colidx + sz * (zz - 1);
 #pragma unroll
       for (int c = 0; c < 3; ++c) cen[c] = in2[c * N + n];
@@ -184,8 +190,8 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     // before the next loads are issued and before the class sums exist, where
     // the fewest registers are live; aM is additive.
     uint32_t cm = 0;
-    if (fM & F_ELEM_MASK) cm = ((~fM) & F_ELEM_MASK) | ((fM >> F_STORED_SHIFT) & 0xFFu);
-    if (__ballot(cm != 0u)) {
+    if (!FAST_ONLY && (fM & F_ELEM_MASK)) cm = ((~fM) & F_ELEM_MASK) | ((fM >> F_STORED_SHIFT) & 0xFFu);
+    if (!FAST_ONLY && __ballot(cm != 0u)) {
       const int cnt = __popc(cm);
       int off = cnt;                                   // inclusive prefix over the wave
 #pragma unroll
@@ -319,7 +325,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     }
     // write output zz-1
     const int zo_ = zz - 1;
-    if (lane_ok && zo_ >= z0 && zo_ < z1 && (fM & F_ELEM_MASK)) {
+    if (lane_ok && zo_ >= z0 && zo_ < z1 && (fM & OUT_MASK)) {
       const long long n = colidx + sz * zo_;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -462,12 +468,12 @@ long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, D
 template <typename T, typename TR>
 int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
                     int mode, const TR* in, T omega, const T* in2, T* out, double* partial,
-                    const int* done, hipStream_t s, const CycleOut<T>* co) {
+                    const int* done, hipStream_t s, const CycleOut<T>* co, bool fast_only) {
   if (ntiles <= 0) return 0;
   if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("grid too large for the z-marching operator");
-  const bool eop = op.eop != nullptr;
+  const int e = fast_only ? 2 : (op.eop != nullptr ? 1 : 0);
   const CycleOut<T> cov = co ? *co : CycleOut<T>{nullptr, nullptr, nullptr, nullptr, nullptr, (T)0};
-  if (mode == AP_SMOOTH_SUM && (!co || eop || !op.invd))
+  if (mode == AP_SMOOTH_SUM && (!co || e != 0 || !op.invd))
     throw std::invalid_argument("SMOOTH_SUM: level 1 only, with its outputs");
 #define GMGS_ZM(MODE, E, TXV)                                                                        \
   k_apply_zm<T, TR, MODE, E, TXV><<<(unsigned)ntiles, 16 * TXV, 0, s>>>(op, tiles, zc, in, omega, in2, \
@@ -478,19 +484,23 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
     else if (tx == 16) GMGS_ZM(MODE, E, 16); \
     else GMGS_ZM(MODE, E, 8);             \
   } while (0)
+#define GMGS_ZM_E(MODE)                   \
+  do {                                    \
+    if (e == 1) GMGS_ZM_TX(MODE, 1);      \
+    else if (e == 2) GMGS_ZM_TX(MODE, 2); \
+    else GMGS_ZM_TX(MODE, 0);             \
+  } while (0)
   if (mode == AP_AX) {
-    if (eop) GMGS_ZM_TX(AP_AX, true);
-    else GMGS_ZM_TX(AP_AX, false);
+    GMGS_ZM_E(AP_AX);
   } else if (mode == AP_RESID) {
-    if (eop) GMGS_ZM_TX(AP_RESID, true);
-    else GMGS_ZM_TX(AP_RESID, false);
+    GMGS_ZM_E(AP_RESID);
   } else if (mode == AP_SMOOTH_SUM) {
-    if constexpr (std::is_same<T, TR>::value) GMGS_ZM_TX(AP_SMOOTH_SUM, false);
+    if constexpr (std::is_same<T, TR>::value) GMGS_ZM_TX(AP_SMOOTH_SUM, 0);
     else throw std::invalid_argument("SMOOTH_SUM: input in the cycle precision");
   } else {
-    if (eop) GMGS_ZM_TX(AP_SMOOTH, true);
-    else GMGS_ZM_TX(AP_SMOOTH, false);
+    GMGS_ZM_E(AP_SMOOTH);
   }
+#undef GMGS_ZM_E
 #undef GMGS_ZM_TX
 #undef GMGS_ZM
   GMGS_LAUNCH_CHECK();
@@ -499,12 +509,12 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
 
 template int launch_apply_zm<double, double>(const LevelOp<double>&, const ColTile*, long long, int, int,
                                              int, const double*, double, const double*, double*, double*,
-                                             const int*, hipStream_t, const CycleOut<double>*);
+                                             const int*, hipStream_t, const CycleOut<double>*, bool);
 template int launch_apply_zm<float, float>(const LevelOp<float>&, const ColTile*, long long, int, int,
                                            int, const float*, float, const float*, float*, double*,
-                                           const int*, hipStream_t, const CycleOut<float>*);
+                                           const int*, hipStream_t, const CycleOut<float>*, bool);
 template int launch_apply_zm<float, double>(const LevelOp<float>&, const ColTile*, long long, int, int,
                                             int, const double*, float, const float*, float*, double*,
-                                            const int*, hipStream_t, const CycleOut<float>*);
+                                            const int*, hipStream_t, const CycleOut<float>*, bool);
 
 }  // namespace gmgs

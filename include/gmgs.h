@@ -73,9 +73,14 @@ enum {
                                      on [0.22, 1.1] x lambda_max(D^-1 A), estimated per level by
                                      20 Jacobi-CG (Lanczos) steps when the BCs are set (no
                                      reference counterpart; JacobiSmoother.m uses one weight) */
-  GMGS_FLAG_ASSEMBLED_COARSE = 1024 /* coarse smoothed levels apply an assembled 27-point
+  GMGS_FLAG_ASSEMBLED_COARSE = 1024, /* coarse smoothed levels apply an assembled 27-point
                                      stencil (243 coefficients per node, set-up time) wherever it
                                      fits, instead of the z-marching or element-sum kernels */
+  GMGS_FLAG_SPLIT_COARSE = 2048   /* coarse smoothed levels with a universal element: the
+                                     z-marching kernel on the nodes whose eight elements are all
+                                     present and universal, the assembled rows on the others (one
+                                     lane per row); elsewhere as GMGS_FLAG_ASSEMBLED_COARSE.
+                                     GMGS_FLAG_AUTOTUNE also times this operator per level */
 };
 
 typedef struct gmgs_params {
@@ -134,7 +139,8 @@ enum {
   GMGS_Q_NUM_TILES = 26,            /* int64 active 16x4x1 tiles of the level */
   GMGS_Q_NUM_FREE_COARSEST = 27,    /* int64 */
   GMGS_Q_TILE_CONFIG = 28,          /* int64[3]: level kernel (0 tile kernel with element sums,
-                                       1 z-marching, 2 tile kernel with the assembled stencil),
+                                       1 z-marching, 2 tile kernel with the assembled stencil,
+                                       3 split: z-marching on F_FAST nodes + assembled rows),
                                        z-marching tile TX, z-chunk */
   GMGS_Q_SMOOTHER = 29              /* double[5] (levels < numLevels, after gmgs_set_bc):
                                        lambda_max(D^-1 A) estimate (0 = none), the weights of the

@@ -157,23 +157,32 @@ def test_vcycle(case, oracle):
     assert rel(z, zr) < 1e-9
 
 
-def test_assembled_coarse_operators(case, oracle, pkg):
-    """The assembled 27-point stencil on every smoothed coarse level: operator,
-    Jacobi step and V-cycle against the oracle's element sums."""
+@pytest.mark.parametrize("split", [False, True])
+def test_assembled_coarse_operators(case, oracle, pkg, split):
+    """The assembled 27-point stencil on every smoothed coarse level (split: the
+    z-marching kernel on the F_FAST nodes and the rows on the others, where the
+    level has a universal element): operator, Jacobi step and V-cycle against
+    the oracle's element sums."""
     m = case["m"]
+    flag = pkg.FLAG_SPLIT_COARSE if split else pkg.FLAG_ASSEMBLED_COARSE
     s = pkg.Solver(case["vox"], case["bbox"], num_levels=m.numLevels, jacobi_weight=case["omega"],
-                   tol=1e-8, max_it=2000, flags=pkg.FLAG_ASSEMBLED_COARSE)
+                   tol=1e-8, max_it=2000, flags=flag)
     s.set_bc(case["fixed"], case["loads"])
     rng = np.random.default_rng(21)
+    kinds = []
     for li in range(1, m.numLevels - 1):
-        assert s.query("TILE_CONFIG", li + 1)[0] == 2
+        kinds.append(int(s.query("TILE_CONFIG", li + 1)[0]))
+        assert kinds[-1] in ((2, 3) if split else (2,))
         x = rng.standard_normal(m.levels[li].numDOFs)
         y, ref = s.atx(x, li + 1), oracle.atx(m, x, li + 1)
-        log(f"  assembled AtX level {li + 1}: rel {rel(y, ref):.2e}")
+        log(f"  {'split' if split else 'assembled'} AtX level {li + 1} (kernel {kinds[-1]}): "
+            f"rel {rel(y, ref):.2e}")
         assert rel(y, ref) < 1e-12
         dx, r2 = s.smooth(x, li + 1)
         xr, rr = oracle.jacobi(m, 0.0, x, li + 1)
         assert rel(dx, xr) < 1e-13 and rel(r2, rr) < 1e-11
+    if split and m.numLevels > 3:
+        assert 3 in kinds   # level 2 of a hierarchy with a level 3 below it keeps a universal element
     r = rng.standard_normal(m.levels[0].numDOFs)
     assert rel(s.vcycle(r), oracle.vcycle(m, r)) < 1e-9
     s.close()
@@ -190,23 +199,26 @@ def _mixed_cycle(pkg, vox, bbox, fixed, loads, L, omega, extra):
     return z
 
 
-def test_assembled_coarse_mixed_cycle(case, pkg):
-    """FP32 cycle with assembled coarse rows against the element-sum kernels.
-    The rows are stored whole: a row stored as a difference from the universal
-    stencil cancels in FP32 on sparsely filled coarse elements."""
+@pytest.mark.parametrize("split", [False, True])
+def test_assembled_coarse_mixed_cycle(case, pkg, split):
+    """FP32 cycle with assembled coarse rows (or the split operator) against the
+    element-sum kernels.  The rows are stored whole: a row stored as a
+    difference from the universal stencil cancels in FP32 on sparsely filled
+    coarse elements."""
     a = (pkg, case["vox"], case["bbox"], case["fixed"], case["loads"], case["m"].numLevels, case["omega"])
-    za = _mixed_cycle(*a, pkg.FLAG_ASSEMBLED_COARSE)
+    za = _mixed_cycle(*a, pkg.FLAG_SPLIT_COARSE if split else pkg.FLAG_ASSEMBLED_COARSE)
     zs = _mixed_cycle(*a, pkg.FLAG_SIMPLE_KERNELS)
-    log(f"  mixed cycle assembled vs element sums: rel {rel(za, zs):.2e}")
+    log(f"  mixed cycle {'split' if split else 'assembled'} vs element sums: rel {rel(za, zs):.2e}")
     assert rel(za, zs) < 1e-4
 
 
-def test_assembled_coarse_mixed_cycle_six_levels(pkg):
+@pytest.mark.parametrize("split", [False, True])
+def test_assembled_coarse_mixed_cycle_six_levels(pkg, split):
     p = pkg.workloads.ellipsoid_problem(96, 88, 160, 6, seed=3)
     a = (pkg, p.vox, p.bbox, p.fixed, p.loads, p.num_levels, p.omega)
-    za = _mixed_cycle(*a, pkg.FLAG_ASSEMBLED_COARSE)
+    za = _mixed_cycle(*a, pkg.FLAG_SPLIT_COARSE if split else pkg.FLAG_ASSEMBLED_COARSE)
     zs = _mixed_cycle(*a, pkg.FLAG_SIMPLE_KERNELS)
-    log(f"  6-level mixed cycle assembled vs element sums: rel {rel(za, zs):.2e}")
+    log(f"  6-level mixed cycle {'split' if split else 'assembled'} vs element sums: rel {rel(za, zs):.2e}")
     assert rel(za, zs) < 1e-4
 
 
