@@ -212,8 +212,14 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
   const long long N = g.N, sx = g.py, sz = (long long)g.py * g.px;
   double dsum = 0.0;
   TileRec tr{0, 0, 0, 0};
+  int arow = 0;                                     // this lane's row (F_FAST lanes: the universal row)
+  uint32_t fl = 0;
+  TR cen[3] = {0, 0, 0};                            // the lane's own input values (SMOOTH, AX dot)
+  const int ly = lane & 15, lx = lane >> 4;
+  long long n = 0;
   if (t < ntiles) {
     tr = tiles[t];
+    arow = op.arow[t * 64 + lane];                  // with the tile record: it needs only t
     // Staging, all loads issued before the first is used.  Index 0 of every
     // padded array is a corner halo entry holding 0: a position outside the
     // grid loads from there (32-bit indices, 3 N < 2^31 checked at set-up).
@@ -257,9 +263,19 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
         de[1] = op.invd[m17];
       }
     }
+    // the lane's own flags, loaded with the staging batch
+    const int xp = tr.x0 + lx;
+    if (xp < g.px) {
+      n = (tr.y0 + ly) + sx * xp + sz * tr.z;
+      fl = op.flags[n];
+      if (MODE == AP_SMOOTH || partial)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cen[c] = in[c * N + n];
+    }
     // every loaded value is taken here, after the last load is issued: the
     // compiler may not sink a load into the branch that stores it (which
     // would put a wait between the loads)
+    asm volatile("" : "+v"(fl), "+v"(cen[0]), "+v"(cen[1]), "+v"(cen[2]));
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       asm volatile("" : "+v"(va[k]), "+v"(vb[k]));
@@ -287,15 +303,11 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
       }
     }
   }
-  __syncthreads();
-  const int ly = lane & 15, lx = lane >> 4;
-  const int yp = tr.y0 + ly, xp = tr.x0 + lx;
-  uint32_t fl = 0;
-  long long n = 0;
-  if (t < ntiles && xp < g.px) {
-    n = yp + sx * xp + sz * tr.z;
-    fl = op.flags[n];
-  }
+  // each wave stages and reads only its own tile: a wave-level barrier (LDS
+  // operations of one wave complete in order)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (fl & F_ELEM_MASK) {
     T a[3] = {0, 0, 0};
     // F_FAST nodes (all eight elements present and universal): the universal
@@ -349,7 +361,7 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
       // at a uniform base (q0 rows groups) plus one 32-bit byte offset (16
       // rows < 2^32, checked at set-up); a wave whose nodes are all F_FAST
       // skips these loads
-      const unsigned boff = (unsigned)op.arow[t * 64 + lane] * 16u;
+      const unsigned boff = (unsigned)arow * 16u;
       const char* __restrict__ base = reinterpret_cast<const char*>(op.asc);
       const long long gbytes = op.as_stride * 16;
 #pragma unroll 1
@@ -377,9 +389,9 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
       const T ac = fx ? (T)0 : a[c];          // AtX output zeroed on fixed DOFs (kernel/AtX.m:57)
       if (MODE == AP_AX) {
         out[c * N + n] = ac;
-        if (partial) dsum += (double)in[c * N + n] * (double)ac;
+        if (partial) dsum += (double)cen[c] * (double)ac;
       } else {
-        out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : (T)in[c * N + n] - ac;
+        out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : (T)cen[c] - ac;
       }
     }
   }
