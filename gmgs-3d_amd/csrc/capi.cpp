@@ -958,6 +958,15 @@ int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* par
     const TileRec* tl = l.tiles.as<TileRec>();
     const T* invd = make_op<T>(c, li, true).invd;
     launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
+    if (li == 0 && nu == 1 && l.zm) {
+      // V(1, 1): the residual kernel writes the cycle's sum itself (no Q_0,
+      // no separate sum)
+      const CycleOut<T> co{rin, nullptr, nullptr, zprev, z, (T)0};
+      nz = launch_apply_zm<T, T>(make_op<T>(c, 0, true), l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc,
+                                 AP_RESID_SUM, vec<T>(l.e), (T)l.om[0], Rbuf<T>(l, 1), nullptr, partial, done,
+                                 s, &co);
+      continue;
+    }
     apply_level<T, T>(c, li, AP_RESID, vec<T>(l.e), (T)0, Qbuf<T>(l, 0), nullptr, nullptr, done, Rbuf<T>(l, nu),
                       true);
     if (li == 0 && nu == 2 && l.zm) {

@@ -51,8 +51,10 @@ struct PcgScalars {
 // AX: out = A in; SMOOTH: out = in - A (omega in / d); RESID: out = in2 - A in;
 // SMOOTH_SUM: the smoothing sweep Q_1 = Q_0 - A (omega Q_0 / d) from in = Q_0
 // that ends the symmetric V(2, 2) cycle on level 1, writing its sum instead
-// of Q_1 (CycleOut)
-enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3 };
+// of Q_1 (CycleOut).  RESID_SUM: the residual Q_0 = in2 - A e from in = e
+// that ends the symmetric V(1, 1) cycle on level 1, writing its sum
+// z = omega (r0 + Q_0) / d + e instead of Q_0 (CycleOut without R1, e, w1)
+enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3, AP_RESID_SUM = 4 };
 
 // outputs of AP_SMOOTH_SUM: z = (omega (r0 + Q_0) + w1 (R1 + Q_1)) / d + e,
 // fixed components 0, and partials of r0.z and r0.zprev (zprev may be null)

@@ -54,8 +54,9 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   constexpr int TY = 16, NT = TY * TX, SY = TY + 2, SX = TX + 2;
   constexpr int NSTAGE = 3 * SX * SY;
   constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
-  constexpr bool SUM = MODE == AP_SMOOTH_SUM;
+  constexpr bool SUM = MODE == AP_SMOOTH_SUM, RSUM = MODE == AP_RESID_SUM;
   constexpr bool SM = MODE == AP_SMOOTH || SUM;
+  constexpr bool TWO = SUM || RSUM;                 // two dot partials (r0.z, r0.zprev)
   constexpr bool HAS_EOP = E == 1, FAST_ONLY = E == 2;
   // output nodes of this kernel
   constexpr uint32_t OUT_MASK = FAST_ONLY ? F_FAST : F_ELEM_MASK;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     // RESID: centre values of in2 at the output completed in this plane,
     // loaded now and used after the stencil
     T cen[3] = {0, 0, 0};
-    if (MODE == AP_RESID && lane_ok && (fM & OUT_MASK)) {
+    if ((MODE == AP_RESID || RSUM) && lane_ok && (fM & OUT_MASK)) {
       const long long n = colidx + sz * (zz - 1);
 #pragma unroll
       for (int c = 0; c < 3; ++c) cen[c] = in2[c * N + n];
@@ -335,6 +336,19 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
           if (partial) dsum += (double)sU[b1][c][lx + 1][ly + 1] * (double)a;
         } else if (MODE == AP_RESID) {
           out[c * N + n] = cen[c] - a;
+        } else if (RSUM) {
+          // z = omega (r0 + Q_0) / d + e, Q_0 = in2 - A e; e at the node is
+          // the staged input's centre
+          const long long m = c * N + n;
+          const double r0 = co.r0[m];
+          const bool fx = (fM & (1u << (F_FIX_SHIFT + c))) != 0;
+          T v = op.invd[m] * (omega * ((T)r0 + (cen[c] - a))) + sU[b1][c][lx + 1][ly + 1];
+          if (fx) v = 0;
+          co.z[m] = v;
+          if (partial) {
+            dsum += r0 * (double)v;
+            if (co.zprev) dsum2 += r0 * (double)co.zprev[m];
+          }
         } else if (SUM) {
           // the other terms of the cycle's sum, loaded here to keep them out
           // of the stencil's registers
@@ -377,22 +391,22 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       v += __shfl_xor(v, off);
-      if (SUM) v2 += __shfl_xor(v2, off);
+      if (TWO) v2 += __shfl_xor(v2, off);
     }
-    __shared__ double sh[SUM ? 2 : 1][NT / 64];
+    __shared__ double sh[TWO ? 2 : 1][NT / 64];
     if ((tid & 63) == 0) {
       sh[0][tid >> 6] = v;
-      if (SUM) sh[SUM ? 1 : 0][tid >> 6] = v2;
+      if (TWO) sh[TWO ? 1 : 0][tid >> 6] = v2;
     }
     __syncthreads();
     if (tid == 0) {
       double r = 0.0, r2 = 0.0;
       for (int w = 0; w < NT / 64; ++w) {
         r += sh[0][w];
-        if (SUM) r2 += sh[SUM ? 1 : 0][w];
+        if (TWO) r2 += sh[TWO ? 1 : 0][w];
       }
       partial[blockIdx.x] = r;
-      if (SUM) partial[gridDim.x + blockIdx.x] = r2;
+      if (TWO) partial[gridDim.x + blockIdx.x] = r2;
     }
   }
 }
@@ -473,8 +487,8 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("grid too large for the z-marching operator");
   const int e = fast_only ? 2 : (op.eop != nullptr ? 1 : 0);
   const CycleOut<T> cov = co ? *co : CycleOut<T>{nullptr, nullptr, nullptr, nullptr, nullptr, (T)0};
-  if (mode == AP_SMOOTH_SUM && (!co || e != 0 || !op.invd))
-    throw std::invalid_argument("SMOOTH_SUM: level 1 only, with its outputs");
+  if ((mode == AP_SMOOTH_SUM || mode == AP_RESID_SUM) && (!co || e != 0 || !op.invd))
+    throw std::invalid_argument("SMOOTH_SUM / RESID_SUM: level 1 only, with its outputs");
 #define GMGS_ZM(MODE, E, TXV)                                                                        \
   k_apply_zm<T, TR, MODE, E, TXV><<<(unsigned)ntiles, 16 * TXV, 0, s>>>(op, tiles, zc, in, omega, in2, \
                                                                         out, partial, done, cov)
@@ -497,6 +511,9 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   } else if (mode == AP_SMOOTH_SUM) {
     if constexpr (std::is_same<T, TR>::value) GMGS_ZM_TX(AP_SMOOTH_SUM, 0);
     else throw std::invalid_argument("SMOOTH_SUM: input in the cycle precision");
+  } else if (mode == AP_RESID_SUM) {
+    if constexpr (std::is_same<T, TR>::value) GMGS_ZM_TX(AP_RESID_SUM, 0);
+    else throw std::invalid_argument("RESID_SUM: input in the cycle precision");
   } else {
     GMGS_ZM_E(AP_SMOOTH);
   }
