@@ -1206,7 +1206,7 @@ double dot_host(gmgs_ctx* c, const double* a, const double* b) {
 // Chebyshev sweep weights of the symmetric cycle (no reference counterpart)
 constexpr int kLanczosSteps = 20;       // CG steps of the lambda_max estimate
 constexpr double kChebSafety = 1.1;     // upper bound = kChebSafety * estimate
-constexpr double kChebLower = 0.2;      // lower bound = kChebLower * upper bound
+constexpr double kChebLower = 0.05;     // lower bound = kChebLower * upper bound
 
 // largest eigenvalue of the symmetric tridiagonal (diag d, off-diagonal e) by
 // bisection on Sturm sequence counts

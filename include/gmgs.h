@@ -70,7 +70,7 @@ enum {
                                      (CG_solver.m:18-19) */
   GMGS_FLAG_CHEBYSHEV = 512,      /* with GMGS_FLAG_SYM_VCYCLE: the nu sweeps of a level use
                                      the reciprocal roots of the degree-nu Chebyshev polynomial
-                                     on [0.22, 1.1] x lambda_max(D^-1 A), estimated per level by
+                                     on [0.055, 1.1] x lambda_max(D^-1 A), estimated per level by
                                      20 Jacobi-CG (Lanczos) steps when the BCs are set (no
                                      reference counterpart; JacobiSmoother.m uses one weight) */
   GMGS_FLAG_ASSEMBLED_COARSE = 1024, /* coarse smoothed levels apply an assembled 27-point

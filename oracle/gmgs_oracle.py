@@ -763,7 +763,7 @@ def vcycle_sym(model: Model, r: np.ndarray, nu: int = 2, weights=None) -> np.nda
     return xs[0]
 
 
-def chebyshev_weights(lmax: float, nu: int, lower: float = 0.2, safety: float = 1.1) -> list:
+def chebyshev_weights(lmax: float, nu: int, lower: float = 0.05, safety: float = 1.1) -> list:
     """Sweep weights of a degree-nu Chebyshev smoother (no reference counterpart).
 
     nu Jacobi sweeps x += w_i r/d have the error propagator prod_i (I - w_i D^-1 A),
