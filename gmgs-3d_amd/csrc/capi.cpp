@@ -1094,9 +1094,38 @@ void finish_deferred_x(gmgs_ctx* c, hipStream_t s) {
                      c->p.as<double>(), c->sc.as<PcgScalars>(), s);
 }
 
+// Batch graphs (gmgs_batch_solve): one iteration pair of each context of a
+// list, forked from and joined into one stream, plus a gather of their done
+// flags; kept for the list until one of its contexts changes.
+struct BatchGraph {
+  hipGraphExec_t ge = nullptr;
+  DevBuf dptrs, dflags;   // the contexts' done flags (device pointers) and their gather
+};
+using BatchKey = std::vector<gmgs_ctx*>;
+
+std::map<BatchKey, BatchGraph>& batch_cache() {
+  static std::map<BatchKey, BatchGraph> m;
+  return m;
+}
+
+std::mutex& batch_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 void invalidate_graph(gmgs_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   c->gexec = nullptr;
+  std::lock_guard<std::mutex> lk(batch_mutex());
+  auto& bc = batch_cache();
+  for (auto it = bc.begin(); it != bc.end();) {
+    if (std::find(it->first.begin(), it->first.end(), c) != it->first.end()) {
+      if (it->second.ge) (void)hipGraphExecDestroy(it->second.ge);
+      it = bc.erase(it);
+    } else {
+      ++it;
+    }
+  }
 }
 
 // capture iterations k = 0, 1 once; their kernels read every scalar from
@@ -1687,6 +1716,12 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
     if (!cs[i]->bc) return fail(cs[i], GMGS_ERR_STATE, "boundary conditions not set");
     if (cs[i]->device != cs[0]->device) return fail(cs[i], GMGS_ERR_ARGUMENT, "contexts on different devices");
   }
+  {
+    std::vector<gmgs_ctx*> u(cs, cs + n);
+    std::sort(u.begin(), u.end());
+    if (std::adjacent_find(u.begin(), u.end()) != u.end())
+      return fail(cs[0], GMGS_ERR_ARGUMENT, "a context appears twice in the batch");
+  }
   gmgs_ctx* c0 = cs[0];
   return guarded(c0, [&]() -> int {
     GMGS_HIP(hipSetDevice(c0->device));
@@ -1694,41 +1729,50 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
     GMGS_HIP(hipStreamCreateWithFlags(&s0, hipStreamNonBlocking));
     std::vector<hipEvent_t> ev(n + 2);
     for (auto& e : ev) GMGS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    DevBuf dptrs, dflags;
-    std::vector<const int*> hp(n);
-    for (int i = 0; i < n; ++i) hp[i] = &cs[i]->sc.as<PcgScalars>()->done;
-    upload(dptrs, hp.data(), (size_t)n, s0);
-    dflags.alloc_zero(sizeof(int) * n, s0);
     int* pinned = nullptr;
     GMGS_HIP(hipHostMalloc((void**)&pinned, sizeof(int) * n, hipHostMallocDefault));
     int maxk = 0;
-    // initial state of every problem, then fork/join capture of one
-    // iteration pair of every problem into one graph
     for (int i = 0; i < n; ++i) {
       pcg_init(cs[i], cs[i]->prm.tol);
       GMGS_HIP(hipEventRecord(ev[i], cs[i]->stream));
       GMGS_HIP(hipStreamWaitEvent(s0, ev[i], 0));
       maxk = std::max(maxk, cs[i]->prm.max_it + 1);
     }
-    GMGS_HIP(hipStreamSynchronize(s0));
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    GMGS_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
-    GMGS_HIP(hipEventRecord(ev[n], s0));
-    for (int i = 0; i < n; ++i) {
-      gmgs_ctx* c = cs[i];
-      GMGS_HIP(hipStreamWaitEvent(c->stream, ev[n], 0));
-      for (int k = 0; k < 2; ++k) {
-        if (c->mixed) pcg_iteration<float>(c, k);
-        else pcg_iteration<double>(c, k);
+    // the graph of one iteration pair of every context of 'act' (fork/join
+    // capture), from the cache or captured now
+    auto graph_for = [&](const BatchKey& act) -> BatchGraph& {
+      std::lock_guard<std::mutex> lk(batch_mutex());
+      auto it = batch_cache().find(act);
+      if (it != batch_cache().end()) return it->second;
+      BatchGraph& bg = batch_cache()[act];
+      const int m = (int)act.size();
+      std::vector<const int*> hp(m);
+      for (int i = 0; i < m; ++i) hp[i] = &act[i]->sc.as<PcgScalars>()->done;
+      upload(bg.dptrs, hp.data(), (size_t)m, s0);
+      bg.dflags.alloc_zero(sizeof(int) * m, s0);
+      GMGS_HIP(hipStreamSynchronize(s0));
+      hipGraph_t g = nullptr;
+      GMGS_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+      GMGS_HIP(hipEventRecord(ev[n], s0));
+      for (int i = 0; i < m; ++i) {
+        gmgs_ctx* c = act[i];
+        GMGS_HIP(hipStreamWaitEvent(c->stream, ev[n], 0));
+        for (int k = 0; k < 2; ++k) {
+          if (c->mixed) pcg_iteration<float>(c, k);
+          else pcg_iteration<double>(c, k);
+        }
+        GMGS_HIP(hipEventRecord(ev[i % n], c->stream));
+        GMGS_HIP(hipStreamWaitEvent(s0, ev[i % n], 0));
       }
-      GMGS_HIP(hipEventRecord(ev[i], c->stream));
-      GMGS_HIP(hipStreamWaitEvent(s0, ev[i], 0));
-    }
-    launch_gather_flags(dptrs.as<const int*>(), n, dflags.as<int>(), s0);
-    GMGS_HIP(hipStreamEndCapture(s0, &g));
-    GMGS_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-    GMGS_HIP(hipGraphDestroy(g));
+      launch_gather_flags(bg.dptrs.as<const int*>(), m, bg.dflags.as<int>(), s0);
+      GMGS_HIP(hipStreamEndCapture(s0, &g));
+      GMGS_HIP(hipGraphInstantiate(&bg.ge, g, nullptr, nullptr, 0));
+      GMGS_HIP(hipGraphDestroy(g));
+      return bg;
+    };
+    BatchKey act(cs, cs + n);
+    BatchGraph* bg = &graph_for(act);
+    GMGS_HIP(hipStreamSynchronize(s0));
     hipEvent_t t0, t1;
     GMGS_HIP(hipEventCreate(&t0));
     GMGS_HIP(hipEventCreate(&t1));
@@ -1736,14 +1780,21 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
     int k = 0;
     int chunk = 2;
     while (k < maxk) {
-      for (int i = 0; i < chunk && k < maxk; i += 2, k += 2) GMGS_HIP(hipGraphLaunch(ge, s0));
-      GMGS_HIP(hipMemcpyAsync(pinned, dflags.as<int>(), sizeof(int) * n, hipMemcpyDeviceToHost, s0));
+      for (int i = 0; i < chunk && k < maxk; i += 2, k += 2) GMGS_HIP(hipGraphLaunch(bg->ge, s0));
+      GMGS_HIP(hipMemcpyAsync(pinned, bg->dflags.as<int>(), sizeof(int) * act.size(), hipMemcpyDeviceToHost, s0));
       GMGS_HIP(hipEventRecord(ev[n + 1], s0));
       GMGS_HIP(hipEventSynchronize(ev[n + 1]));
-      bool all = true;
-      for (int i = 0; i < n; ++i) all = all && pinned[i];
-      if (all) break;
-      chunk = std::min(chunk * 2, 16);
+      BatchKey left;
+      for (size_t i = 0; i < act.size(); ++i)
+        if (!pinned[i]) left.push_back(act[i]);
+      if (left.empty()) break;
+      // once half of the graph's problems have converged, continue on a graph
+      // of the others (their kernels would only return at once)
+      if (2 * left.size() <= act.size()) {
+        act = left;
+        bg = &graph_for(act);
+      }
+      chunk = std::min(chunk * 2, 4);
     }
     for (int i = 0; i < n; ++i) finish_deferred_x(cs[i], s0);
     GMGS_HIP(hipEventRecord(t1, s0));
@@ -1764,7 +1815,6 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
       if (st != GMGS_OK && status == GMGS_OK) status = st;
       if (U && U[i]) copy_solution(c, U[i]);
     }
-    GMGS_HIP(hipGraphExecDestroy(ge));
     GMGS_HIP(hipEventDestroy(t0));
     GMGS_HIP(hipEventDestroy(t1));
     for (auto& e : ev) GMGS_HIP(hipEventDestroy(e));

@@ -160,11 +160,14 @@ int gmgs_stress(gmgs_ctx* c, const double* U, double* S);
 int gmgs_adjoint(gmgs_ctx* c, const double* g, const double* U, double* lambda, double* dJdE,
                  gmgs_solve_info* info);
 
-/* Batched solve (no reference counterpart): solves the problems of n contexts
- * (same device, BCs set) together.  One captured graph holds a PCG iteration
- * pair of every problem as parallel branches and is replayed until every
- * problem's device-side convergence flag is set; U[i] (may be NULL) and
- * infos[i] receive each problem's result. */
+/* Batched solve (no reference counterpart): solves the problems of n distinct
+ * contexts (same device, BCs set) together.  One captured graph holds a PCG
+ * iteration pair of every problem as parallel branches and is replayed until
+ * every problem's device-side convergence flag is set; once half of a graph's
+ * problems have converged the replays continue on a graph of the others.  The
+ * graphs are kept for the same list of contexts until one of them changes
+ * (set_bc, set_modulus, destroy).  U[i] (may be NULL) and infos[i] receive
+ * each problem's result. */
 int gmgs_batch_solve(gmgs_ctx** ctxs, int n, double** U, gmgs_solve_info* infos);
 
 int gmgs_atx(gmgs_ctx* c, int level, const double* x, double* y);
