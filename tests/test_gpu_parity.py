@@ -406,7 +406,7 @@ def cheb_solver(case, pkg, precision, nu, nu_coarse=0):
     return s
 
 
-@pytest.mark.parametrize("nu,nu_coarse", [(1, 0), (1, 4), (2, 0), (2, 3), (2, 4)])
+@pytest.mark.parametrize("nu,nu_coarse", [(1, 0), (1, 3), (1, 4), (2, 0), (2, 3), (2, 4)])
 def test_chebyshev_bounds_weights_and_cycle(case, oracle, pkg, nu, nu_coarse):
     m = case["m"]
     s2 = cheb_solver(case, pkg, pkg.GMGS_PREC_FP64, nu, nu_coarse)
