@@ -60,8 +60,10 @@ struct Level {
   long long as_stride = 0;
   bool as_built = false, as = false;
   // split operator: the z-marching kernel on the F_FAST nodes and the
-  // assembled rows on the others (nfnode: padded index of row r's node)
-  DevBuf nfnode;
+  // assembled rows on the others, over the tiles that hold them (mtiles, with
+  // their rows per tile and lane in marow)
+  DevBuf mtiles, marow;
+  long long nmtiles = 0;
   bool split_ok = false, split = false;
   int col_tx = 8, col_zc = 16;
   // level 1 of a mixed context: the FP64 matvec's own column tiles (the
@@ -239,8 +241,10 @@ int apply_level(gmgs_ctx* c, int li, int mode, const TR* in, T omega, T* out, TR
   if (l.split) {
     int np = launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc, mode, in, omega,
                                     in2, out, partial, done, c->stream, nullptr, true);
-    np += launch_apply_rows<T, TR>(op, l.nfnode.as<int>(), l.as_stride - 1, mode, in, omega, in2, out,
-                                   partial ? partial + np : nullptr, done, c->stream);
+    LevelOp<T> opr = op;
+    opr.arow = l.marow.as<int>();
+    np += launch_apply_as_rows<T, TR>(opr, l.mtiles.as<TileRec>(), l.nmtiles, mode, in, omega, in2, out,
+                                      partial ? partial + np : nullptr, done, c->stream);
     if (xacc)
       launch_jacobi_acc<T, TR>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles,
                                op.invd, in, omega, xacc, c->stream);
@@ -517,16 +521,19 @@ void build_operators(gmgs_ctx* c) {
     // the universal row last; used where the z-marching kernel is not
     l.as = l.as_built = false;
     l.split = l.split_ok = false;
-    l.nfnode.release();
+    l.mtiles.release();
+    l.marow.release();
+    l.nmtiles = 0;
     l.asc64.release();
     l.asc32.release();
     l.arow.release();
     l.as_stride = 0;
     long long rows = 0;
-    std::vector<int> ar, nodes;
+    std::vector<int> ar;
+    std::vector<TileRec> th;
     if (li >= 1 && li < L - 1 && !c->simple_kernels && l.ntiles > 0) {
       const Grid& g = l.mesh.g;
-      const std::vector<TileRec> th = download<TileRec>(l.tiles, (size_t)l.ntiles, s);
+      th = download<TileRec>(l.tiles, (size_t)l.ntiles, s);
       const std::vector<uint32_t> fh = download<uint32_t>(l.mesh.flags, (size_t)g.N, s);
       ar.assign((size_t)l.ntiles * 64, -1);
       for (long long t = 0; t < l.ntiles; ++t)
@@ -534,10 +541,7 @@ void build_operators(gmgs_ctx* c) {
           const int yp = th[t].y0 + (lane & 15), xp = th[t].x0 + (lane >> 4);
           if (xp >= g.px) continue;
           const uint32_t f = fh[yp + (long long)g.py * (xp + (long long)g.px * th[t].z)];
-          if ((f & F_ELEM_MASK) && !(f & F_FAST)) {
-            ar[t * 64 + lane] = (int)rows++;
-            nodes.push_back((int)(yp + (long long)g.py * (xp + (long long)g.px * th[t].z)));
-          }
+          if ((f & F_ELEM_MASK) && !(f & F_FAST)) ar[t * 64 + lane] = (int)rows++;
         }
     }
     const long long stride = rows + 1;
@@ -559,11 +563,25 @@ void build_operators(gmgs_ctx* c) {
         l.asc64.alloc_zero(sizeof(double) * stencil_group_elems(stride, sizeof(double)), s);
         launch_relayout_stencil<double>(canon.as<double>(), stride, l.asc64.as<double>(), s);
       }
-      if (!nodes.empty()) upload(l.nfnode, nodes.data(), nodes.size(), s);
+      // tiles holding a row of their own (not F_FAST), and those rows
+      std::vector<TileRec> mt;
+      std::vector<int> mr;
+      for (long long t = 0; t < l.ntiles; ++t) {
+        bool any = false;
+        for (int lane = 0; lane < 64; ++lane) any = any || ar[t * 64 + lane] != (int)(stride - 1);
+        if (!any) continue;
+        mt.push_back(th[t]);
+        mr.insert(mr.end(), ar.begin() + t * 64, ar.begin() + (t + 1) * 64);
+      }
+      l.nmtiles = (long long)mt.size();
+      if (!mt.empty()) {
+        upload(l.mtiles, mt.data(), mt.size(), s);
+        upload(l.marow, mr.data(), mr.size(), s);
+      }
       GMGS_HIP(hipStreamSynchronize(s));
       l.as_built = true;
       // the split operator needs the z-marching kernel's tiles
-      l.split_ok = l.zm && !nodes.empty();
+      l.split_ok = l.zm && l.nmtiles > 0;
       l.split = l.split_ok && c->prefer_split;
       l.as = !l.split && (!l.zm || c->prefer_as || c->prefer_split);
       if (c->prefer_as || c->prefer_split) l.zm = false;
@@ -659,8 +677,10 @@ float time_split(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx
   auto run = [&]() {
     launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, nullptr, out, nullptr, nullptr, s,
                            nullptr, true);
-    launch_apply_rows<T, TR>(op, l.nfnode.as<int>(), l.as_stride - 1, AP_SMOOTH, in, (T)0.5, nullptr, out,
-                             nullptr, nullptr, s);
+    LevelOp<T> opr = op;
+    opr.arow = l.marow.as<int>();
+    launch_apply_as_rows<T, TR>(opr, l.mtiles.as<TileRec>(), l.nmtiles, AP_SMOOTH, in, (T)0.5, nullptr, out,
+                                nullptr, nullptr, s);
   };
   hipEvent_t a, b;
   GMGS_HIP(hipEventCreate(&a));
@@ -685,7 +705,7 @@ long long partial_capacity(gmgs_ctx* c) {
   long long n = c->nparts;
   for (const Level& l : c->lev) {
     n = std::max<long long>(n, std::max<long long>(apply_blocks(l.ntiles), std::max(l.ncol, l.ncol64)));
-    if (l.split_ok) n = std::max<long long>(n, l.ncol + apply_rows_blocks(l.as_stride - 1));
+    if (l.split_ok) n = std::max<long long>(n, l.ncol + (long long)apply_blocks(l.nmtiles));
   }
   return n;
 }

@@ -184,19 +184,13 @@ void launch_xpby(const Grid& g, const uint32_t* flags, const TileRec* tiles, lon
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                 const double* a, const double* b, double* partial, hipStream_t s);
 
-// rows per wave of launch_apply_rows (three lanes per row), and its blocks
-// of four waves (one dot partial each)
-constexpr int kRowsPerWave = 21;
-inline long long apply_rows_blocks(long long nrows) {
-  return (nrows + 4 * kRowsPerWave - 1) / (4 * kRowsPerWave);
-}
-
-// the assembled rows of the non-F_FAST solid nodes (nodes[r] = padded index of
-// row r's node) applied to those nodes: AX, SMOOTH or RESID as launch_apply;
-// returns the number of dot partials written (AX with partial)
+// the assembled rows on the solid nodes that are not F_FAST, over the tiles
+// that hold such nodes (op.arow: their rows, per tile and lane): AX, SMOOTH
+// or RESID (out = in2 - A in) as launch_apply; returns the number of dot
+// partials written (one per block of four tiles)
 template <typename T, typename TR>
-int launch_apply_rows(const LevelOp<T>& op, const int* nodes, long long nrows, int mode, const TR* in,
-                      T omega, const T* in2, T* out, double* partial, const int* done, hipStream_t s);
+int launch_apply_as_rows(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode, const TR* in,
+                         T omega, const T* in2, T* out, double* partial, const int* done, hipStream_t s);
 
 // start of a solve from x0 = 0: x = p = z0 = z1 = 0, r = b, partials of b.b
 template <typename TZ>

@@ -197,10 +197,13 @@ struct alignas(16) CoefGroup {
   T v[16 / sizeof(T)];
 };
 
-template <typename T, typename TR, int MODE>
+// ROWS_ONLY: only the solid nodes that are not F_FAST are computed and written
+// (the split operator's second half, over the tiles that hold such nodes)
+template <typename T, typename TR, int MODE, bool ROWS_ONLY>
 __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* __restrict__ tiles,
                                                   long long ntiles, const TR* __restrict__ in, T omega,
-                                                  T* __restrict__ out, double* __restrict__ partial,
+                                                  const T* __restrict__ in2, T* __restrict__ out,
+                                                  double* __restrict__ partial,
                                                   const int* __restrict__ done) {
   if (done && *done) return;
   constexpr int SY = 18, SX = 6, NS = SY * SX * 3;   // staged nodes per tile
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
   TileRec tr{0, 0, 0, 0};
   int arow = 0;                                     // this lane's row (F_FAST lanes: the universal row)
   uint32_t fl = 0;
-  TR cen[3] = {0, 0, 0};                            // the lane's own input values (SMOOTH, AX dot)
+  TR cen[3] = {0, 0, 0};                            // the lane's own in (SMOOTH, AX dot) or in2 (RESID)
   const int ly = lane & 15, lx = lane >> 4;
   long long n = 0;
   if (t < ntiles) {
@@ -268,9 +271,13 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
     if (xp < g.px) {
       n = (tr.y0 + ly) + sx * xp + sz * tr.z;
       fl = op.flags[n];
-      if (MODE == AP_SMOOTH || partial)
+      if (MODE == AP_RESID) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cen[c] = (TR)in2[c * N + n];
+      } else if (MODE == AP_SMOOTH || partial) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) cen[c] = in[c * N + n];
+      }
     }
     // every loaded value is taken here, after the last load is issued: the
     // compiler may not sink a load into the branch that stores it (which
@@ -308,14 +315,14 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (fl & F_ELEM_MASK) {
+  if ((fl & F_ELEM_MASK) && !(ROWS_ONLY && (fl & F_FAST))) {
     T a[3] = {0, 0, 0};
     // F_FAST nodes (all eight elements present and universal): the universal
     // stencil from its 30 reflection-class coefficients, read as scalar
     // operands, over the three staged planes (input plane z - 1 + zz, so
     // oz = zz - 1 is the input's z offset from the output)
     const bool ucls = op.cls != nullptr;
-    if (ucls && (fl & F_FAST)) {
+    if (!ROWS_ONLY && ucls && (fl & F_FAST)) {
       const cptr<T> C = opaque(op.cls);
       T cf[30];
 #pragma unroll
@@ -390,102 +397,10 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
       if (MODE == AP_AX) {
         out[c * N + n] = ac;
         if (partial) dsum += (double)cen[c] * (double)ac;
+      } else if (MODE == AP_RESID) {
+        out[c * N + n] = (T)cen[c] - ac;
       } else {
         out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : (T)cen[c] - ac;
-      }
-    }
-  }
-  if (partial) {
-    const double b = block_sum(dsum);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
-  }
-}
-
-// The assembled rows applied to their own nodes (the non-F_FAST solid nodes
-// of a level, listed in row order): the second half of the split level
-// operator, whose F_FAST nodes the z-marching kernel computes from the
-// universal stencil.  Three lanes share a row, lane kz taking the input plane
-// z - 1 + kz: its 9 x 3 neighbour values read from the input directly (void
-// and halo entries are zero; a solid node is never on the grid's outer
-// layer) and the row's groups of that plane (16-byte groups, consecutive rows
-// consecutive); the three partial sums are added in the order kz = 0, 1, 2 by
-// the row's first lane, which writes the output.  A wave takes 21 rows.
-// 32-bit indices: 3 N < 2^31.
-
-template <typename T, typename TR, int MODE>
-__global__ __launch_bounds__(256) void k_apply_rows(LevelOp<T> op, const int* __restrict__ nodes,
-                                                    long long nrows, const TR* __restrict__ in, T omega,
-                                                    const T* __restrict__ in2, T* __restrict__ out,
-                                                    double* __restrict__ partial,
-                                                    const int* __restrict__ done) {
-  if (done && *done) return;
-  constexpr int G = 16 / (int)sizeof(T), NGP = stencil_plane_groups((int)sizeof(T));
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int sub = lane / 3, kz = lane - 3 * sub;
-  const long long r = ((long long)blockIdx.x * 4 + wave) * kRowsPerWave + sub;
-  const bool ok = lane < 3 * kRowsPerWave && r < nrows;
-  const int iN = (int)op.g.N, sx = op.g.py, sz = op.g.py * op.g.px;
-  int n = 0;
-  T a[3] = {0, 0, 0};
-  uint32_t fl = 0;
-  T cen[3] = {0, 0, 0};                              // the output's centre term (in, or in2 for RESID)
-  if (ok) {
-    n = nodes[r];
-    if (kz == 0) {                                   // loaded with the others, used after the sums
-      fl = op.flags[n];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (MODE == AP_RESID) cen[c] = in2[c * iN + n];
-        else if (MODE == AP_SMOOTH) cen[c] = (T)in[c * iN + n];
-      }
-    }
-    T u[27];                                         // neighbour (dy, dx) = (j % 3 - 1, j / 3 - 1)
-    const int m0 = n + sz * (kz - 1);
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-      const int m = m0 + (j % 3 - 1) + sx * (j / 3 - 1);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        T v = (T)in[d * iN + m];
-        if (MODE == AP_SMOOTH) v = omega * op.invd[d * iN + m] * v;
-        u[3 * j + d] = v;
-      }
-    }
-    const char* __restrict__ base = reinterpret_cast<const char*>(op.asc) + (long long)kz * NGP * op.as_stride * 16;
-    const long long gbytes = op.as_stride * 16;
-#pragma unroll
-    for (int g0 = 0; g0 < NGP; ++g0) {
-      const CoefGroup<T> cg = *reinterpret_cast<const CoefGroup<T>*>(base + (long long)g0 * gbytes + r * 16);
-#pragma unroll
-      for (int k = 0; k < G; ++k) {
-        const int q = g0 * G + k;                     // coefficient within the z-plane
-        if (q < 81) {
-          const int j = q / 9, cc = (q / 3) % 3, d = q % 3;
-          a[cc] += cg.v[k] * u[3 * j + d];
-        }
-      }
-    }
-  }
-  // lanes 3 sub + 1, 3 sub + 2 hand their sums to lane 3 sub
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T a1 = __shfl_down(a[c], 1), a2 = __shfl_down(a[c], 2);
-    a[c] = (a[c] + a1) + a2;
-  }
-  double dsum = 0.0;
-  if (ok && kz == 0) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int m = c * iN + n;
-      const bool fx = (fl & (1u << (F_FIX_SHIFT + c))) != 0;
-      const T ac = fx ? (T)0 : a[c];          // AtX output zeroed on fixed DOFs (kernel/AtX.m:57)
-      if (MODE == AP_AX) {
-        out[m] = ac;
-        if (partial) dsum += (double)in[m] * (double)ac;
-      } else if (MODE == AP_RESID) {
-        out[m] = cen[c] - ac;
-      } else {
-        out[m] = (fx && op.zero_fixed) ? (T)0 : cen[c] - ac;
       }
     }
   }
@@ -1172,9 +1087,11 @@ void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, 
   const bool eop = op.eop != nullptr, emod = op.emod != nullptr;
   if (op.asc && !xacc && mode != AP_RESID) {
     if (mode == AP_AX)
-      k_apply_as<T, TR, AP_AX><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, partial, done);
+      k_apply_as<T, TR, AP_AX, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, nullptr, out, partial,
+                                                          done);
     else
-      k_apply_as<T, TR, AP_SMOOTH><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, partial, done);
+      k_apply_as<T, TR, AP_SMOOTH, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, nullptr, out,
+                                                              partial, done);
     GMGS_LAUNCH_CHECK();
     return;
   }
@@ -1194,30 +1111,31 @@ void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, 
 }
 
 template <typename T, typename TR>
-int launch_apply_rows(const LevelOp<T>& op, const int* nodes, long long nrows, int mode, const TR* in,
-                      T omega, const T* in2, T* out, double* partial, const int* done, hipStream_t s) {
-  if (nrows <= 0) return 0;
-  if (!op.asc || 3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("apply_rows: no assembled rows");
-  const unsigned nb = (unsigned)apply_rows_blocks(nrows);
+int launch_apply_as_rows(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode, const TR* in,
+                         T omega, const T* in2, T* out, double* partial, const int* done, hipStream_t s) {
+  if (ntiles <= 0) return 0;
+  if (!op.asc || !op.arow) throw std::invalid_argument("apply_as_rows: no assembled rows");
+  const unsigned nb = apply_blocks(ntiles);
   if (mode == AP_AX)
-    k_apply_rows<T, TR, AP_AX><<<nb, 256, 0, s>>>(op, nodes, nrows, in, omega, in2, out, partial, done);
+    k_apply_as<T, TR, AP_AX, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, in2, out, partial, done);
   else if (mode == AP_RESID)
-    k_apply_rows<T, TR, AP_RESID><<<nb, 256, 0, s>>>(op, nodes, nrows, in, omega, in2, out, partial, done);
+    k_apply_as<T, TR, AP_RESID, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, in2, out, partial, done);
   else if (mode == AP_SMOOTH)
-    k_apply_rows<T, TR, AP_SMOOTH><<<nb, 256, 0, s>>>(op, nodes, nrows, in, omega, in2, out, partial, done);
+    k_apply_as<T, TR, AP_SMOOTH, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, in2, out, partial, done);
   else
-    throw std::invalid_argument("apply_rows: mode");
+    throw std::invalid_argument("apply_as_rows: mode");
   GMGS_LAUNCH_CHECK();
   return (int)nb;
 }
-template int launch_apply_rows<double, double>(const LevelOp<double>&, const int*, long long, int,
-                                               const double*, double, const double*, double*, double*,
-                                               const int*, hipStream_t);
-template int launch_apply_rows<float, float>(const LevelOp<float>&, const int*, long long, int, const float*,
-                                             float, const float*, float*, double*, const int*, hipStream_t);
-template int launch_apply_rows<float, double>(const LevelOp<float>&, const int*, long long, int,
-                                              const double*, float, const float*, float*, double*,
-                                              const int*, hipStream_t);
+template int launch_apply_as_rows<double, double>(const LevelOp<double>&, const TileRec*, long long, int,
+                                                  const double*, double, const double*, double*, double*,
+                                                  const int*, hipStream_t);
+template int launch_apply_as_rows<float, float>(const LevelOp<float>&, const TileRec*, long long, int,
+                                                const float*, float, const float*, float*, double*, const int*,
+                                                hipStream_t);
+template int launch_apply_as_rows<float, double>(const LevelOp<float>&, const TileRec*, long long, int,
+                                                 const double*, float, const float*, float*, double*,
+                                                 const int*, hipStream_t);
 
 template <typename T>
 void launch_relayout_stencil(const double* src, long long rows, T* dst, hipStream_t s) {

@@ -22,7 +22,7 @@ void build_colrange(const Grid& g, const uint32_t* flags, DevBuf& colz, hipStrea
 // SMOOTH_SUM mode (co: its outputs; partials r0.z at [0, n) and r0.zprev at
 // [n, 2 n)); returns the number n of dot partials written (one per tile).
 // fast_only: only the F_FAST nodes (universal stencil, no element
-// corrections) are computed and written; launch_apply_rows does the rest
+// corrections) are computed and written; launch_apply_as_rows does the rest
 template <typename T, typename TR>
 int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles, int tx, int zc,
                     int mode, const TR* in, T omega, const T* in2, T* out, double* partial,
