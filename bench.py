@@ -103,7 +103,7 @@ def run_batch(args, pkg):
     for p in probs:
         s = pkg.Solver(p.vox, p.bbox, num_levels=p.num_levels, jacobi_weight=p.omega, tol=1e-6,
                        max_it=args.max_it, precision=pkg.GMGS_PREC_MIXED, flags=flags,
-                       vcycle_sweeps=1, vcycle_sweeps_coarse=3)
+                       vcycle_sweeps=1, vcycle_sweeps_level2=2, vcycle_sweeps_coarse=4)
         s.set_bc(p.fixed, p.loads)
         solvers.append(s)
     setup_s = time.perf_counter() - t0
@@ -144,7 +144,7 @@ def run_batch(args, pkg):
         "data": "synthetic (64 seeded uneven ellipsoid-union solids)",
         "config": {"workload": f"batch64s{args.seed}", "problems": 64, "dofs": total,
                    "dofs_min": int(min(dofs)), "dofs_max": int(max(dofs)), "rtol": 1e-6,
-                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, 3 below), "
+                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, 2 on level 2, 4 below), "
                              "exact Galerkin coarse operators, no Jacobi step, FP64 flexible PCG"),
                    "sequential_ms": round(ms_seq, 3),
                    "speedup_vs_sequential": round(ms_seq / ms_batch, 3),
@@ -196,10 +196,11 @@ def main():
     base_flags = pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE
     sym_flags = base_flags | pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP
     cheb_flags = sym_flags | pkg.FLAG_CHEBYSHEV | pkg.FLAG_EXACT_GALERKIN
-    cheb_sweeps = dict(vcycle_sweeps=1, vcycle_sweeps_coarse=3)
+    cheb_sweeps = dict(vcycle_sweeps=1, vcycle_sweeps_level2=2, vcycle_sweeps_coarse=4)
 
     # headline: FP32 symmetric cycle with Chebyshev sweeps (degree 1 on level 1,
-    # 3 below) on exact Galerkin coarse operators, inside the FP64 flexible PCG
+    # 2 on level 2, 4 below) on exact Galerkin coarse operators, inside the FP64
+    # flexible PCG
     t0 = time.perf_counter()
     s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, flags=cheb_flags,
                    **cheb_sweeps, **common)
@@ -295,7 +296,7 @@ def main():
         "data": "synthetic (seeded ellipsoid-union voxel solid)",
         "config": {"workload": prob.name, "dofs": ndofs, "levels": s.num_levels,
                    "rtol": 1e-6,
-                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, 3 below), "
+                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, 2 on level 2, 4 below), "
                              "exact Galerkin coarse operators, no Jacobi step, FP64 flexible PCG"),
                    "fp64": "reference algorithm: kernel/Vcycle.m V-cycle, Jacobi step, FP64 PCG",
                    "its_mixed": info.its, "converged_mixed": bool(info.converged),

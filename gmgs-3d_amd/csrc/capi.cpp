@@ -99,6 +99,7 @@ struct gmgs_ctx {
   bool autotune = false;
   bool sym = false;              // symmetric V(nu, nu) cycle
   int sweeps = 2, sweeps_coarse = 2;   // nu of level 1, of the coarser smoothed levels
+  int sweeps_l2 = 2;                   // nu of level 2
   bool cheb = false;             // Chebyshev sweep weights in the symmetric cycle
   bool prefer_as = false;        // assembled stencil on every coarse level where it fits
   bool prefer_split = false;     // split operator wherever it exists, the assembled stencil elsewhere
@@ -1533,7 +1534,8 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
         !(p->tol > 0) || p->max_it < 0 || !(p->jacobi_weight > 0) || p->num_levels < 0 ||
         (p->precision != GMGS_PREC_FP64 && p->precision != GMGS_PREC_MIXED) ||
         p->vcycle_sweeps < 0 || p->vcycle_sweeps > kMaxSweeps || p->vcycle_sweeps_coarse < 0 ||
-        p->vcycle_sweeps_coarse > kMaxSweeps)
+        p->vcycle_sweeps_coarse > kMaxSweeps || p->vcycle_sweeps_level2 < 0 ||
+        p->vcycle_sweeps_level2 > kMaxSweeps)
       throw Status(GMGS_ERR_ARGUMENT, "invalid solver parameter");
     c->device = p->device;
     GMGS_HIP(hipSetDevice(c->device));
@@ -1551,6 +1553,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->sym = (p->flags & GMGS_FLAG_SYM_VCYCLE) != 0;
     c->sweeps = p->vcycle_sweeps == 0 ? 2 : p->vcycle_sweeps;
     c->sweeps_coarse = p->vcycle_sweeps_coarse == 0 ? c->sweeps : p->vcycle_sweeps_coarse;
+    c->sweeps_l2 = p->vcycle_sweeps_level2 == 0 ? c->sweeps_coarse : p->vcycle_sweeps_level2;
     c->no_jacobi = (p->flags & GMGS_FLAG_NO_JACOBI_STEP) != 0;
     c->cheb = (p->flags & GMGS_FLAG_CHEBYSHEV) != 0;
     c->prefer_as = (p->flags & GMGS_FLAG_ASSEMBLED_COARSE) != 0;
@@ -1571,7 +1574,8 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     for (int d = 0; d < 3; ++d)
       if (!(c->bbox[3 + d] > c->bbox[d])) throw Status(GMGS_ERR_ARGUMENT, "empty bounding box");
     build_mesh(c.get(), vox);
-    for (int li = 0; li < c->L; ++li) c->lev[li].nu = li == 0 ? c->sweeps : c->sweeps_coarse;
+    for (int li = 0; li < c->L; ++li)
+      c->lev[li].nu = li == 0 ? c->sweeps : (li == 1 ? c->sweeps_l2 : c->sweeps_coarse);
     double probe[8][3];
     probe_coords(c.get(), probe);
     element_stiffness(p->youngs_modulus, p->poisson_ratio, probe, c->Ke0);

@@ -69,7 +69,8 @@ class Params(C.Structure):
                 ("jacobi_weight", C.c_double), ("coarsest_resolution_control", C.c_double),
                 ("max_it", C.c_int32), ("num_levels", C.c_int32), ("precision", C.c_int32),
                 ("flags", C.c_int32), ("device", C.c_int32), ("vcycle_sweeps", C.c_int32),
-                ("vcycle_sweeps_coarse", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("vcycle_sweeps_coarse", C.c_int32), ("vcycle_sweeps_level2", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class SolveInfo(C.Structure):

@@ -96,7 +96,8 @@ typedef struct gmgs_params {
   int32_t device;                      /* HIP device ordinal */
   int32_t vcycle_sweeps;               /* nu of GMGS_FLAG_SYM_VCYCLE on level 1: 1..4 (0 = 2) */
   int32_t vcycle_sweeps_coarse;        /* nu on levels 2..numLevels-1: 1..4 (0 = vcycle_sweeps) */
-  int32_t reserved[5];
+  int32_t vcycle_sweeps_level2;        /* nu on level 2 alone: 1..4 (0 = vcycle_sweeps_coarse) */
+  int32_t reserved[4];
 } gmgs_params;
 
 typedef struct gmgs_solve_info {

@@ -16,7 +16,7 @@ def _solvers(pkg, probs, cycle):
         else:   # the symmetric Chebyshev cycle of the benchmark, V(1,1) on level 1
             kw = dict(flags=pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP
                       | pkg.FLAG_CHEBYSHEV | pkg.FLAG_EXACT_GALERKIN,
-                      vcycle_sweeps=1, vcycle_sweeps_coarse=3)
+                      vcycle_sweeps=1, vcycle_sweeps_level2=2, vcycle_sweeps_coarse=4)
         s = pkg.Solver(p.vox, p.bbox, num_levels=p.num_levels, tol=1e-6, max_it=3000,
                        precision=pkg.GMGS_PREC_MIXED, **kw)
         s.set_bc(p.fixed, p.loads)

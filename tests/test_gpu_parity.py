@@ -396,20 +396,24 @@ def true_lmax(m, oracle, ii):
     return float(sla.eigsh(op, k=1, which="LA", tol=1e-10, return_eigenvectors=False)[0])
 
 
-def cheb_solver(case, pkg, precision, nu, nu_coarse=0):
+def cheb_solver(case, pkg, precision, nu, nu_coarse=0, nu2=0):
     m = case["m"]
     s = pkg.Solver(case["vox"], case["bbox"], num_levels=m.numLevels, jacobi_weight=case["omega"],
                    tol=1e-8, max_it=2000, precision=precision, vcycle_sweeps=nu,
-                   vcycle_sweeps_coarse=nu_coarse,
+                   vcycle_sweeps_coarse=nu_coarse, vcycle_sweeps_level2=nu2,
                    flags=pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP | pkg.FLAG_CHEBYSHEV)
     s.set_bc(case["fixed"], case["loads"])
     return s
 
 
-@pytest.mark.parametrize("nu,nu_coarse", [(1, 0), (1, 3), (1, 4), (2, 0), (2, 3), (2, 4)])
-def test_chebyshev_bounds_weights_and_cycle(case, oracle, pkg, nu, nu_coarse):
+@pytest.mark.parametrize("nu,nu_coarse,nu2", [(1, 0, 0), (1, 3, 0), (1, 4, 2), (2, 0, 0), (2, 3, 0),
+                                               (2, 4, 0)])
+def test_chebyshev_bounds_weights_and_cycle(case, oracle, pkg, nu, nu_coarse, nu2):
     m = case["m"]
-    s2 = cheb_solver(case, pkg, pkg.GMGS_PREC_FP64, nu, nu_coarse)
+    s2 = cheb_solver(case, pkg, pkg.GMGS_PREC_FP64, nu, nu_coarse, nu2)
+    # Chebyshev degree per level: level 1, level 2 (nu2, 0 = as below), levels 3..
+    deg = {ii: nu if ii == 1 else (nu2 if ii == 2 and nu2 else (nu_coarse or nu))
+           for ii in range(1, m.numLevels)}
     w, lm64 = {}, {}
     for ii in range(1, m.numLevels):
         lm, ws = s2.smoother(ii)
@@ -418,14 +422,14 @@ def test_chebyshev_bounds_weights_and_cycle(case, oracle, pkg, nu, nu_coarse):
         log(f"  level {ii}: lanczos {lm:.6f} true {lt:.6f} weights {ws}")
         # a Ritz value: never above lambda_max, and close to it after 20 steps
         assert lm <= lt * (1 + 1e-8) and lm >= 0.95 * lt
-        ref = oracle.chebyshev_weights(lm, nu if ii == 1 else (nu_coarse or nu))
+        ref = oracle.chebyshev_weights(lm, deg[ii])
         assert len(ws) == len(ref)
         assert all(abs(a - b) <= 1e-14 * b for a, b in zip(ws, ref))
         w[ii] = ref
     rng = np.random.default_rng(13)
     r = rng.standard_normal(m.levels[0].numDOFs)
     z, zr = s2.vcycle(r), oracle.vcycle_sym(m, r, weights=w)
-    log(f"  chebyshev vcycle nu={nu}/{nu_coarse} rel {rel(z, zr):.2e}")
+    log(f"  chebyshev vcycle nu={nu}/{nu2}/{nu_coarse} rel {rel(z, zr):.2e}")
     assert rel(z, zr) < 1e-9
     fx = m.levels[0].fixedDOFs - 1
     a, b = rng.standard_normal(r.size), rng.standard_normal(r.size)
@@ -435,17 +439,17 @@ def test_chebyshev_bounds_weights_and_cycle(case, oracle, pkg, nu, nu_coarse):
     assert abs(Ma @ b - a @ Mb) <= 1e-12 * abs(Ma @ b)
     s2.close()
     # the mixed-precision context estimates in FP32 from the same start vector
-    s3 = cheb_solver(case, pkg, pkg.GMGS_PREC_MIXED, nu, nu_coarse)
+    s3 = cheb_solver(case, pkg, pkg.GMGS_PREC_MIXED, nu, nu_coarse, nu2)
     for ii in range(1, m.numLevels):
         lm3 = s3.smoother(ii)[0]
         assert abs(lm3 - lm64[ii]) <= 1e-3 * lm64[ii]
     s3.close()
 
 
-@pytest.mark.parametrize("nu_coarse", [0, 3])
-def test_chebyshev_cycle_solves(case, oracle, pkg, nu_coarse):
+@pytest.mark.parametrize("nu,nu_coarse,nu2", [(2, 0, 0), (2, 3, 0), (1, 4, 2)])
+def test_chebyshev_cycle_solves(case, oracle, pkg, nu, nu_coarse, nu2):
     m = case["m"]
-    s2 = cheb_solver(case, pkg, pkg.GMGS_PREC_FP64, 2, nu_coarse)
+    s2 = cheb_solver(case, pkg, pkg.GMGS_PREC_FP64, nu, nu_coarse, nu2)
     w = {ii: s2.smoother(ii)[1] for ii in range(1, m.numLevels)}
     U, info, _ = s2.solve(raise_on_nonconvergence=False)
     res = oracle.cg_solver(m, m.F, 1e-8, 2000, jacobi_step=False,
@@ -453,12 +457,13 @@ def test_chebyshev_cycle_solves(case, oracle, pkg, nu_coarse):
     jac = oracle.cg_solver(m, m.F, 1e-8, 2000, jacobi_step=False,
                            cycle=lambda mm, r: oracle.vcycle_sym(mm, r, 2))
     e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
-    log(f"  chebyshev fp64 nu_coarse={nu_coarse}: gpu its {info.its} oracle its {res.its} "
+    log(f"  chebyshev fp64 nu={nu}/{nu2}/{nu_coarse}: gpu its {info.its} oracle its {res.its} "
         f"(jacobi sweeps {jac.its}) err {e:.2e}")
     assert abs(info.its - res.its) <= 1 and e < 1e-6
-    assert res.its <= jac.its
+    if nu == 2:
+        assert res.its <= jac.its    # against V(2,2) Jacobi sweeps
     s2.close()
-    s3 = cheb_solver(case, pkg, pkg.GMGS_PREC_MIXED, 2, nu_coarse)
+    s3 = cheb_solver(case, pkg, pkg.GMGS_PREC_MIXED, nu, nu_coarse, nu2)
     U3, info3, _ = s3.solve(raise_on_nonconvergence=False)
     ref = direct_solution(m, oracle)
     e3 = np.linalg.norm(U3 - ref) / np.linalg.norm(ref)
