@@ -238,11 +238,16 @@ void launch_copy_probe(const double* a, double* b, long long n_doubles, hipStrea
 void launch_gather_flags(const int* const* flags, int n, int* out, hipStream_t s);
 
 inline unsigned apply_blocks(long long ntiles) { return (unsigned)((ntiles + 3) / 4); }
-// grid of the pointwise kernels (update_p, update_xr, dot, prolong_smooth):
-// also their number of dot partials
+// grid of the pointwise kernels that reduce (update_xr, dot, the start and
+// exit norms, prolong_smooth and combine with dot partials): also their
+// number of dot partials.  The others take one wave per tile (apply_blocks),
+// no slower and faster for combine (level 2: 61 vs 89 us).
 inline unsigned pointwise_blocks(long long ntiles) {
   const unsigned b = apply_blocks(ntiles);
   return b < 2048u ? b : 2048u;
+}
+inline unsigned grid_for(long long ntiles, const double* partial) {
+  return partial ? pointwise_blocks(ntiles) : apply_blocks(ntiles);
 }
 
 }  // namespace gmgs

@@ -604,20 +604,25 @@ __global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restr
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
     const long long n = ln_.n;
+    // the sums of all three components before the first store
+    T v[3];
+    TR r0v[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long m = c * N + n;
-      T acc = w0 * ((T)r0[m] + q0[m]);
+      r0v[c] = r0[m];
+      T acc = w0 * ((T)r0v[c] + q0[m]);
       if (NU > 1) acc += w1 * (r1[m] + q1[m]);
       if (NU > 2) acc += w2 * (r2[m] + q2[m]);
       if (NU > 3) acc += w3 * (r3[m] + q3[m]);
-      T v = invd[m] * acc + e[m];
-      if (fix && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) v = 0;
-      x[m] = (TZ)v;
-      if (partial) {
-        d0 += (double)r0[m] * (double)(TZ)v;
-        if (zprev) d1 += (double)r0[m] * (double)zprev[m];
-      }
+      v[c] = invd[m] * acc + e[m];
+      if (fix && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) v[c] = 0;
+      if (partial && zprev) d1 += (double)r0v[c] * (double)zprev[m];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[c * N + n] = (TZ)v[c];
+      if (partial) d0 += (double)r0v[c] * (double)(TZ)v[c];
     }
   }
   if (partial) {
@@ -640,8 +645,14 @@ __global__ __launch_bounds__(256) void k_rsub(Grid g, const uint32_t* __restrict
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
+    T av[3], ov[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[c * N + ln_.n] = a[c * N + ln_.n] - out[c * N + ln_.n];
+    for (int c = 0; c < 3; ++c) {
+      av[c] = a[c * N + ln_.n];
+      ov[c] = out[c * N + ln_.n];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c * N + ln_.n] = av[c] - ov[c];
   }
 }
 
@@ -658,11 +669,20 @@ __global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __rest
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
     const long long n = ln_.n;
+    // every load before the first store (the components share the arrays, so
+    // the compiler could not move a load above an earlier component's store)
+    double po[3], xo[3] = {0, 0, 0};
+    TZ zv[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double po = p[c * N + n];
-      if (x) x[c * N + n] += alpha * po;   // the previous iteration's x update
-      p[c * N + n] = (double)z[c * N + n] + beta * po;
+      po[c] = p[c * N + n];
+      zv[c] = z[c * N + n];
+      if (x) xo[c] = x[c * N + n];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (x) x[c * N + n] = xo[c] + alpha * po[c];   // the previous iteration's x update
+      p[c * N + n] = (double)zv[c] + beta * po[c];
     }
   }
 }
@@ -677,8 +697,14 @@ __global__ __launch_bounds__(256) void k_add_alpha_p(Grid g, const uint32_t* __r
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
     const long long n = ln_.n;
+    double xo[3], po[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) x[c * N + n] += alpha * p[c * N + n];
+    for (int c = 0; c < 3; ++c) {
+      xo[c] = x[c * N + n];
+      po[c] = p[c * N + n];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c * N + n] = xo[c] + alpha * po[c];
   }
 }
 
@@ -701,15 +727,23 @@ __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __res
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
     const long long N = g.N, n = ln_.n;
+    // every load before the first store (r_a may be r itself)
+    double rav[3], qv[3], xv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, uv[3] = {0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      if (x && invd) {
-        const double u = omega * invd[c * N + n] * r[c * N + n];
-        x[c * N + n] = (x[c * N + n] + u) + alpha * p[c * N + n];
-      } else if (x) {
-        x[c * N + n] += alpha * p[c * N + n];
+      rav[c] = ra[c * N + n];
+      qv[c] = q[c * N + n];
+      if (x) {
+        xv[c] = x[c * N + n];
+        pv[c] = p[c * N + n];
+        if (invd) uv[c] = omega * invd[c * N + n] * r[c * N + n];
       }
-      const double rv = ra[c * N + n] - alpha * q[c * N + n];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (x && invd) x[c * N + n] = (xv[c] + uv[c]) + alpha * pv[c];
+      else if (x) x[c * N + n] = xv[c] + alpha * pv[c];
+      const double rv = rav[c] - alpha * qv[c];
       r[c * N + n] = rv;
       d += rv * rv;
     }
@@ -1172,7 +1206,7 @@ void launch_prolong_smooth(const Grid& gf, const uint32_t* flags_f, const TileRe
                            T omega, TZ* x, bool fix, const TZ* zprev, double* partial,
                            const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_prolong_smooth<T, TR, TZ><<<pointwise_blocks(nt), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, invd, r,
+  k_prolong_smooth<T, TR, TZ><<<grid_for(nt, partial), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, invd, r,
                                                                rp, omega, x, fix ? 1 : 0, zprev,
                                                                partial, done);
   GMGS_LAUNCH_CHECK();
@@ -1182,7 +1216,7 @@ template <typename T>
 void launch_rsub(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, const T* a,
                  T* out, const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_rsub<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, out, done);
+  k_rsub<T><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, out, done);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_rsub<double>(const Grid&, const uint32_t*, const TileRec*, long long, const double*,
@@ -1194,7 +1228,7 @@ template <typename T>
 void launch_prolong(const Grid& gf, const uint32_t* flags_f, const TileRec* tiles, long long nt,
                     const Grid& gc, const T* xc, T* e, bool mask, const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_prolong<T><<<pointwise_blocks(nt), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, e, mask ? 1 : 0, done);
+  k_prolong<T><<<apply_blocks(nt), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, e, mask ? 1 : 0, done);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_prolong<double>(const Grid&, const uint32_t*, const TileRec*, long long, const Grid&,
@@ -1206,7 +1240,7 @@ template <typename T>
 void launch_mask_fixed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                        const T* src, T* dst, hipStream_t s) {
   if (nt <= 0) return;
-  k_mask_fixed<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, src, dst);
+  k_mask_fixed<T><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, src, dst);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_mask_fixed<double>(const Grid&, const uint32_t*, const TileRec*, long long,
@@ -1219,7 +1253,7 @@ void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, 
                     const T* invd, const CycleSum<T>& cs, const TR* r0, const T* e, TZ* x, bool fix,
                     const TZ* zprev, double* partial, const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  const unsigned nb = pointwise_blocks(nt);
+  const unsigned nb = grid_for(nt, partial);
   const int f = fix ? 1 : 0;
 #define GMGS_COMBINE(NU)                                                                            \
   k_combine<T, TR, TZ, NU><<<nb, 256, 0, s>>>(g, flags, tiles, nt, invd, cs.w[0], cs.w[1], cs.w[2],  \
@@ -1248,14 +1282,14 @@ template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                      const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s,
                      double* x) {
-  k_update_p<TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, p, x, sc, done);
+  k_update_p<TZ><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, p, x, sc, done);
   GMGS_LAUNCH_CHECK();
 }
 
 void launch_add_alpha_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, double* x,
                         const double* p, const PcgScalars* sc, hipStream_t s) {
   if (nt <= 0) return;
-  k_add_alpha_p<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, sc);
+  k_add_alpha_p<<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, sc);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1272,7 +1306,7 @@ template <typename T, typename TR>
 void launch_jacobi_acc(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                        const T* invd, const TR* r, T omega, TR* x, hipStream_t s) {
   if (nt <= 0) return;
-  k_jacobi_acc<T, TR><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, r, omega, x);
+  k_jacobi_acc<T, TR><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, r, omega, x);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_jacobi_acc<double, double>(const Grid&, const uint32_t*, const TileRec*, long long,
@@ -1287,7 +1321,7 @@ template <typename T>
 void launch_seed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, T* v,
                  hipStream_t s) {
   if (nt <= 0) return;
-  k_seed<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, v);
+  k_seed<T><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, v);
   GMGS_LAUNCH_CHECK();
 }
 template <typename T>
@@ -1302,7 +1336,7 @@ template <typename T>
 void launch_xpby(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, const T* z,
                  double beta, T* p, hipStream_t s) {
   if (nt <= 0) return;
-  k_xpby<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, beta, p);
+  k_xpby<T><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, beta, p);
   GMGS_LAUNCH_CHECK();
 }
 #define GMGS_INST_LANCZOS(T)                                                                          \
