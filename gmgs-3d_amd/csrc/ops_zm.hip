@@ -35,6 +35,14 @@ namespace {
 template <typename T>
 using cptr = const __attribute__((address_space(4))) T*;
 
+// LDS written by some lanes of a wave and read by others of the same wave:
+// LDS operations of one wave complete in order
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
 __device__ inline cptr<T> opaque(const T* p) {
   cptr<T> q = (cptr<T>)p;
@@ -66,6 +74,10 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   // output completion, element corrections)
   __shared__ T sU[4][3][SX][SY];
   __shared__ T sR[SM ? 2 : 1][3][TX][TY];            // SMOOTH: centre values of r
+  // per wave: the element corrections' task list and results
+  __shared__ int sTask[FAST_ONLY ? 1 : NT / 64][64];
+  __shared__ T sRes[FAST_ONLY ? 1 : NT / 64][3][64];
+  const int wv = (int)(threadIdx.x >> 6);
   // XCD-contiguous tile order: blocks b = 8 j + k (k = XCD) of the first
   // 8 floor(n / 8) blocks take tile k floor(n / 8) + j; the rest keep their own
   const int nb = (int)gridDim.x, q8 = nb >> 3, bid = (int)blockIdx.x;
@@ -194,40 +206,43 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     if (!FAST_ONLY && (fM & F_ELEM_MASK)) cm = ((~fM) & F_ELEM_MASK) | ((fM >> F_STORED_SHIFT) & 0xFFu);
     if (!FAST_ONLY && __ballot(cm != 0u)) {
       const int cnt = __popc(cm);
-      int off = cnt;                                   // inclusive prefix over the wave
+      // exclusive prefix of the pair counts over the wave and their total,
+      // from one ballot per element bit (no chain of cross-lane moves)
+      int off = 0, total = 0;
 #pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(off, d);
-        if (lane >= d) off += v;
+      for (int b = 0; b < 8; ++b) {
+        const unsigned long long mb = __ballot((cm >> b) & 1u);
+        off += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
+        total += __popcll(mb);
       }
-      const int total = __shfl(off, 63);
-      off -= cnt;
-      int mcnt = cnt;
-#pragma unroll
-      for (int d = 32; d > 0; d >>= 1) mcnt = max(mcnt, __shfl_xor(mcnt, d));
       T D0 = 0, D1 = 0, D2 = 0;
       for (int rb = 0; rb < total; rb += 64) {
-        const int j = rb + lane;
-        int own = 0;                                   // last lane whose first pair is <= j
-#pragma unroll
-        for (int st = 32; st > 0; st >>= 1) {
-          const int o = __shfl(off, own + st);
-          if (o <= j) own += st;
+        // each lane lists its pairs of this round in the wave's task slots:
+        // owner lane, element present (stored) and element index
+        {
+          uint32_t mm = cm;
+          int idx = off;
+          while (mm) {
+            const int p = __ffs(mm) - 1;
+            mm &= mm - 1;
+            if (idx >= rb && idx < rb + 64)
+              sTask[wv][idx - rb] = (lane << 4) | ((int)((fM >> p) & 1u) << 3) | p;
+            ++idx;
+          }
         }
-        const uint32_t cmo = __shfl(cm, own);
-        const uint32_t fo = __shfl(fM, own);
-        const int offo = __shfl(off, own);
+        wave_lds_sync();
+        const int j = rb + lane;
         T e0 = 0, e1 = 0, e2 = 0;
         if (j < total) {
-          uint32_t mm = cmo;
-          for (int q = j - offo; q > 0; --q) mm &= mm - 1;
-          const int p = __ffs(mm) - 1;
+          const int task = sTask[wv][lane];
+          const int own = task >> 4, p = task & 7;
           const int cn = p ^ 7;                        // corner code of the node in element p
           const T sgx = (cn & 2) ? (T)-1 : (T)1, sgy = (cn & 1) ? (T)-1 : (T)1,
                   sgz = (cn & 4) ? (T)-1 : (T)1;
           const int oly = own % TY + 1, olx = lx_w + own / TY + 1;   // staged coords of the node
           const T* Kst = nullptr;
-          if (HAS_EOP && ((fo >> p) & 1u)) {
+          if (HAS_EOP && ((task >> 3) & 1)) {
             const int pyb = p & 1, pxb = (p >> 1) & 1, pzb = p >> 2;
             const long long no = (t.y0 + oly - 1) + sx * (t.x0 + olx - 1) + sz * (zz - 1);
             const long long cell = no - (1 - pyb) - (1 - pxb) * sx - (1 - pzb) * sz;
@@ -259,16 +274,19 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
           e1 = sgy * e1 - h1;
           e2 = sgz * e2 - h2;
         }
+        sRes[wv][0][lane] = e0;
+        sRes[wv][1][lane] = e1;
+        sRes[wv][2][lane] = e2;
+        wave_lds_sync();
         // owners collect this round's pairs, in order
-        for (int q = 0; q < mcnt; ++q) {
-          const int idx = off + q, src = (idx - rb) & 63;
-          const T v0 = __shfl(e0, src), v1 = __shfl(e1, src), v2 = __shfl(e2, src);
-          if (q < cnt && idx >= rb && idx < rb + 64) {
-            D0 += v0;
-            D1 += v1;
-            D2 += v2;
+        for (int q = 0, idx = off; q < cnt; ++q, ++idx) {
+          if (idx >= rb && idx < rb + 64) {
+            D0 += sRes[wv][0][idx - rb];
+            D1 += sRes[wv][1][idx - rb];
+            D2 += sRes[wv][2][idx - rb];
           }
         }
+        wave_lds_sync();   // the slots are written again by the next round
       }
       aM[0] -= D0;
       aM[1] -= D1;
