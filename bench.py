@@ -46,14 +46,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(name: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    --pmc summary (line 'traffic_bytes_per_launch: N'), or None."""
+def read_traffic(name: str, tx: int, zc: int):
+    """HBM bytes per launch of the dominant kernel on (tx, zc) column tiles from
+    the committed rocprofv3 --pmc summary (line 'traffic_bytes_per_launch
+    tx=TX zc=ZC: N'), or None."""
     f = REPO / "profiles" / f"{name}_traffic.txt"
     if not f.exists():
         return None
+    key = f"traffic_bytes_per_launch tx={tx} zc={zc}:"
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return float(line.split(":", 1)[1])
     return None
 
@@ -261,7 +263,11 @@ def main():
     achieved = alg_bytes / (mv_ms * 1e-3) / 1e9
     log(f"[bench] matvec64 {mv_ms:.4f} ms ({achieved:.1f} GB/s), jacobi64 {jac_ms:.4f} ms "
         f"({jac_bytes / (jac_ms * 1e-3) / 1e9:.1f} GB/s)")
-    traffic = read_traffic(f"matvec64_{prob.name}")
+    # the autotuner's column tile of the FP64 matvec (it picks by time; the
+    # HBM traffic differs between tile shapes)
+    tcfg = s.query("TILE_CONFIG", 1)
+    tx64, zc64 = int(tcfg[3]), int(tcfg[4])
+    traffic = read_traffic(f"matvec64_{prob.name}", tx64, zc64)
 
     # per-kernel breakdown from one instrumented solve (events around each
     # level-1 launch, graphs off)
@@ -308,7 +314,7 @@ def main():
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
+                     "traffic": traffic, "tile_tx_zc": [tx64, zc64], "alg_bytes_per_launch": alg_bytes,
                      "avg_launch_ms": round(mv_ms, 5), "launches": args.matvec_reps},
         "roofline_jacobi64": {"kernel": "k_apply_zm<f64,SMOOTH> (level-1 Jacobi step)",
                               "achieved": round(jac_bytes / (jac_ms * 1e-3) / 1e9, 1),

@@ -711,6 +711,11 @@ long long partial_capacity(gmgs_ctx* c) {
   return n;
 }
 
+// a candidate replaces the best so far only when it is 2% faster: shapes
+// within timing noise of each other resolve to the earlier one in the list,
+// so repeated set-ups of one problem pick the same shapes
+inline bool clearly_faster(float ms, float best_ms) { return ms < 0.98f * best_ms; }
+
 void autotune_levels(gmgs_ctx* c) {
   static const int cand[][2] = {{8, 16}, {4, 16}, {16, 16}, {8, 8}, {4, 8}, {16, 8}, {8, 4},
                                 {4, 4}, {16, 4}, {8, 32}, {4, 32}};
@@ -744,13 +749,13 @@ void autotune_levels(gmgs_ctx* c) {
             if (c->mixed) ms = time_zm<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(), l.rp.as<float>());
             else ms = time_zm<double, double>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<double>(), l.rp.as<double>());
           }
-          if (ms < best_ms) { best_ms = ms; best = {cd[0], cd[1]}; }
+          if (clearly_faster(ms, best_ms)) { best_ms = ms; best = {cd[0], cd[1]}; }
           if (li > 0 && l.split_ok) {
             ms = c->mixed ? time_split<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(),
                                                      l.rp.as<float>())
                           : time_split<double, double>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1],
                                                        l.r.as<double>(), l.rp.as<double>());
-            if (ms < best_ms) { best_ms = ms; best = {-cd[0], cd[1]}; }
+            if (clearly_faster(ms, best_ms)) { best_ms = ms; best = {-cd[0], cd[1]}; }
           }
         }
       }
@@ -760,7 +765,7 @@ void autotune_levels(gmgs_ctx* c) {
         for (int as = 0; as <= (l.as_built ? 1 : 0); ++as) {
           const float ms = c->mixed ? time_v1<float, float>(c, li, l.r.as<float>(), l.rp.as<float>(), as)
                                     : time_v1<double, double>(c, li, l.r.as<double>(), l.rp.as<double>(), as);
-          if (ms < best_ms) { best_ms = ms; best = {0, as}; }
+          if (clearly_faster(ms, best_ms)) { best_ms = ms; best = {0, as}; }
         }
       }
       std::lock_guard<std::mutex> lk(tune_mutex());
@@ -801,7 +806,7 @@ void autotune_levels(gmgs_ctx* c) {
         const long long nt = build_coltiles(l1.mesh.g, l1.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
         const float ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
                                                  c->q.as<double>(), AP_AX);
-        if (ms < best_ms) { best_ms = ms; best = {cd[0], cd[1]}; }
+        if (clearly_faster(ms, best_ms)) { best_ms = ms; best = {cd[0], cd[1]}; }
       }
       std::lock_guard<std::mutex> lk(tune_mutex());
       tune_cache()[key] = best;
@@ -2098,7 +2103,7 @@ int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count) {
       case GMGS_Q_INTERMEDIATE_NUM_NODES: case GMGS_Q_STORING_STATE: case GMGS_Q_NUM_TILES:
       case GMGS_Q_NUM_FREE_COARSEST:
         n = 1; break;
-      case GMGS_Q_TILE_CONFIG: n = 3; break;
+      case GMGS_Q_TILE_CONFIG: n = 5; break;
       case GMGS_Q_RES: n = 3; break;
       case GMGS_Q_ELE_MAP_BACK: n = l->mesh.ne; break;
       case GMGS_Q_ELE_MAP_FORWARD: n = (int64_t)g->ey * g->ex * g->ez; break;
@@ -2163,7 +2168,11 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
       case GMGS_Q_NUM_TILES: i64[0] = l->ntiles; break;
       case GMGS_Q_NUM_FREE_COARSEST: i64[0] = c->nfree; break;
       case GMGS_Q_TILE_CONFIG:
-        i64[0] = l->split ? 3 : (l->zm ? 1 : (l->as ? 2 : 0)); i64[1] = l->col_tx; i64[2] = l->col_zc; break;
+        i64[0] = l->split ? 3 : (l->zm ? 1 : (l->as ? 2 : 0)); i64[1] = l->col_tx; i64[2] = l->col_zc;
+        // the FP64 matvec of the PCG runs on column tiles of its own when tuned
+        i64[3] = l->coltiles64.empty() ? l->col_tx : l->col_tx64;
+        i64[4] = l->coltiles64.empty() ? l->col_zc : l->col_zc64;
+        break;
       case GMGS_Q_ELE_MAP_BACK: copy_dev(l->mesh.eleMapBack, 4 * need); break;
       case GMGS_Q_ELE_MAP_FORWARD: copy_dev(l->mesh.eleMapForward, 4 * need); break;
       case GMGS_Q_NOD_MAP_BACK: copy_dev(l->mesh.nodMapBack, 4 * need); break;

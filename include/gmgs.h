@@ -139,10 +139,12 @@ enum {
   GMGS_Q_F = 25,                    /* double[numDOFs] load vector F_ */
   GMGS_Q_NUM_TILES = 26,            /* int64 active 16x4x1 tiles of the level */
   GMGS_Q_NUM_FREE_COARSEST = 27,    /* int64 */
-  GMGS_Q_TILE_CONFIG = 28,          /* int64[3]: level kernel (0 tile kernel with element sums,
+  GMGS_Q_TILE_CONFIG = 28,          /* int64[5]: level kernel (0 tile kernel with element sums,
                                        1 z-marching, 2 tile kernel with the assembled stencil,
                                        3 split: z-marching on F_FAST nodes + assembled rows),
-                                       z-marching tile TX, z-chunk */
+                                       z-marching tile TX, z-chunk, then TX and z-chunk of the
+                                       FP64 matvec of the PCG (level 1 of a mixed context tunes
+                                       its own; the level's otherwise) */
   GMGS_Q_SMOOTHER = 29              /* double[5] (levels < numLevels, after gmgs_set_bc):
                                        lambda_max(D^-1 A) estimate (0 = none), the weights of the
                                        level's nu sweeps, 0 beyond nu */
