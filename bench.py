@@ -252,22 +252,28 @@ def main():
                "mixed_jacobi_sweeps_ms": round(msmj, 3), "its_mixed_jacobi_sweeps": infomj.its,
                "mixed_vs_fp64_rel_l2": err}
 
-    # the level-1 FP64 operators, back-to-back launches between one event pair
+    # the level-1 FP64 operators, back-to-back launches between one event pair:
+    # the solver's fused direction update + matvec (its largest FP64 kernel),
+    # the plain matvec and the reference algorithm's Jacobi step
+    mvp_ms = s.time_operator("matvec64_p", args.matvec_reps)
     mv_ms = s.time_operator("matvec64", args.matvec_reps)
     jac_ms = s.time_operator("jacobi64", args.matvec_reps)
-    # algorithmic bytes: q = A p reads p and writes q once, plus one 32-bit
-    # flag word per node; the Jacobi step reads r and 1/diag and writes r_a
-    # (its x += omega r/d rides in the update of x and r)
+    # algorithmic bytes per solid node, one 32-bit flag word each: the fused
+    # kernel reads p_old, z (FP32) and x and writes q, p and x (3 x (8 + 4 +
+    # 8) + 4 read, 3 x 3 x 8 written); q = A p reads p and writes q once; the
+    # Jacobi step reads r and 1/diag and writes r_a
+    alg_bytes_p = (3 * (8 + 4 + 8) + 4 + 3 * 3 * 8) * nn
     alg_bytes = 2 * 8 * ndofs + 4 * nn
     jac_bytes = 3 * 8 * ndofs + 4 * nn
+    achieved_p = alg_bytes_p / (mvp_ms * 1e-3) / 1e9
     achieved = alg_bytes / (mv_ms * 1e-3) / 1e9
-    log(f"[bench] matvec64 {mv_ms:.4f} ms ({achieved:.1f} GB/s), jacobi64 {jac_ms:.4f} ms "
-        f"({jac_bytes / (jac_ms * 1e-3) / 1e9:.1f} GB/s)")
-    # the autotuner's column tile of the FP64 matvec (it picks by time; the
+    log(f"[bench] matvec64_p {mvp_ms:.4f} ms ({achieved_p:.1f} GB/s), matvec64 {mv_ms:.4f} ms "
+        f"({achieved:.1f} GB/s), jacobi64 {jac_ms:.4f} ms ({jac_bytes / (jac_ms * 1e-3) / 1e9:.1f} GB/s)")
+    # the autotuner's column tile of the fused kernel (it picks by time; the
     # HBM traffic differs between tile shapes)
     tcfg = s.query("TILE_CONFIG", 1)
     tx64, zc64 = int(tcfg[3]), int(tcfg[4])
-    traffic = read_traffic(f"matvec64_{prob.name}", tx64, zc64)
+    traffic = read_traffic(f"matvec64p_{prob.name}", tx64, zc64)
 
     # per-kernel breakdown from one instrumented solve (events around each
     # level-1 launch, graphs off)
@@ -310,12 +316,16 @@ def main():
                    **(cmp or {"comparisons": "skipped (--headline-only)"}),
                    "smoother_lmax": [round(s.smoother(li)[0], 5) for li in range(1, s.num_levels)],
                    "setup_s": round(setup_s, 3)},
-        "roofline": {"kernel": "k_apply_zm<f64,AX> (level-1 matvec q = A p)", "bound": "hbm",
-                     "achieved": round(achieved, 1),
+        "roofline": {"kernel": ("k_apply_zm<f64,AX_P> (level-1 direction update and matvec: "
+                                "p = z + beta p_old, q = A p, x += alpha p_old)"), "bound": "hbm",
+                     "achieved": round(achieved_p, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "tile_tx_zc": [tx64, zc64], "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_ms": round(mv_ms, 5), "launches": args.matvec_reps},
+                     "frac": round(achieved_p / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "tile_tx_zc": [tx64, zc64], "alg_bytes_per_launch": alg_bytes_p,
+                     "avg_launch_ms": round(mvp_ms, 5), "launches": args.matvec_reps},
+        "roofline_matvec64": {"kernel": "k_apply_zm<f64,AX> (level-1 matvec q = A p)",
+                              "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                              "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(mv_ms, 5)},
         "roofline_jacobi64": {"kernel": "k_apply_zm<f64,SMOOTH> (level-1 Jacobi step)",
                               "achieved": round(jac_bytes / (jac_ms * 1e-3) / 1e9, 1),
                               "frac": round(jac_bytes / (jac_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

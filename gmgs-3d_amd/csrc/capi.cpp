@@ -118,7 +118,7 @@ struct gmgs_ctx {
   DevBuf free_code, dofmap, A, W, Kinv, gv_b, gv_part;
   long long nfree = 0, nfp = 0;
   // PCG state
-  DevBuf x, r, ra, z[2], p, q, tmp, tmp2, part, sc, hist_rel, hist_t;
+  DevBuf x, r, ra, z[2], p, p2, q, tmp, tmp2, part, sc, hist_rel, hist_t;   // p2: see fused_direction
   long long nparts = 0;
   bool have_solution = false;
   std::string err;
@@ -142,6 +142,11 @@ struct gmgs_ctx {
 namespace {
 
 Level& L1(gmgs_ctx* c) { return c->lev[0]; }
+
+// the mixed solver without the Jacobi step folds p = z + beta p and the
+// deferred x update into its level-1 FP64 matvec when level 1 runs the
+// z-marching operator (p then alternates between p and p2)
+bool fused_direction(gmgs_ctx* c) { return c->mixed && c->no_jacobi && !c->p2.empty() && L1(c).zm; }
 
 hipEvent_t kt_event(gmgs_ctx* c) {
   if (c->kpool_used == c->kpool.size()) {
@@ -641,7 +646,7 @@ float time_v1(gmgs_ctx* c, int li, const TR* in, T* out, bool as = false) {
 
 template <typename T, typename TR>
 float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, int zc, const TR* in,
-              T* out, int mode = AP_SMOOTH) {
+              T* out, int mode = AP_SMOOTH, const CycleOut<T>* co = nullptr) {
   Level& l = c->lev[li];
   hipStream_t s = c->stream;
   LevelOp<T> op = make_op<T>(c, li);
@@ -649,11 +654,11 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
   GMGS_HIP(hipEventCreate(&a));
   GMGS_HIP(hipEventCreate(&b));
   for (int w = 0; w < 2; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, nullptr, out, nullptr, nullptr, s);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, nullptr, out, nullptr, nullptr, s, co);
   GMGS_HIP(hipEventRecord(a, s));
   const int reps = 5;
   for (int w = 0; w < reps; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, nullptr, out, nullptr, nullptr, s);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, nullptr, out, nullptr, nullptr, s, co);
   GMGS_HIP(hipEventRecord(b, s));
   GMGS_HIP(hipEventSynchronize(b));
   float ms = 0;
@@ -791,7 +796,15 @@ void autotune_levels(gmgs_ctx* c) {
   // level 1 of a mixed context: the FP64 matvec of the PCG on tiles of its own
   Level& l1 = L1(c);
   if (c->mixed && l1.zm) {
-    const TuneKey key{l1.mesh.g.ey, l1.mesh.g.ex, l1.mesh.g.ez, 2};
+    // the kernel the PCG runs: the matvec, or the matvec with the direction
+    // update fused (timed on scratch vectors: p_old = tmp, p = tmp2, x = ra)
+    const bool fused = fused_direction(c);
+    CycleOut<double> co{};
+    co.zf = c->z[0].as<float>();
+    co.pn = c->tmp2.as<double>();
+    co.x = c->ra.as<double>();
+    co.sc = c->sc.as<PcgScalars>();
+    const TuneKey key{l1.mesh.g.ey, l1.mesh.g.ex, l1.mesh.g.ez, fused ? 3 : 2};
     std::pair<int, int> best{l1.col_tx, l1.col_zc};
     bool cached = false;
     {
@@ -805,7 +818,8 @@ void autotune_levels(gmgs_ctx* c) {
         DevBuf tl;
         const long long nt = build_coltiles(l1.mesh.g, l1.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
         const float ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
-                                                 c->q.as<double>(), AP_AX);
+                                                 c->q.as<double>(), fused ? AP_AX_P : AP_AX,
+                                                 fused ? &co : nullptr);
         if (clearly_faster(ms, best_ms)) { best_ms = ms; best = {cd[0], cd[1]}; }
       }
       std::lock_guard<std::mutex> lk(tune_mutex());
@@ -848,6 +862,7 @@ void alloc_vectors(gmgs_ctx* c) {
   c->r.alloc_zero(sizeof(double) * n3, s);
   c->ra.alloc_zero(sizeof(double) * n3, s);
   c->p.alloc_zero(sizeof(double) * n3, s);
+  if (c->mixed && c->no_jacobi) c->p2.alloc_zero(sizeof(double) * n3, s);
   c->q.alloc_zero(sizeof(double) * n3, s);
   c->tmp.alloc_zero(sizeof(double) * n3, s);
   c->tmp2.alloc_zero(sizeof(double) * n3, s);
@@ -1031,8 +1046,27 @@ int run_vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* par
 }
 
 // ------------------------------------------------------------------ PCG
+// p = z + beta p_old, q = A p (into c->q), x += alpha p_old and the p'q
+// partials in one level-1 z-march; returns the number of partials
+int launch_matvec_p(gmgs_ctx* c, const double* p_old, double* p, double* x, const float* z, double* part,
+                    const int* done) {
+  Level& l = L1(c);
+  CycleOut<double> co{};
+  co.zf = z;
+  co.pn = p;
+  co.x = x;
+  co.sc = c->sc.as<PcgScalars>();
+  const LevelOp<double> op = make_op<double>(c, 0);
+  const bool own = !l.coltiles64.empty();
+  return launch_apply_zm<double, double>(op, own ? l.coltiles64.as<ColTile>() : l.coltiles.as<ColTile>(),
+                                         own ? l.ncol64 : l.ncol, own ? l.col_tx64 : l.col_tx,
+                                         own ? l.col_zc64 : l.col_zc, AP_AX_P, p_old, 0.0, nullptr,
+                                         c->q.as<double>(), part, done, c->stream, &co);
+}
+
 template <typename T>
 void pcg_iteration(gmgs_ctx* c, int k) {
+  static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value, "cycle precision");
   hipStream_t s = c->stream;
   Level& l = L1(c);
   const Grid& g = l.mesh.g;
@@ -1058,21 +1092,39 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   const T* zprev = c->flexible ? c->z[(k + 1) & 1].as<T>() : nullptr;
   const int nz = run_vcycle<T>(c, ra, z, zprev, part, done);
   launch_finalize(1, part, nz, sc, nullptr, nullptr, done, s);
+  int npq;
+  double* p = c->p.as<double>();
+  if constexpr (std::is_same<T, float>::value) {
+    if (fused_direction(c)) {
+      // p = z + beta p_old, q = A p, p'q and the previous iteration's
+      // x += alpha p_old in one z-march (CG_solver.m:23-31); p alternates
+      // between two buffers since tiles read their neighbours' p_old
+      p = (k & 1) ? c->p2.as<double>() : c->p.as<double>();
+      const double* p_old = (k & 1) ? c->p.as<double>() : c->p2.as<double>();
+      {
+        KTime t(c, GMGS_K_MATVEC64_P);
+        npq = launch_matvec_p(c, p_old, p, c->x.as<double>(), z, part, done);
+      }
+      launch_finalize(2, part, npq, sc, nullptr, nullptr, done, s);
+      launch_update_xr(g, fl, tl, l.ntiles, nullptr, p, ra, c->q.as<double>(), c->r.as<double>(), nullptr,
+                       c->prm.jacobi_weight, sc, part, done, s);
+      launch_finalize(3, part, np, sc, c->hist_rel.as<double>(), c->hist_t.as<unsigned long long>(),
+                      done, s);
+      return;
+    }
+  }
   // p = z + beta p (CG_solver.m:23-28)
   // without the Jacobi step, x += alpha p of the previous iteration rides in
   // this pass over p (read once per iteration); the last one follows the loop
-  launch_update_p<T>(g, fl, tl, l.ntiles, z, c->p.as<double>(), sc, done, s,
-                     c->no_jacobi ? c->x.as<double>() : nullptr);
+  launch_update_p<T>(g, fl, tl, l.ntiles, z, p, sc, done, s, c->no_jacobi ? c->x.as<double>() : nullptr);
   // q = A p, p'q (CG_solver.m:29-31)
-  int npq;
   {
     KTime t(c, 0);
-    npq = apply_level<double, double>(c, 0, AP_AX, c->p.as<double>(), 0.0, c->q.as<double>(), nullptr,
-                                      part, done);
+    npq = apply_level<double, double>(c, 0, AP_AX, p, 0.0, c->q.as<double>(), nullptr, part, done);
   }
   launch_finalize(2, part, npq, sc, nullptr, nullptr, done, s);
   // x += alpha p; r = r - alpha q; ||r|| (CG_solver.m:32-39)
-  launch_update_xr(g, fl, tl, l.ntiles, c->no_jacobi ? nullptr : c->x.as<double>(), c->p.as<double>(), ra,
+  launch_update_xr(g, fl, tl, l.ntiles, c->no_jacobi ? nullptr : c->x.as<double>(), p, ra,
                    c->q.as<double>(),
                    c->r.as<double>(), c->no_jacobi ? nullptr : l.invd64.as<double>(),
                    c->prm.jacobi_weight, sc, part, done, s);
@@ -1096,11 +1148,11 @@ void pcg_init(gmgs_ctx* c, double tol) {
   const TileRec* tl = l.tiles.as<TileRec>();
   if (c->mixed)
     launch_pcg_start<float>(g, fl, tl, l.ntiles, c->b.as<double>(), c->x.as<double>(), c->p.as<double>(),
-                            c->r.as<double>(), c->z[0].as<float>(), c->z[1].as<float>(),
+                            c->p2.empty() ? nullptr : c->p2.as<double>(), c->r.as<double>(), c->z[0].as<float>(), c->z[1].as<float>(),
                             c->part.as<double>(), s);
   else
     launch_pcg_start<double>(g, fl, tl, l.ntiles, c->b.as<double>(), c->x.as<double>(), c->p.as<double>(),
-                             c->r.as<double>(), c->z[0].as<double>(), c->z[1].as<double>(),
+                             c->p2.empty() ? nullptr : c->p2.as<double>(), c->r.as<double>(), c->z[0].as<double>(), c->z[1].as<double>(),
                              c->part.as<double>(), s);
   launch_finalize(0, c->part.as<double>(), (int)c->nparts, c->sc.as<PcgScalars>(), nullptr, nullptr,
                   nullptr, s);
@@ -1117,7 +1169,8 @@ void finish_deferred_x(gmgs_ctx* c, hipStream_t s) {
   if (!c->no_jacobi) return;
   Level& l = L1(c);
   launch_add_alpha_p(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->x.as<double>(),
-                     c->p.as<double>(), c->sc.as<PcgScalars>(), s);
+                     c->p.as<double>(), fused_direction(c) ? c->p2.as<double>() : nullptr,
+                     c->sc.as<PcgScalars>(), s);
 }
 
 // Batch graphs (gmgs_batch_solve): one iteration pair of each context of a
@@ -2276,7 +2329,7 @@ int gmgs_kernel_timing(gmgs_ctx* c, int enable) {
 }
 
 int gmgs_kernel_timing_read(gmgs_ctx* c, int kernel, double* avg_ms, int64_t* count) {
-  if (!c || kernel < 0 || kernel > 5 || !avg_ms || !count) return GMGS_ERR_ARGUMENT;
+  if (!c || kernel < 0 || kernel > GMGS_K_MATVEC64_P || !avg_ms || !count) return GMGS_ERR_ARGUMENT;
   *count = c->kcnt[kernel];
   *avg_ms = c->kcnt[kernel] ? c->ktot[kernel] / (double)c->kcnt[kernel] : 0.0;
   return GMGS_OK;
@@ -2284,14 +2337,20 @@ int gmgs_kernel_timing_read(gmgs_ctx* c, int kernel, double* avg_ms, int64_t* co
 
 int gmgs_time_operator(gmgs_ctx* c, int kernel, int reps, double* avg_ms) {
   if (!c || c->lev.empty() || reps < 1 || !avg_ms) return GMGS_ERR_ARGUMENT;
-  if (kernel != GMGS_K_MATVEC64 && kernel != GMGS_K_JACOBI64) return GMGS_ERR_ARGUMENT;
+  if (kernel != GMGS_K_MATVEC64 && kernel != GMGS_K_JACOBI64 && kernel != GMGS_K_MATVEC64_P)
+    return GMGS_ERR_ARGUMENT;
   if (!c->bc) return fail(c, GMGS_ERR_STATE, "boundary conditions not set");
+  if (kernel == GMGS_K_MATVEC64_P && !fused_direction(c))
+    return fail(c, GMGS_ERR_STATE, "this solver does not fuse the direction update into its matvec");
   return guarded(c, [&]() -> int {
     GMGS_HIP(hipSetDevice(c->device));
     auto launch = [&]() {
       if (kernel == GMGS_K_MATVEC64)
         apply_level<double, double>(c, 0, AP_AX, c->p.as<double>(), 0.0, c->q.as<double>(), nullptr,
                                     c->part.as<double>(), nullptr);
+      else if (kernel == GMGS_K_MATVEC64_P)
+        launch_matvec_p(c, c->tmp.as<double>(), c->tmp2.as<double>(), c->ra.as<double>(),
+                        c->z[0].as<float>(), c->part.as<double>(), nullptr);
       else
         apply_level<double, double>(c, 0, AP_SMOOTH, c->r.as<double>(), c->prm.jacobi_weight,
                                     c->tmp.as<double>(), nullptr, nullptr, nullptr);

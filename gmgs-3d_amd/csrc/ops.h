@@ -53,8 +53,11 @@ struct PcgScalars {
 // that ends the symmetric V(2, 2) cycle on level 1, writing its sum instead
 // of Q_1 (CycleOut).  RESID_SUM: the residual Q_0 = in2 - A e from in = e
 // that ends the symmetric V(1, 1) cycle on level 1, writing its sum
-// z = omega (r0 + Q_0) / d + e instead of Q_0 (CycleOut without R1, e, w1)
-enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3, AP_RESID_SUM = 4 };
+// z = omega (r0 + Q_0) / d + e instead of Q_0 (CycleOut without R1, e, w1).
+// AX_P: the PCG's direction update fused into its matvec (level 1, FP64, FP32
+// z): p = z + beta p_old from in = p_old, out = A p, p written to CycleOut::pn
+// (a buffer other than in), x += alpha p_old, and the p.q partials
+enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3, AP_RESID_SUM = 4, AP_AX_P = 5 };
 
 // outputs of AP_SMOOTH_SUM: z = (omega (r0 + Q_0) + w1 (R1 + Q_1)) / d + e,
 // fixed components 0, and partials of r0.z and r0.zprev (zprev may be null)
@@ -66,6 +69,11 @@ struct CycleOut {
   const T* zprev;
   T* z;
   T w1;
+  // AX_P: z of this iteration, the new direction, x, and beta / alpha
+  const float* zf;
+  double* pn;
+  double* x;
+  const PcgScalars* sc;
 };
 
 // ---- launches (all on stream s; 'done' may be null) ----
@@ -148,9 +156,11 @@ void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles,
                      const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s,
                      double* x = nullptr);
 
-// x += alpha p, unconditionally (the last deferred x update, after the PCG loop)
+// x += alpha p, unconditionally (the last deferred x update, after the PCG
+// loop); with p_odd, the directions alternate between p (even iterations)
+// and p_odd, and the last iteration's one is taken
 void launch_add_alpha_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, double* x,
-                        const double* p, const PcgScalars* sc, hipStream_t s);
+                        const double* p, const double* p_odd, const PcgScalars* sc, hipStream_t s);
 
 // x = (x + omega r / d) + alpha p (the Jacobi increment of the iteration, from
 // r before it is replaced; none when invd is null; no x update when x is
@@ -192,11 +202,12 @@ template <typename T, typename TR>
 int launch_apply_as_rows(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode, const TR* in,
                          T omega, const T* in2, T* out, double* partial, const int* done, hipStream_t s);
 
-// start of a solve from x0 = 0: x = p = z0 = z1 = 0, r = b, partials of b.b
+// start of a solve from x0 = 0: x = p = z0 = z1 = 0 (and p2, if given),
+// r = b, partials of b.b
 template <typename TZ>
 void launch_pcg_start(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
-                      const double* b, double* x, double* p, double* r, TZ* z0, TZ* z1, double* partial,
-                      hipStream_t s);
+                      const double* b, double* x, double* p, double* p2, double* r, TZ* z0, TZ* z1,
+                      double* partial, hipStream_t s);
 
 // partials of |b - q|^2 over tiles
 void launch_resid_sq(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,

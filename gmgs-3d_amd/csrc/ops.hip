@@ -691,8 +691,12 @@ __global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __rest
 __global__ __launch_bounds__(256) void k_add_alpha_p(Grid g, const uint32_t* __restrict__ flags,
                                                      const TileRec* __restrict__ tiles, long long nt,
                                                      double* __restrict__ x, const double* __restrict__ p,
+                                                     const double* __restrict__ p_odd,
                                                      const PcgScalars* __restrict__ sc) {
   const double alpha = sc->alpha;
+  // directions alternating between two buffers: iteration k (its = k + 1
+  // after it) wrote p[k & 1]
+  if (p_odd && ((sc->its - 1) & 1)) p = p_odd;
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -858,7 +862,8 @@ template <typename TZ>
 __global__ __launch_bounds__(256) void k_pcg_start(Grid g, const uint32_t* __restrict__ flags,
                                                    const TileRec* __restrict__ tiles, long long nt,
                                                    const double* __restrict__ b, double* __restrict__ x,
-                                                   double* __restrict__ p, double* __restrict__ r,
+                                                   double* __restrict__ p, double* __restrict__ p2,
+                                                   double* __restrict__ r,
                                                    TZ* __restrict__ z0, TZ* __restrict__ z1,
                                                    double* __restrict__ partial) {
   double d = 0.0;
@@ -871,6 +876,7 @@ __global__ __launch_bounds__(256) void k_pcg_start(Grid g, const uint32_t* __res
       const double v = b[m];
       x[m] = 0.0;
       p[m] = 0.0;
+      if (p2) p2[m] = 0.0;
       r[m] = v;
       z0[m] = (TZ)0;
       z1[m] = (TZ)0;
@@ -1287,9 +1293,9 @@ void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles,
 }
 
 void launch_add_alpha_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, double* x,
-                        const double* p, const PcgScalars* sc, hipStream_t s) {
+                        const double* p, const double* p_odd, const PcgScalars* sc, hipStream_t s) {
   if (nt <= 0) return;
-  k_add_alpha_p<<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, sc);
+  k_add_alpha_p<<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, p_odd, sc);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1357,17 +1363,17 @@ void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long
 
 template <typename TZ>
 void launch_pcg_start(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
-                      const double* b, double* x, double* p, double* r, TZ* z0, TZ* z1, double* partial,
-                      hipStream_t s) {
-  k_pcg_start<TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, b, x, p, r, z0, z1, partial);
+                      const double* b, double* x, double* p, double* p2, double* r, TZ* z0, TZ* z1,
+                      double* partial, hipStream_t s) {
+  k_pcg_start<TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, b, x, p, p2, r, z0, z1, partial);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_pcg_start<double>(const Grid&, const uint32_t*, const TileRec*, long long,
                                        const double*, double*, double*, double*, double*, double*,
-                                       double*, hipStream_t);
+                                       double*, double*, hipStream_t);
 template void launch_pcg_start<float>(const Grid&, const uint32_t*, const TileRec*, long long,
-                                      const double*, double*, double*, double*, float*, float*, double*,
-                                      hipStream_t);
+                                      const double*, double*, double*, double*, double*, float*, float*,
+                                      double*, hipStream_t);
 
 void launch_resid_sq(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                      const double* b, const double* q, double* partial, hipStream_t s) {

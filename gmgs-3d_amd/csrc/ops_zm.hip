@@ -19,6 +19,8 @@
 // wave gathers those (node, element) pairs and spreads them over its lanes,
 // so the few boundary nodes of a wave do not serialise it.  Together: the
 // per-element sum of kernel/AtX.m:1-58.
+#include <type_traits>
+
 #include "ops.h"
 #include "ops_zm.h"
 #include "mesh.h"
@@ -64,7 +66,10 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
   constexpr bool SUM = MODE == AP_SMOOTH_SUM, RSUM = MODE == AP_RESID_SUM;
   constexpr bool SM = MODE == AP_SMOOTH || SUM;
+  constexpr bool AXP = MODE == AP_AX_P;              // p = z + beta p_old staged, x updated
   constexpr bool TWO = SUM || RSUM;                 // two dot partials (r0.z, r0.zprev)
+  constexpr int ND = (SM || AXP) ? PER : 1;          // second staged operand: 1/d or z
+  using TD = typename std::conditional<AXP, float, T>::type;
   constexpr bool HAS_EOP = E == 1, FAST_ONLY = E == 2;
   // output nodes of this kernel
   constexpr uint32_t OUT_MASK = FAST_ONLY ? F_FAST : F_ELEM_MASK;
@@ -73,7 +78,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   // buffer k % 4 and read while computing planes k, k + 1 and k + 2 (stencil,
   // output completion, element corrections)
   __shared__ T sU[4][3][SX][SY];
-  __shared__ T sR[SM ? 2 : 1][3][TX][TY];            // SMOOTH: centre values of r
+  __shared__ T sR[(SM || AXP) ? 2 : 1][3][TX][TY];   // SMOOTH: centre values of r; AX_P: of p_old
   // per wave: the element corrections' task list and results
   __shared__ int sTask[FAST_ONLY ? 1 : NT / 64][64];
   __shared__ T sRes[FAST_ONLY ? 1 : NT / 64][3][64];
@@ -124,18 +129,19 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     }
   }
   // raw loads only (no arithmetic on them until they are staged, two planes
-  // later): the input and, for the smoother, the inverse diagonal.  Index 0 of
+  // later): the input and, for the smoother, the inverse diagonal (AX_P: z).  Index 0 of
   // every padded array is a corner halo entry and holds 0, so a skipped
   // position loads from there: the loads need no branches.
   // The index is formed in 32 bits (3 N < 2^31) and selected, not branched on.
-  auto fetch = [&](int zz, TR* vr, T* vd) {
+  auto fetch = [&](int zz, TR* vr, TD* vd) {
     const int szz = (int)sz * zz;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const bool ok = (unsigned)(zz - (zr[i] & 0xffff)) <= (unsigned)(zr[i] >> 16);
       const unsigned m = ok ? (unsigned)(soff[i] + szz) : 0u;
       vr[i] = in[m];
-      if (SM) vd[i] = op.invd[m];
+      if (SM) vd[i] = (TD)op.invd[m];
+      if (AXP) vd[i] = (TD)co.zf[m];
     }
   };
 
@@ -146,7 +152,12 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   // two planes of loads in flight: plane zz is staged from one register set
   // while the other holds plane zz + 1; z0 < z1 for every tile
   TR rA[PER], rB[PER];
-  T dA[SM ? PER : 1], dB[SM ? PER : 1];
+  TD dA[ND], dB[ND];
+  T beta = 0, alpha = 0;
+  if (AXP) {
+    beta = (T)co.sc->beta;
+    alpha = (T)co.sc->alpha;
+  }
   fetch(z0 - 1, rA, dA);
   fetch(z0, rB, dB);
   // flags of the outputs this lane owns, loaded one plane before they are used
@@ -156,14 +167,15 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   };
   uint32_t fN = flags_at(z0);
 
-  auto plane = [&](const int zz, TR (&sr)[PER], T (&sd)[SM ? PER : 1]) {
+  auto plane = [&](const int zz, TR (&sr)[PER], TD (&sd)[ND]) {
     T (*const sB)[SX][SY] = sU[kb];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int e = tid + i * NT;
       if (e < NSTAGE) {
         T v = (T)sr[i];
-        if (SM) v = omega * sd[i] * v;
+        if (SM) v = omega * (T)sd[i] * v;
+        if (AXP) v = (T)sd[i] + beta * v;              // p = z + beta p_old (CG_solver.m:28)
         (&sB[0][0][0])[e] = v;
       }
     }
@@ -171,8 +183,8 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     // zz - 4 .. zz - 2, which every thread finished before the barrier of
     // plane zz - 1
     __syncthreads();
-    if (SM) {
-      // centre values of r for the output completed at plane zz + 1; buffer
+    if (SM || AXP) {
+      // centre values of r (p_old) for the output completed at plane zz + 1; buffer
       // zz % 2 was last read at plane zz - 1, before this plane's barrier
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
@@ -185,13 +197,13 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     }
     // planes zz-2, zz-1, zz are staged in buffers b2, b1, kb
     const int b1 = (kb + 3) & 3, b2 = (kb + 2) & 3;
-    // RESID: centre values of in2 at the output completed in this plane,
-    // loaded now and used after the stencil
+    // RESID: centre values of in2 at the output completed in this plane
+    // (AX_P: of x), loaded now and used after the stencil
     T cen[3] = {0, 0, 0};
-    if ((MODE == AP_RESID || RSUM) && lane_ok && (fM & OUT_MASK)) {
+    if ((MODE == AP_RESID || RSUM || AXP) && lane_ok && (fM & OUT_MASK)) {
       const long long n = colidx + sz * (zz - 1);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) cen[c] = in2[c * N + n];
+      for (int c = 0; c < 3; ++c) cen[c] = AXP ? (T)co.x[c * N + n] : in2[c * N + n];
     }
     // Element corrections of output zz-1.  An owned node lacking incident
     // element p subtracts that element's universal row block, one touching a
@@ -352,6 +364,14 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
         if (MODE == AP_AX) {
           out[c * N + n] = a;
           if (partial) dsum += (double)sU[b1][c][lx + 1][ly + 1] * (double)a;
+        } else if (AXP) {
+          // q = A p, the new p, and the previous iteration's x += alpha p_old
+          const long long m = c * N + n;
+          const T pn = sU[b1][c][lx + 1][ly + 1];
+          out[m] = a;
+          co.pn[m] = pn;
+          co.x[m] = cen[c] + alpha * sR[zo_ & 1][c][lx][ly];
+          if (partial) dsum += (double)pn * (double)a;
         } else if (MODE == AP_RESID) {
           out[c * N + n] = cen[c] - a;
         } else if (RSUM) {
@@ -504,9 +524,11 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   if (ntiles <= 0) return 0;
   if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("grid too large for the z-marching operator");
   const int e = fast_only ? 2 : (op.eop != nullptr ? 1 : 0);
-  const CycleOut<T> cov = co ? *co : CycleOut<T>{nullptr, nullptr, nullptr, nullptr, nullptr, (T)0};
+  const CycleOut<T> cov = co ? *co : CycleOut<T>{};
   if ((mode == AP_SMOOTH_SUM || mode == AP_RESID_SUM) && (!co || e != 0 || !op.invd))
     throw std::invalid_argument("SMOOTH_SUM / RESID_SUM: level 1 only, with its outputs");
+  if (mode == AP_AX_P && (!co || e != 0 || !co->zf || !co->pn || !co->x || !co->sc || co->pn == (const void*)in))
+    throw std::invalid_argument("AX_P: level 1 only, with z, x, the scalars and a separate new p");
 #define GMGS_ZM(MODE, E, TXV)                                                                        \
   k_apply_zm<T, TR, MODE, E, TXV><<<(unsigned)ntiles, 16 * TXV, 0, s>>>(op, tiles, zc, in, omega, in2, \
                                                                         out, partial, done, cov)
@@ -532,6 +554,9 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   } else if (mode == AP_RESID_SUM) {
     if constexpr (std::is_same<T, TR>::value) GMGS_ZM_TX(AP_RESID_SUM, 0);
     else throw std::invalid_argument("RESID_SUM: input in the cycle precision");
+  } else if (mode == AP_AX_P) {
+    if constexpr (std::is_same<T, double>::value && std::is_same<TR, double>::value) GMGS_ZM_TX(AP_AX_P, 0);
+    else throw std::invalid_argument("AX_P: FP64 operator and input");
   } else {
     GMGS_ZM_E(AP_SMOOTH);
   }

@@ -311,7 +311,7 @@ class Solver:
         return lam, dJ, info
 
     # ---- kernel timing
-    KERNELS = dict(matvec64=0, jacobi64=1, vsmooth1=2, restrict1=3, prolong1=4, iteration=5)
+    KERNELS = dict(matvec64=0, jacobi64=1, vsmooth1=2, restrict1=3, prolong1=4, iteration=5, matvec64_p=6)
 
     def kernel_timing(self, enable=True):
         self._check(self._lib.gmgs_kernel_timing(self._ctx, 1 if enable else 0))
@@ -327,7 +327,9 @@ class Solver:
 
     def time_operator(self, kernel="matvec64", reps=20):
         """Mean duration (ms) of ``reps`` back-to-back launches of a level-1
-        FP64 operator ("matvec64" or "jacobi64") on device-resident vectors."""
+        FP64 operator ("matvec64", "jacobi64", or "matvec64_p": the matvec
+        with the direction update fused, where the solver uses it) on
+        device-resident vectors."""
         ms = C.c_double()
         self._check(self._lib.gmgs_time_operator(self._ctx, self.KERNELS[kernel], int(reps),
                                                  C.byref(ms)))

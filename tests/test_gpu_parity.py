@@ -396,10 +396,10 @@ def true_lmax(m, oracle, ii):
     return float(sla.eigsh(op, k=1, which="LA", tol=1e-10, return_eigenvectors=False)[0])
 
 
-def cheb_solver(case, pkg, precision, nu, nu_coarse=0, nu2=0):
+def cheb_solver(case, pkg, precision, nu, nu_coarse=0, nu2=0, tol=1e-8, max_it=2000):
     m = case["m"]
     s = pkg.Solver(case["vox"], case["bbox"], num_levels=m.numLevels, jacobi_weight=case["omega"],
-                   tol=1e-8, max_it=2000, precision=precision, vcycle_sweeps=nu,
+                   tol=tol, max_it=max_it, precision=precision, vcycle_sweeps=nu,
                    vcycle_sweeps_coarse=nu_coarse, vcycle_sweeps_level2=nu2,
                    flags=pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP | pkg.FLAG_CHEBYSHEV)
     s.set_bc(case["fixed"], case["loads"])
@@ -469,4 +469,34 @@ def test_chebyshev_cycle_solves(case, oracle, pkg, nu, nu_coarse, nu2):
     e3 = np.linalg.norm(U3 - ref) / np.linalg.norm(ref)
     log(f"  chebyshev mixed: its {info3.its} true {info3.true_relres:.2e} err {e3:.2e}")
     assert info3.converged and info3.true_relres < 1e-7 and e3 < 1e-5
+    s3.close()
+
+
+@pytest.mark.parametrize("max_it", [2, 3])
+def test_mixed_direction_update_fixed_iterations(case, oracle, pkg, max_it):
+    """The mixed solver without the Jacobi step fuses p = z + beta p and the
+    deferred x += alpha p into its level-1 matvec, with p alternating between
+    two buffers; stopped after an odd and an even number of iterations
+    (max_it + 1), x must match the FP64 flexible PCG of the oracle with the
+    same sweep weights (a wrong buffer for the last x update, or a stale p,
+    is an O(1) error here, FP32 cycle rounding ~1e-6)."""
+    m = case["m"]
+    s3 = cheb_solver(case, pkg, pkg.GMGS_PREC_MIXED, 1, 4, 2, tol=1e-30, max_it=max_it)
+    w = {ii: s3.smoother(ii)[1] for ii in range(1, m.numLevels)}
+    res = oracle.cg_solver(m, m.F, 1e-30, max_it, flexible=True, jacobi_step=False,
+                           cycle=lambda mm, r: oracle.vcycle_sym(mm, r, weights=w))
+    # once launch by launch with events (shows the fused kernel ran), once
+    # through the captured iteration pairs
+    s3.kernel_timing(True)
+    U1, info1, _ = s3.solve(raise_on_nonconvergence=False)
+    kt = s3.kernel_times()
+    s3.kernel_timing(False)
+    assert kt["matvec64_p"][1] == max_it + 1 and kt["matvec64"][1] == 0
+    U2, info2, _ = s3.solve(raise_on_nonconvergence=False)
+    for U, info in ((U1, info1), (U2, info2)):
+        e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
+        log(f"  fused direction: max_it {max_it} gpu its {info.its} oracle its {res.its} err {e:.2e}")
+        assert info.its == res.its == max_it + 1 and not info.converged
+        assert e < 1e-4
+    assert np.array_equal(U1, U2)
     s3.close()
