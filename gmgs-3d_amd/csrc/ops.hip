@@ -907,10 +907,20 @@ __global__ __launch_bounds__(256) void k_resid_sq(Grid g, const uint32_t* __rest
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// one block of 256 threads; fixed-order sums
+// one block of 256 threads; fixed-order sums (thread t adds part[t],
+// part[t + 256], ... in turn; the loads are issued eight at a time so the
+// chain of adds does not wait for each load in turn)
 __device__ inline double reduce_parts(const double* __restrict__ part, int n) {
   double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) v += part[i];
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < n; i += 8 * 256) {
+    double a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = part[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v += a[u];
+  }
+  for (; i < n; i += 256) v += part[i];
   __shared__ double sh[256];
   sh[threadIdx.x] = v;
   __syncthreads();
