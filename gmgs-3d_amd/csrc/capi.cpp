@@ -1722,10 +1722,9 @@ static int finish_solve(gmgs_ctx* c, const LoopResult& r, gmgs_solve_info* info,
                         int64_t hist_cap) {
   hipStream_t s = c->stream;
   Level& l = L1(c);
-  // true FP64 residual at exit: tmp = b - A x
-  LevelOp<double> op = make_op<double>(c, 0);
-  launch_apply<double, double>(op, l.tiles.as<TileRec>(), l.ntiles, AP_AX, c->x.as<double>(), 0.0,
-                               c->tmp.as<double>(), nullptr, nullptr, nullptr, s);
+  // true FP64 residual at exit: tmp = b - A x (the level's fastest FP64 operator)
+  apply_level<double, double>(c, 0, AP_AX, c->x.as<double>(), 0.0, c->tmp.as<double>(), nullptr, nullptr,
+                              nullptr);
   launch_resid_sq(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->b.as<double>(),
                   c->tmp.as<double>(), c->part.as<double>(), s);
   std::vector<double> parts((size_t)c->nparts);
