@@ -1146,14 +1146,20 @@ void pcg_init(gmgs_ctx* c, double tol) {
   const Grid& g = l.mesh.g;
   const uint32_t* fl = l.mesh.flags.as<uint32_t>();
   const TileRec* tl = l.tiles.as<TileRec>();
+  // With the direction update fused into the matvec, iteration 0 reads
+  // only p2 (zeroed: x += alpha p2 there) and writes p and z0 before any
+  // read; z1 enters only r'z_prev, which the first iteration's beta = 0
+  // does not use.
+  const bool fused = fused_direction(c);
   if (c->mixed)
-    launch_pcg_start<float>(g, fl, tl, l.ntiles, c->b.as<double>(), c->x.as<double>(), c->p.as<double>(),
-                            c->p2.empty() ? nullptr : c->p2.as<double>(), c->r.as<double>(), c->z[0].as<float>(), c->z[1].as<float>(),
-                            c->part.as<double>(), s);
+    launch_pcg_start<float>(g, fl, tl, l.ntiles, c->b.as<double>(), c->x.as<double>(),
+                            fused ? nullptr : c->p.as<double>(), c->p2.empty() ? nullptr : c->p2.as<double>(),
+                            c->r.as<double>(), fused ? nullptr : c->z[0].as<float>(),
+                            fused ? nullptr : c->z[1].as<float>(), c->part.as<double>(), s);
   else
     launch_pcg_start<double>(g, fl, tl, l.ntiles, c->b.as<double>(), c->x.as<double>(), c->p.as<double>(),
-                             c->p2.empty() ? nullptr : c->p2.as<double>(), c->r.as<double>(), c->z[0].as<double>(), c->z[1].as<double>(),
-                             c->part.as<double>(), s);
+                             c->p2.empty() ? nullptr : c->p2.as<double>(), c->r.as<double>(),
+                             c->z[0].as<double>(), c->z[1].as<double>(), c->part.as<double>(), s);
   launch_finalize(0, c->part.as<double>(), (int)c->nparts, c->sc.as<PcgScalars>(), nullptr, nullptr,
                   nullptr, s);
 }

@@ -202,8 +202,8 @@ template <typename T, typename TR>
 int launch_apply_as_rows(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode, const TR* in,
                          T omega, const T* in2, T* out, double* partial, const int* done, hipStream_t s);
 
-// start of a solve from x0 = 0: x = p = z0 = z1 = 0 (and p2, if given),
-// r = b, partials of b.b
+// start of a solve from x0 = 0: x = 0, r = b, partials of b.b, and p, p2,
+// z0, z1 = 0 where given (null: not written)
 template <typename TZ>
 void launch_pcg_start(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                       const double* b, double* x, double* p, double* p2, double* r, TZ* z0, TZ* z1,

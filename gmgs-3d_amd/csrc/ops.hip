@@ -875,11 +875,11 @@ __global__ __launch_bounds__(256) void k_pcg_start(Grid g, const uint32_t* __res
       const long long m = c * N + n;
       const double v = b[m];
       x[m] = 0.0;
-      p[m] = 0.0;
+      if (p) p[m] = 0.0;
       if (p2) p2[m] = 0.0;
       r[m] = v;
-      z0[m] = (TZ)0;
-      z1[m] = (TZ)0;
+      if (z0) z0[m] = (TZ)0;
+      if (z1) z1[m] = (TZ)0;
       d += v * v;
     }
   }
