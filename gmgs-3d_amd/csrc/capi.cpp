@@ -1728,12 +1728,19 @@ static int finish_solve(gmgs_ctx* c, const LoopResult& r, gmgs_solve_info* info,
                         int64_t hist_cap) {
   hipStream_t s = c->stream;
   Level& l = L1(c);
-  // true FP64 residual at exit: tmp = b - A x (the level's fastest FP64 operator)
-  apply_level<double, double>(c, 0, AP_AX, c->x.as<double>(), 0.0, c->tmp.as<double>(), nullptr, nullptr,
-                              nullptr);
-  launch_resid_sq(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->b.as<double>(),
-                  c->tmp.as<double>(), c->part.as<double>(), s);
-  std::vector<double> parts((size_t)c->nparts);
+  // true FP64 residual at exit: tmp = b - A x and the partials of |tmp|^2,
+  // in one z-march where level 1 has it
+  long long np = c->nparts;
+  if (l.zm && !l.split) {
+    np = apply_level<double, double>(c, 0, AP_RESID, c->x.as<double>(), 0.0, c->tmp.as<double>(), nullptr,
+                                     c->part.as<double>(), nullptr, c->b.as<double>());
+  } else {
+    apply_level<double, double>(c, 0, AP_AX, c->x.as<double>(), 0.0, c->tmp.as<double>(), nullptr, nullptr,
+                                nullptr);
+    launch_resid_sq(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->b.as<double>(),
+                    c->tmp.as<double>(), c->part.as<double>(), s);
+  }
+  std::vector<double> parts((size_t)np);
   PcgScalars h{};
   GMGS_HIP(hipMemcpyAsync(parts.data(), c->part.as<void>(), sizeof(double) * parts.size(),
                           hipMemcpyDeviceToHost, s));

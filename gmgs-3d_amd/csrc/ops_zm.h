@@ -18,7 +18,8 @@ long long build_coltiles(const Grid& g, const uint32_t* flags, int tx, int zc, D
 void build_colrange(const Grid& g, const uint32_t* flags, DevBuf& colz, hipStream_t s);
 
 // same contract as launch_apply without the x += u accumulation, plus the
-// RESID mode (out = in2 - A in) and, on level 1 (no stored elements), the
+// RESID mode (out = in2 - A in; with partials, of |out|^2) and, on level 1
+// (no stored elements), the
 // SMOOTH_SUM mode (co: its outputs; partials r0.z at [0, n) and r0.zprev at
 // [n, 2 n)); returns the number n of dot partials written (one per tile).
 // fast_only: only the F_FAST nodes (universal stencil, no element

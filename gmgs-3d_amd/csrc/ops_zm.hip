@@ -373,7 +373,9 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
           co.x[m] = cen[c] + alpha * sR[zo_ & 1][c][lx][ly];
           if (partial) dsum += (double)pn * (double)a;
         } else if (MODE == AP_RESID) {
-          out[c * N + n] = cen[c] - a;
+          const T v = cen[c] - a;
+          out[c * N + n] = v;
+          if (partial) dsum += (double)v * (double)v;   // |in2 - A in|^2
         } else if (RSUM) {
           // z = omega (r0 + Q_0) / d + e, Q_0 = in2 - A e; e at the node is
           // the staged input's centre
