@@ -499,4 +499,16 @@ def test_mixed_direction_update_fixed_iterations(case, oracle, pkg, max_it):
         assert info.its == res.its == max_it + 1 and not info.converged
         assert e < 1e-4
     assert np.array_equal(U1, U2)
+    # the tile query reports the fused kernel's shape; its timing hook runs
+    # on scratch vectors and leaves the next solve unchanged
+    tc = s3.query("TILE_CONFIG", 1)
+    assert tc.size == 5 and tc[3] in (4, 8, 16) and tc[4] in (4, 8, 16, 32)
+    assert s3.time_operator("matvec64_p", 2) > 0.0
+    U3, info3, _ = s3.solve(raise_on_nonconvergence=False)
+    assert np.array_equal(U2, U3)
     s3.close()
+    # a solver without the fused update rejects its timing hook
+    s2 = cheb_solver(case, pkg, pkg.GMGS_PREC_FP64, 1, 4, 2, tol=1e-30, max_it=max_it)
+    with pytest.raises(pkg.GMGSError):
+        s2.time_operator("matvec64_p", 1)
+    s2.close()
