@@ -646,7 +646,7 @@ float time_v1(gmgs_ctx* c, int li, const TR* in, T* out, bool as = false) {
 
 template <typename T, typename TR>
 float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, int zc, const TR* in,
-              T* out, int mode = AP_SMOOTH, const CycleOut<T>* co = nullptr) {
+              T* out, int mode = AP_SMOOTH, const CycleOut<T>* co = nullptr, const T* in2 = nullptr) {
   Level& l = c->lev[li];
   hipStream_t s = c->stream;
   LevelOp<T> op = make_op<T>(c, li);
@@ -654,14 +654,14 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
   GMGS_HIP(hipEventCreate(&a));
   GMGS_HIP(hipEventCreate(&b));
   for (int w = 0; w < 2; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, nullptr, out, nullptr, nullptr, s, co);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, in2, out, nullptr, nullptr, s, co);
   // the better of two timed batches: one disturbed batch does not decide
   const int reps = 5;
   float best = 1e30f;
   for (int batch = 0; batch < 2; ++batch) {
     GMGS_HIP(hipEventRecord(a, s));
     for (int w = 0; w < reps; ++w)
-      launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, nullptr, out, nullptr, nullptr, s, co);
+      launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, in2, out, nullptr, nullptr, s, co);
     GMGS_HIP(hipEventRecord(b, s));
     GMGS_HIP(hipEventSynchronize(b));
     float ms = 0;
@@ -753,7 +753,19 @@ void autotune_levels(gmgs_ctx* c) {
           const long long nt = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
           float ms;
           if (li == 0) {
-            if (c->mixed) ms = time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<float>());
+            if (c->mixed) {
+              ms = time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
+                                          l.rp.as<float>());
+              // the symmetric V(1, 1) cycle runs its residual-and-sum kernel on
+              // the same tiles: time both (scratch: e, R_1 = rp, r0 = tmp, z = z[1])
+              if (c->sym && l.nu == 1 && !l.e.empty()) {
+                CycleOut<float> co{};
+                co.r0 = c->tmp.as<double>();
+                co.z = c->z[1].as<float>();
+                ms += time_zm<float, float>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], l.e.as<float>(),
+                                            nullptr, AP_RESID_SUM, &co, l.rp.as<float>());
+              }
+            }
             else ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<double>());
           } else {
             if (c->mixed) ms = time_zm<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(), l.rp.as<float>());
