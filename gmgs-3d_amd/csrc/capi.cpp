@@ -335,6 +335,11 @@ void build_mesh(gmgs_ctx* c, const int32_t* vox) {
     c->bbox[3 + d] = lo + (hi - lo) * (double)res[d] / (double)nel[d];
     c->eleSize[d] = (c->bbox[3 + d] - lo) / (double)res[d];
   }
+  {
+    bool any = false;
+    for (long long i = 0, n = nelx * nely * nelz; i < n && !any; ++i) any = vox[i] == 1;
+    if (!any) throw Status(GMGS_ERR_RESOLUTION, "the voxel model has no solid voxel");
+  }
   DevBuf dvox;
   upload(dvox, vox, (size_t)(nelx * nely * nelz), s);
   c->lev.resize(c->L);
@@ -624,9 +629,7 @@ float time_v1(gmgs_ctx* c, int li, const TR* in, T* out, bool as = false) {
   l.as = as;
   LevelOp<T> op = make_op<T>(c, li);
   l.as = keep;
-  hipEvent_t a, b;
-  GMGS_HIP(hipEventCreate(&a));
-  GMGS_HIP(hipEventCreate(&b));
+  HipEvent a, b;
   for (int w = 0; w < 2; ++w)
     launch_apply<T, TR>(op, l.tiles.as<TileRec>(), l.ntiles, AP_SMOOTH, in, (T)0.5, out, nullptr, nullptr,
                         nullptr, s);
@@ -639,8 +642,6 @@ float time_v1(gmgs_ctx* c, int li, const TR* in, T* out, bool as = false) {
   GMGS_HIP(hipEventSynchronize(b));
   float ms = 0;
   GMGS_HIP(hipEventElapsedTime(&ms, a, b));
-  GMGS_HIP(hipEventDestroy(a));
-  GMGS_HIP(hipEventDestroy(b));
   return ms / reps;
 }
 
@@ -650,9 +651,7 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
   Level& l = c->lev[li];
   hipStream_t s = c->stream;
   LevelOp<T> op = make_op<T>(c, li);
-  hipEvent_t a, b;
-  GMGS_HIP(hipEventCreate(&a));
-  GMGS_HIP(hipEventCreate(&b));
+  HipEvent a, b;
   for (int w = 0; w < 2; ++w)
     launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, in2, out, nullptr, nullptr, s, co);
   // the better of two timed batches: one disturbed batch does not decide
@@ -668,8 +667,6 @@ float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, i
     GMGS_HIP(hipEventElapsedTime(&ms, a, b));
     best = std::min(best, ms / reps);
   }
-  GMGS_HIP(hipEventDestroy(a));
-  GMGS_HIP(hipEventDestroy(b));
   (void)l;
   return best;
 }
@@ -693,9 +690,7 @@ float time_split(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx
     launch_apply_as_rows<T, TR>(opr, l.mtiles.as<TileRec>(), l.nmtiles, AP_SMOOTH, in, (T)0.5, nullptr, out,
                                 nullptr, nullptr, s);
   };
-  hipEvent_t a, b;
-  GMGS_HIP(hipEventCreate(&a));
-  GMGS_HIP(hipEventCreate(&b));
+  HipEvent a, b;
   for (int w = 0; w < 2; ++w) run();
   GMGS_HIP(hipEventRecord(a, s));
   const int reps = 5;
@@ -704,8 +699,6 @@ float time_split(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx
   GMGS_HIP(hipEventSynchronize(b));
   float ms = 0;
   GMGS_HIP(hipEventElapsedTime(&ms, a, b));
-  GMGS_HIP(hipEventDestroy(a));
-  GMGS_HIP(hipEventDestroy(b));
   return ms / reps;
 }
 
@@ -1235,13 +1228,23 @@ void invalidate_graph(gmgs_ctx* c) {
 void capture_iteration_pair(gmgs_ctx* c) {
   hipGraph_t g = nullptr;
   GMGS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-  for (int k = 0; k < 2; ++k) {
-    if (c->mixed) pcg_iteration<float>(c, k);
-    else pcg_iteration<double>(c, k);
+  try {
+    for (int k = 0; k < 2; ++k) {
+      if (c->mixed) pcg_iteration<float>(c, k);
+      else pcg_iteration<double>(c, k);
+    }
+  } catch (...) {
+    // leave the stream out of capture mode before the error propagates
+    if (hipStreamEndCapture(c->stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    throw;
   }
   GMGS_HIP(hipStreamEndCapture(c->stream, &g));
-  GMGS_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
-  GMGS_HIP(hipGraphDestroy(g));
+  const hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    GMGS_HIP(e);
+  }
 }
 
 LoopResult pcg_run(gmgs_ctx* c) {
@@ -1253,9 +1256,7 @@ LoopResult pcg_run(gmgs_ctx* c) {
   }
   const bool graph = c->use_graphs && !c->ktime;
   if (graph && !c->gexec) capture_iteration_pair(c);
-  hipEvent_t e0, e1;
-  GMGS_HIP(hipEventCreate(&e0));
-  GMGS_HIP(hipEventCreate(&e1));
+  HipEvent e0, e1;
   GMGS_HIP(hipEventRecord(e0, s));
   const int maxk = c->prm.max_it + 1;
   int k = 0;   // iterations launched
@@ -1312,8 +1313,6 @@ LoopResult pcg_run(gmgs_ctx* c) {
   GMGS_HIP(hipMemcpy(&h, c->sc.as<void>(), sizeof(h), hipMemcpyDeviceToHost));
   float ms = 0;
   GMGS_HIP(hipEventElapsedTime(&ms, e0, e1));
-  GMGS_HIP(hipEventDestroy(e0));
-  GMGS_HIP(hipEventDestroy(e1));
   LoopResult r;
   r.its = h.its;
   r.converged = h.converged;
@@ -1448,6 +1447,12 @@ void smoother_weights(gmgs_ctx* c) {
 
 void set_bc_impl(gmgs_ctx* c, const int32_t* fixed, int64_t k, const double* loads, int64_t m) {
   hipStream_t s = c->stream;
+  // the BC-dependent state (F_, fixed flags, coarsest factor, smoother
+  // weights) is rebuilt in place: until this call completes the context has
+  // no valid BCs, so a failure part-way leaves solves refused (GMGS_ERR_STATE)
+  // instead of mixing old buffers with the new free-DOF count
+  c->bc = false;
+  c->have_solution = false;
   invalidate_graph(c);
   Level& l = L1(c);
   const long long nn = l.mesh.nn;
@@ -1717,8 +1722,13 @@ int gmgs_set_modulus(gmgs_ctx* c, const double* modulus, int64_t n) {
       c->simp = true;
     }
     invalidate_graph(c);
+    // the operators are rebuilt in place: BCs count as unset until the
+    // BC-dependent set-up has been redone on them
+    const bool had_bc = c->bc;
+    c->bc = false;
+    c->have_solution = false;
     build_operators(c);
-    if (c->bc) {
+    if (had_bc) {
       // the coarse operators changed: redo the BC-dependent set-up
       std::vector<int32_t> fx = c->bc_fixed;
       std::vector<double> ld = c->bc_loads;
@@ -1836,12 +1846,15 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
   gmgs_ctx* c0 = cs[0];
   return guarded(c0, [&]() -> int {
     GMGS_HIP(hipSetDevice(c0->device));
-    hipStream_t s0;
-    GMGS_HIP(hipStreamCreateWithFlags(&s0, hipStreamNonBlocking));
-    std::vector<hipEvent_t> ev(n + 2);
-    for (auto& e : ev) GMGS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    int* pinned = nullptr;
-    GMGS_HIP(hipHostMalloc((void**)&pinned, sizeof(int) * n, hipHostMallocDefault));
+    // owned here and released on every exit path
+    HipStream s0;
+    std::vector<std::unique_ptr<HipEvent>> evs;
+    std::vector<hipEvent_t> ev;
+    for (int i = 0; i < n + 2; ++i) {
+      evs.push_back(std::make_unique<HipEvent>(hipEventDisableTiming));
+      ev.push_back(*evs.back());
+    }
+    PinnedBuf<int> pinned((size_t)n);
     int maxk = 0;
     for (int i = 0; i < n; ++i) {
       pcg_init(cs[i], cs[i]->prm.tol);
@@ -1884,15 +1897,14 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
     BatchKey act(cs, cs + n);
     BatchGraph* bg = &graph_for(act);
     GMGS_HIP(hipStreamSynchronize(s0));
-    hipEvent_t t0, t1;
-    GMGS_HIP(hipEventCreate(&t0));
-    GMGS_HIP(hipEventCreate(&t1));
+    HipEvent t0, t1;
     GMGS_HIP(hipEventRecord(t0, s0));
     int k = 0;
     int chunk = 2;
     while (k < maxk) {
       for (int i = 0; i < chunk && k < maxk; i += 2, k += 2) GMGS_HIP(hipGraphLaunch(bg->ge, s0));
-      GMGS_HIP(hipMemcpyAsync(pinned, bg->dflags.as<int>(), sizeof(int) * act.size(), hipMemcpyDeviceToHost, s0));
+      GMGS_HIP(hipMemcpyAsync(pinned.get(), bg->dflags.as<int>(), sizeof(int) * act.size(), hipMemcpyDeviceToHost,
+                              s0));
       GMGS_HIP(hipEventRecord(ev[n + 1], s0));
       GMGS_HIP(hipEventSynchronize(ev[n + 1]));
       BatchKey left;
@@ -1926,11 +1938,6 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
       if (st != GMGS_OK && status == GMGS_OK) status = st;
       if (U && U[i]) copy_solution(c, U[i]);
     }
-    GMGS_HIP(hipEventDestroy(t0));
-    GMGS_HIP(hipEventDestroy(t1));
-    for (auto& e : ev) GMGS_HIP(hipEventDestroy(e));
-    GMGS_HIP(hipHostFree(pinned));
-    GMGS_HIP(hipStreamDestroy(s0));
     return status;
   });
 }
@@ -1973,6 +1980,8 @@ int gmgs_adjoint(gmgs_ctx* c, const double* g, const double* U, double* lambda, 
     hipStream_t s = c->stream;
     Level& l = L1(c);
     const long long nn = l.mesh.nn, N = l.mesh.g.N;
+    for (long long i = 0; i < 3 * nn; ++i)
+      if (!std::isfinite(g[i])) throw Status(GMGS_ERR_NONFINITE, "non-finite objective gradient g");
     // forward displacement (dense) into tmp2
     if (U) {
       DevBuf dU;
@@ -2000,12 +2009,27 @@ int gmgs_adjoint(gmgs_ctx* c, const double* g, const double* U, double* lambda, 
           if ((fl[np[k]] >> (F_FIX_SHIFT + cc)) & 1u) gz[3 * k + cc] = 0.0;
     }
     upload(dg, gz.data(), gz.size(), s);
-    launch_fill<double>(c->b.as<double>(), 0.0, 3 * N, s);
-    launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dg.as<double>(), c->b.as<double>(), N, s);
-    const int st = solve_into(c, info, nullptr, 0);
-    // restore the load vector; the adjoint solution is in x
-    GMGS_HIP(hipMemcpyAsync(c->b.as<double>(), saved.as<double>(), sizeof(double) * 3 * N,
-                            hipMemcpyDeviceToDevice, s));
+    // b holds g until this scope ends, on every exit path: a thrown solve
+    // error must not leave later solves working on K u = g
+    struct RestoreLoads {
+      gmgs_ctx* c;
+      const DevBuf& saved;
+      long long n;
+      ~RestoreLoads() {
+        (void)hipMemcpyAsync(c->b.as<double>(), saved.as<double>(), sizeof(double) * n,
+                             hipMemcpyDeviceToDevice, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+      }
+    };
+    c->have_solution = false;
+    int st;
+    {
+      RestoreLoads restore{c, saved, 3 * N};
+      launch_fill<double>(c->b.as<double>(), 0.0, 3 * N, s);
+      launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, dg.as<double>(), c->b.as<double>(), N, s);
+      st = solve_into(c, info, nullptr, 0);
+    }
+    // the adjoint solution is in x (not a solution of K u = F)
     c->have_solution = false;
     if (lambda) {
       DevBuf comp;
@@ -2176,14 +2200,15 @@ int gmgs_vcycle(gmgs_ctx* c, const double* r, double* z) {
 int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count) {
   if (!c || c->lev.empty() || !count) return GMGS_ERR_ARGUMENT;
   return guarded(c, [&]() -> int {
-    if (key != GMGS_Q_NUM_LEVELS && key != GMGS_Q_KE0 && key != GMGS_Q_BBOX) check_level(c, level);
+    if (key != GMGS_Q_NUM_LEVELS && key != GMGS_Q_KE0 && key != GMGS_Q_BBOX && key != GMGS_Q_MAX_IT)
+      check_level(c, level);
     const Level* l = (level >= 1 && level <= c->L) ? &c->lev[level - 1] : nullptr;
     const Grid* g = l ? &l->mesh.g : nullptr;
     int64_t n = 0;
     switch (key) {
       case GMGS_Q_NUM_LEVELS: case GMGS_Q_NUM_ELEMENTS: case GMGS_Q_NUM_NODES:
       case GMGS_Q_INTERMEDIATE_NUM_NODES: case GMGS_Q_STORING_STATE: case GMGS_Q_NUM_TILES:
-      case GMGS_Q_NUM_FREE_COARSEST:
+      case GMGS_Q_NUM_FREE_COARSEST: case GMGS_Q_MAX_IT:
         n = 1; break;
       case GMGS_Q_TILE_CONFIG: n = 5; break;
       case GMGS_Q_RES: n = 3; break;
@@ -2201,6 +2226,11 @@ int gmgs_query_size(gmgs_ctx* c, int key, int level, int64_t* count) {
       case GMGS_Q_KS: n = 576 * l->mesh.ne; break;
       case GMGS_Q_DIAGK: case GMGS_Q_F: n = 3 * l->mesh.nn; break;
       case GMGS_Q_FIXED_DOFS: case GMGS_Q_FREE_DOFS: {
+        if (!c->bc) throw Status(GMGS_ERR_STATE, "boundary conditions not set");
+        if (key == GMGS_Q_FIXED_DOFS && level == 1) {   // fixingCond_ order, duplicates kept
+          n = 3 * (int64_t)c->bc_fixed.size();
+          break;
+        }
         std::vector<uint32_t> fl = download<uint32_t>(l->mesh.flags, (size_t)g->N, c->stream);
         std::vector<long long> np = download<long long>(l->mesh.node_pidx, (size_t)l->mesh.nn, c->stream);
         int64_t nf = 0;
@@ -2249,6 +2279,7 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
       case GMGS_Q_STORING_STATE: i64[0] = level == 1 ? 0 : l->state; break;
       case GMGS_Q_NUM_TILES: i64[0] = l->ntiles; break;
       case GMGS_Q_NUM_FREE_COARSEST: i64[0] = c->nfree; break;
+      case GMGS_Q_MAX_IT: i64[0] = c->prm.max_it; break;
       case GMGS_Q_TILE_CONFIG:
         i64[0] = l->split ? 3 : (l->zm ? 1 : (l->as ? 2 : 0)); i64[1] = l->col_tx; i64[2] = l->col_zc;
         // the FP64 matvec of the PCG runs on column tiles of its own when tuned
@@ -2294,6 +2325,13 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
         break;
       }
       case GMGS_Q_FIXED_DOFS: case GMGS_Q_FREE_DOFS: {
+        if (key == GMGS_Q_FIXED_DOFS && level == 1) {
+          // fixedDOFs = reshape([3f-2 3f-1 3f]', [], 1) over fixingCond_
+          // (ApplyBoundaryCondition.m:15-17): interleaved, in input order
+          for (size_t k = 0; k < c->bc_fixed.size(); ++k)
+            for (int cc = 0; cc < 3; ++cc) i32[3 * k + cc] = 3 * c->bc_fixed[k] - 2 + cc;
+          break;
+        }
         std::vector<uint32_t> fl = download<uint32_t>(l->mesh.flags, (size_t)l->mesh.g.N, s);
         std::vector<long long> np = download<long long>(l->mesh.node_pidx, (size_t)l->mesh.nn, s);
         int64_t j = 0;
@@ -2324,16 +2362,13 @@ int gmgs_copy_probe(int device, int64_t nbytes, int reps, double* avg_ms) {
   if (nbytes < 16 || reps < 1 || !avg_ms) return GMGS_ERR_ARGUMENT;
   return guarded(nullptr, [&]() -> int {
     GMGS_HIP(hipSetDevice(device));
-    hipStream_t s;
-    GMGS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HipStream s;
     DevBuf a, b;
     a.alloc_zero((size_t)nbytes, s);
     b.alloc_zero((size_t)nbytes, s);
     const long long nd = nbytes / 8;
     launch_copy_probe(a.as<double>(), b.as<double>(), nd, s);
-    hipEvent_t e0, e1;
-    GMGS_HIP(hipEventCreate(&e0));
-    GMGS_HIP(hipEventCreate(&e1));
+    HipEvent e0, e1;
     GMGS_HIP(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i) launch_copy_probe(a.as<double>(), b.as<double>(), nd, s);
     GMGS_HIP(hipEventRecord(e1, s));
@@ -2341,9 +2376,6 @@ int gmgs_copy_probe(int device, int64_t nbytes, int reps, double* avg_ms) {
     float ms = 0;
     GMGS_HIP(hipEventElapsedTime(&ms, e0, e1));
     *avg_ms = ms / reps;
-    GMGS_HIP(hipEventDestroy(e0));
-    GMGS_HIP(hipEventDestroy(e1));
-    GMGS_HIP(hipStreamDestroy(s));
     return GMGS_OK;
   });
 }
@@ -2385,9 +2417,7 @@ int gmgs_time_operator(gmgs_ctx* c, int kernel, int reps, double* avg_ms) {
                                     c->tmp.as<double>(), nullptr, nullptr, nullptr);
     };
     launch();
-    hipEvent_t e0, e1;
-    GMGS_HIP(hipEventCreate(&e0));
-    GMGS_HIP(hipEventCreate(&e1));
+    HipEvent e0, e1;
     GMGS_HIP(hipEventRecord(e0, c->stream));
     for (int i = 0; i < reps; ++i) launch();
     GMGS_HIP(hipEventRecord(e1, c->stream));
@@ -2395,8 +2425,6 @@ int gmgs_time_operator(gmgs_ctx* c, int kernel, int reps, double* avg_ms) {
     float ms = 0.f;
     GMGS_HIP(hipEventElapsedTime(&ms, e0, e1));
     *avg_ms = ms / reps;
-    GMGS_HIP(hipEventDestroy(e0));
-    GMGS_HIP(hipEventDestroy(e1));
     return GMGS_OK;
   });
 }

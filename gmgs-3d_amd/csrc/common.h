@@ -132,4 +132,49 @@ class DevBuf {
   size_t bytes_ = 0;
 };
 
+// Owning HIP event, stream and pinned host buffer: released on every exit
+// path, including a HipError thrown between creation and use.
+class HipEvent {
+ public:
+  explicit HipEvent(unsigned flags = hipEventDefault) { GMGS_HIP(hipEventCreateWithFlags(&e_, flags)); }
+  HipEvent(const HipEvent&) = delete;
+  HipEvent& operator=(const HipEvent&) = delete;
+  ~HipEvent() { if (e_) (void)hipEventDestroy(e_); }
+  operator hipEvent_t() const { return e_; }
+
+ private:
+  hipEvent_t e_ = nullptr;
+};
+
+class HipStream {
+ public:
+  HipStream() { GMGS_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
+  HipStream(const HipStream&) = delete;
+  HipStream& operator=(const HipStream&) = delete;
+  ~HipStream() {
+    if (s_) {
+      (void)hipStreamSynchronize(s_);
+      (void)hipStreamDestroy(s_);
+    }
+  }
+  operator hipStream_t() const { return s_; }
+
+ private:
+  hipStream_t s_ = nullptr;
+};
+
+template <typename T>
+class PinnedBuf {
+ public:
+  explicit PinnedBuf(size_t n) { GMGS_HIP(hipHostMalloc((void**)&p_, sizeof(T) * (n ? n : 1), hipHostMallocDefault)); }
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { if (p_) (void)hipHostFree(p_); }
+  T* get() const { return p_; }
+  T& operator[](size_t i) const { return p_[i]; }
+
+ private:
+  T* p_ = nullptr;
+};
+
 }  // namespace gmgs

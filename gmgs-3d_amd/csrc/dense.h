@@ -17,7 +17,8 @@ void assemble_coarsest(const Grid& g, const uint32_t* flags, const int* eop, con
                        const double* Ks, const long long* code, const int* dofmap, long long nfree,
                        long long nfp, double* A, hipStream_t s);
 
-// in place lower Cholesky; returns 0 or the 1-based failing pivot
+// in place lower Cholesky; returns 0 or the 1-based first pivot that is not
+// above 1e-12 x that row's diagonal before factoring (singular or indefinite)
 int cholesky(double* A, long long nfp, hipStream_t s);
 
 // Kinv = inv(A) from the factor L in A (uses W as workspace for L^-1)

@@ -68,7 +68,20 @@ __global__ void k_assemble(Grid g, const uint32_t* __restrict__ flags, const int
   }
 }
 
+// A pivot at or below this fraction of its row's diagonal before
+// factoring marks the matrix singular (a rigid-body mode left free, e.g. a
+// solid component without a fixed node): exactly singular modes leave
+// pivots at rounding level (~1e-16), while a constrained body's scaled
+// stiffness keeps its pivots above 1/cond(D^-1/2 K D^-1/2).
+constexpr double kPivotTol = 1e-12;
+
+__global__ void k_diag_copy(const double* __restrict__ A, long long n, double* __restrict__ d) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = A[i * n + i];
+}
+
 __global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, long long n, int k,
+                                                    const double* __restrict__ diag0,
                                                     int* __restrict__ info) {
   __shared__ double L[DB][LD];
   const int tid = threadIdx.x;
@@ -81,7 +94,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, long
   for (int j = 0; j < DB; ++j) {
     if (tid == 0) {
       double d = L[j][j];
-      if (!(d > 0.0)) {
+      if (!(d > kPivotTol * diag0[o + j])) {
         if (*info == 0) *info = (int)(o + j + 1);
         d = 1.0;
       }
@@ -325,10 +338,13 @@ void assemble_coarsest(const Grid& g, const uint32_t* flags, const int* eop, con
 
 int cholesky(double* A, long long nfp, hipStream_t s) {
   const int nb = (int)(nfp / DB);
-  DevBuf info;
+  DevBuf info, diag0;
   info.alloc_zero(sizeof(int), s);
+  diag0.alloc(sizeof(double) * nfp);
+  k_diag_copy<<<grid1d(nfp, BS), BS, 0, s>>>(A, nfp, diag0.as<double>());
+  GMGS_LAUNCH_CHECK();
   for (int k = 0; k < nb; ++k) {
-    k_potrf_diag<<<1, BS, 0, s>>>(A, nfp, k, info.as<int>());
+    k_potrf_diag<<<1, BS, 0, s>>>(A, nfp, k, diag0.as<double>(), info.as<int>());
     GMGS_LAUNCH_CHECK();
     const int m = nb - k - 1;
     if (m > 0) {

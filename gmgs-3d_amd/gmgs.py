@@ -43,10 +43,10 @@ Q = dict(NUM_LEVELS=0, RES=1, NUM_ELEMENTS=2, NUM_NODES=3, ELE_MAP_BACK=4, ELE_M
          ELEMENTS_ON_BOUNDARY=11, ELEMENT_UPWARD_MAP=12, TRANSFER_MAT=13, TRANSFER_MAT_COEFFI=14,
          SOLID_NODE_MAP_C2F=15, INTERMEDIATE_NUM_NODES=16, STORING_STATE=17, UNIQUE_ELEMENTS=18,
          KS=19, DIAGK=20, FIXED_DOFS=21, FREE_DOFS=22, KE0=23, BBOX=24, F=25, NUM_TILES=26,
-         NUM_FREE_COARSEST=27, TILE_CONFIG=28, SMOOTHER=29)
+         NUM_FREE_COARSEST=27, TILE_CONFIG=28, SMOOTHER=29, MAX_IT=30)
 _Q_DOUBLE = {Q["TRANSFER_MAT_COEFFI"], Q["KS"], Q["DIAGK"], Q["KE0"], Q["BBOX"], Q["F"], Q["SMOOTHER"]}
 _Q_INT64 = {Q["NUM_LEVELS"], Q["RES"], Q["NUM_ELEMENTS"], Q["NUM_NODES"], Q["INTERMEDIATE_NUM_NODES"],
-            Q["STORING_STATE"], Q["NUM_TILES"], Q["NUM_FREE_COARSEST"], Q["TILE_CONFIG"]}
+            Q["STORING_STATE"], Q["NUM_TILES"], Q["NUM_FREE_COARSEST"], Q["TILE_CONFIG"], Q["MAX_IT"]}
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "insufficient resolution", 3: "fewer than two levels",
           4: "no loads", 5: "no constraints", 6: "node id out of range", 7: "non-finite input",

@@ -107,22 +107,24 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
       check(c, gmgs_set_bc(c, fx, (int64_t)k, mxGetPr(prhs[3]), (int64_t)mxGetM(prhs[3])));
       mxFree(fx);
     } else if (!strcmp(cmd, "solve")) {
-      int64_t nn = 0, i;
+      int64_t nn = 0, maxit = 0, cap, rows, i;
       gmgs_solve_info info;
       double *hist, *H;
-      gmgs_params p;
       gmgs_query(c, GMGS_Q_NUM_NODES, 1, &nn, 1);
+      check(c, gmgs_query(c, GMGS_Q_MAX_IT, 1, &maxit, 1));
       plhs[0] = mxCreateDoubleMatrix((mwSize)(3 * nn), 1, mxREAL);
-      gmgs_default_params(&p);
-      hist = (double*)mxMalloc(sizeof(double) * 3 * (20000 + 2));
-      check(c, gmgs_solve(c, mxGetPr(plhs[0]), &info, hist, 20000 + 2));
+      /* iterationHist_ has at most maxIT_ + 1 rows (CG_solver.m:17,49) */
+      cap = maxit + 2;
+      hist = (double*)mxMalloc(sizeof(double) * 3 * (size_t)cap);
+      check(c, gmgs_solve(c, mxGetPr(plhs[0]), &info, hist, cap));
+      rows = info.hist_len < cap ? info.hist_len : cap;
       if (nlhs > 1) {
-        plhs[1] = mxCreateDoubleMatrix((mwSize)info.hist_len, 3, mxREAL);
+        plhs[1] = mxCreateDoubleMatrix((mwSize)rows, 3, mxREAL);
         H = mxGetPr(plhs[1]);
-        for (i = 0; i < info.hist_len; ++i) {
+        for (i = 0; i < rows; ++i) {
           H[i] = hist[3 * i];
-          H[i + info.hist_len] = hist[3 * i + 1];
-          H[i + 2 * info.hist_len] = hist[3 * i + 2];
+          H[i + rows] = hist[3 * i + 1];
+          H[i + 2 * rows] = hist[3 * i + 2];
         }
       }
       if (nlhs > 2) {

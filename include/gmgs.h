@@ -132,7 +132,11 @@ enum {
   GMGS_Q_UNIQUE_ELEMENTS = 18,      /* int32[n] (storing state 2) */
   GMGS_Q_KS = 19,                   /* double[numElements x 576]: each element's 24x24 matrix */
   GMGS_Q_DIAGK = 20,                /* double[numDOFs] (levels < numLevels) */
-  GMGS_Q_FIXED_DOFS = 21,           /* int32[n] sorted 1-based */
+  GMGS_Q_FIXED_DOFS = 21,           /* int32[n] 1-based, after gmgs_set_bc.  Level 1: the
+                                       reference's fixedDOFs, [3f-2 3f-1 3f] per fixingCond_
+                                       entry f in input order, duplicates kept
+                                       (ApplyBoundaryCondition.m:15-17); levels >= 2: sorted
+                                       (SetupSolver.m:250-253) */
   GMGS_Q_FREE_DOFS = 22,            /* int32[n] sorted 1-based */
   GMGS_Q_KE0 = 23,                  /* double[576] */
   GMGS_Q_BBOX = 24,                 /* double[6] adjusted bounding box [xmin ymin zmin xmax ymax zmax] */
@@ -145,9 +149,11 @@ enum {
                                        z-marching tile TX, z-chunk, then TX and z-chunk of the
                                        FP64 matvec of the PCG (level 1 of a mixed context tunes
                                        its own; the level's otherwise) */
-  GMGS_Q_SMOOTHER = 29              /* double[5] (levels < numLevels, after gmgs_set_bc):
+  GMGS_Q_SMOOTHER = 29,             /* double[5] (levels < numLevels, after gmgs_set_bc):
                                        lambda_max(D^-1 A) estimate (0 = none), the weights of the
                                        level's nu sweeps, 0 beyond nu */
+  GMGS_Q_MAX_IT = 30                /* int64 maxIT_ of the context (sizes iterationHist_:
+                                       at most maxIT_ + 1 rows) */
 };
 
 void gmgs_default_params(gmgs_params* p);

@@ -25,6 +25,50 @@ def holey_brick(seed=7, shape=(10, 12, 16)):
     return vox, np.array([[0, 0, 0], [nx, ny, nz]], float)
 
 
+def b3_brick(seed=0, shape=(16, 16, 24)):
+    """Brick whose interior loses random 2x2x2 blocks on the level-2 grid and
+    scattered single voxels: with 4 levels, some level-3 elements then have a
+    void level-2 child while their present children are full, so the
+    reference's level-3 test (SetupSolver.m:96-101) keeps them universal
+    (17 such elements for seed 0)."""
+    rng = np.random.default_rng(seed)
+    vox = np.ones(shape, np.int32)
+    blocks = rng.random((shape[0] // 2, shape[1] // 2, shape[2] // 2)) < 0.08
+    inner = np.kron(blocks, np.ones((2, 2, 2), bool)).astype(bool)
+    inner[:2] = inner[-2:] = False
+    inner[:, :2] = inner[:, -2:] = False
+    inner[:, :, :2] = inner[:, :, -2:] = False
+    vox[inner] = 0
+    sc = rng.random(shape) < 0.01
+    sc[0] = sc[-1] = False
+    sc[:, 0] = sc[:, -1] = False
+    sc[:, :, 0] = sc[:, :, -1] = False
+    vox[sc] = 0
+    return vox, np.array([[0, 0, 0], [shape[1], shape[0], shape[2]]], float)
+
+
+def level3_b3_by_coordinates(vox):
+    """Compact ids of the level-3 elements (4 levels) that have an entirely
+    void level-2 child while every present level-2 child is full: unique in
+    exact Galerkin terms, universal in the reference's test."""
+    ny, nx, nz = (d // 4 for d in vox.shape)
+    cells = []
+    for gz in range(nz):
+        for gx in range(nx):
+            for gy in range(ny):
+                c = vox[4 * gy:4 * gy + 4, 4 * gx:4 * gx + 4, 4 * gz:4 * gz + 4]
+                if c.any():
+                    cells.append((gy + ny * gx + ny * nx * gz, c))
+    cells.sort(key=lambda t: t[0])
+    out = []
+    for k, (_, c) in enumerate(cells):
+        kids = [c[2 * a:2 * a + 2, 2 * b:2 * b + 2, 2 * d:2 * d + 2]
+                for a in (0, 1) for b in (0, 1) for d in (0, 1)]
+        if not c.all() and not any(kd.any() and not kd.all() for kd in kids):
+            out.append(k + 1)
+    return np.array(out, dtype=np.int64)
+
+
 def bc_bottom_top(oracle_model, nz, seed=3):
     rng = np.random.default_rng(seed)
     L1 = oracle_model.levels[0]
@@ -42,6 +86,7 @@ CASES = {
     "holey_L4": dict(kind="holey", L=4, seed=11, shape=(16, 16, 24)),
     "ellip_L4": dict(kind="ellip", L=4, seed=4, shape=(16, 16, 24)),
     "ellip_L2": dict(kind="ellip", L=2, seed=5, shape=(12, 10, 8)),
+    "b3_L4": dict(kind="b3", L=4, seed=0, shape=(16, 16, 24)),
 }
 
 
@@ -49,8 +94,8 @@ CASES = {
 def case(request, oracle, pkg):
     spec = CASES[request.param]
     log(f"[case] {request.param} building")
-    if spec["kind"] == "holey":
-        vox, bbox = holey_brick(spec["seed"], spec["shape"])
+    if spec["kind"] in ("holey", "b3"):
+        vox, bbox = (holey_brick if spec["kind"] == "holey" else b3_brick)(spec["seed"], spec["shape"])
         m = oracle.build(vox, bbox, num_levels=spec["L"], omega=0.5)
         fixed, loads = bc_bottom_top(m, spec["shape"][2])
         omega = 0.5
@@ -115,12 +160,48 @@ def test_element_matrices_and_states(case):
         assert rel(Ks, ref) < 1e-12
 
 
+def test_level3_b3_elements(case, oracle, pkg):
+    """Where the reference's level-3 test misclassifies (B3), the device keeps
+    the reference's unique list and universal 4 Ke0 for those elements, and
+    with GMGS_FLAG_EXACT_GALERKIN lists them as unique with their Galerkin
+    matrices; both against the oracle."""
+    if case["name"] != "b3_L4":
+        pytest.skip("B3 fires only in the b3_L4 mask")
+    m, s = case["m"], case["s"]
+    b3 = level3_b3_by_coordinates(case["vox"])
+    log(f"  B3 elements by coordinates: {b3.size}")
+    assert b3.size == 17
+    dev_ref = s.query("UNIQUE_ELEMENTS", 3)
+    assert np.array_equal(dev_ref, m.levels[2].uniqueElements)
+    assert not np.isin(b3, dev_ref).any()
+    Ks = s.query("KS", 3).reshape(-1, 24, 24)
+    assert rel(Ks[b3 - 1], np.broadcast_to(4.0 * m.Ke0, (b3.size, 24, 24))) < 1e-13
+    se = pkg.Solver(case["vox"], case["bbox"], num_levels=4, jacobi_weight=case["omega"], tol=1e-8,
+                    max_it=2000, flags=pkg.FLAG_EXACT_GALERKIN)
+    se.set_bc(case["fixed"], case["loads"])
+    me = oracle.build(case["vox"], case["bbox"], num_levels=4, omega=case["omega"], exact_galerkin=True)
+    oracle.apply_bc(me, case["loads"], case["fixed"])
+    oracle.factor_coarsest(me)
+    dev_ex = se.query("UNIQUE_ELEMENTS", 3)
+    assert np.array_equal(dev_ex, me.levels[2].uniqueElements)
+    assert np.array_equal(np.setdiff1d(dev_ex, dev_ref), b3)
+    lev = me.levels[2]
+    st = np.zeros(lev.numElements, dtype=np.int64)
+    st[lev.uniqueElements - 1] = np.arange(1, lev.uniqueElements.size + 1)
+    assert rel(se.query("KS", 3).reshape(-1, 24, 24), lev.Ks[st]) < 1e-12
+    r = np.random.default_rng(8).standard_normal(me.levels[0].numDOFs)
+    assert rel(se.vcycle(r), oracle.vcycle(me, r)) < 1e-9
+    se.close()
+
+
 def test_diagonal_and_fixed_dofs(case):
     m, s = case["m"], case["s"]
     for li in range(m.numLevels):
         lev = m.levels[li]
+        # level 1: the reference's interleaved fixingCond_ order, duplicates
+        # kept (ApplyBoundaryCondition.m:15-17); coarse levels: sorted setdiff
         fx = s.query("FIXED_DOFS", li + 1)
-        assert np.array_equal(fx, np.unique(lev.fixedDOFs))
+        assert np.array_equal(fx, lev.fixedDOFs)
         assert np.array_equal(s.query("FREE_DOFS", li + 1), lev.freeDOFs)
         if li < m.numLevels - 1:
             assert rel(s.query("DIAGK", li + 1), lev.diagK) < 1e-13

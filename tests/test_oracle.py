@@ -160,6 +160,27 @@ def brute_force_hierarchy(vox, L):
         enod = np.array([[ncompact[fy(inv[e][0] + a, inv[e][1] + b, inv[e][2] + c, ey + 1, ex + 1)]
                           for (a, b, c) in locl] for e in emb], dtype=np.int64).reshape(-1, 8)
         rec = dict(eleMapBack=np.array(emb), numNodes=len(nmb), eNodMat=enod, ecompact=ecompact)
+        # forward maps, incident-element counts and boundary lists from
+        # coordinates (DiscretizeDesignDomain.m:68-72,86-89,103-116)
+        efwd = np.zeros(ey * ex * ez, dtype=np.int64)
+        for i, e in enumerate(emb):
+            efwd[e - 1] = i + 1
+        nfwd = np.zeros((ey + 1) * (ex + 1) * (ez + 1), dtype=np.int64)
+        for i, n in enumerate(nmb):
+            nfwd[n - 1] = i + 1
+        cnt = np.zeros(len(nmb), dtype=np.int64)
+        for i, n in enumerate(nmb):
+            q = n - 1
+            y, x, z = q % (ey + 1), (q // (ey + 1)) % (ex + 1), q // ((ey + 1) * (ex + 1))
+            cnt[i] = sum((y - a, x - b, z - c) in solid
+                         for a in (0, 1) for b in (0, 1) for c in (0, 1))
+        onb = {i + 1 for i in range(len(nmb)) if cnt[i] < 8}
+        eob = [i + 1 for i, e in enumerate(emb)
+               if any(ncompact[fy(inv[e][0] + a, inv[e][1] + b, inv[e][2] + c, ey + 1, ex + 1)] in onb
+                      for (a, b, c) in locl)]
+        rec.update(eleMapForward=efwd, nodMapForward=nfwd, numNod2Eles=cnt,
+                   nodesOnBoundary=np.array(sorted(onb), dtype=np.int64),
+                   elementsOnBoundary=np.array(eob, dtype=np.int64))
         s = 2 ** (lev - 1)
         Y1, X1 = dims[0] * s + 1, dims[1] * s + 1
 
@@ -214,6 +235,9 @@ def test_hierarchy_brute_force(oracle, shape, L, seed):
         assert lev.numNodes == ref["numNodes"]
         assert np.array_equal(lev.eNodMat, ref["eNodMat"])
         assert np.array_equal(lev.nodMapBack, ref["nodMapBack"])
+        for key in ("eleMapForward", "nodMapForward", "numNod2Eles", "nodesOnBoundary",
+                    "elementsOnBoundary"):
+            assert np.array_equal(np.asarray(getattr(lev, key)).ravel(), ref[key]), key
         if "elementUpwardMap" in ref:
             assert np.array_equal(lev.elementUpwardMap, ref["elementUpwardMap"])
             assert np.array_equal(lev.transferMat, ref["transferMat"])
@@ -541,3 +565,136 @@ def test_chebyshev_cycle_cg_converges_faster(oracle):
                                cycle=lambda mm, r: oracle.vcycle_sym(mm, r, 2, w))
         assert che.converged and (che.its < jac.its if better else che.its <= jac.its)
         assert np.linalg.norm(che.x - jac.x) / np.linalg.norm(jac.x) < 1e-8
+
+
+# ---- B3: the level-3 "unique element" test (SetupSolver.m:94-101) ----------
+
+def level3_classes_by_coordinates(vox, L):
+    """Level-3 elements of a padded mask (L > 3), classified from voxel
+    coordinates: a level-3 cell covers 4x4x4 voxels and its 8 level-2 children
+    2x2x2 each.  Returns (exact, reference) sets of compact ids: exact = some
+    voxel of the cell is void (its Galerkin operator differs from 4 Ke0);
+    reference = some *present* level-2 child holds a void voxel (children that
+    are entirely void are skipped, SetupSolver.m:97-98)."""
+    vox = np.asarray(vox)
+    ny, nx, nz = (d // 4 for d in vox.shape)
+    full_ids, exact, ref = [], set(), set()
+    for gz in range(nz):
+        for gx in range(nx):
+            for gy in range(ny):
+                cell = vox[4 * gy:4 * gy + 4, 4 * gx:4 * gx + 4, 4 * gz:4 * gz + 4]
+                if cell.any():
+                    full_ids.append((gy + ny * gx + ny * nx * gz + 1, cell))
+    full_ids.sort(key=lambda t: t[0])
+    for k, (_, cell) in enumerate(full_ids):
+        kids = [cell[2 * a:2 * a + 2, 2 * b:2 * b + 2, 2 * c:2 * c + 2]
+                for a in (0, 1) for b in (0, 1) for c in (0, 1)]
+        if not cell.all():
+            exact.add(k + 1)
+        if any(kid.any() and not kid.all() for kid in kids):
+            ref.add(k + 1)
+    return exact, ref
+
+
+def b3_mask():
+    """16^3 solid with, inside level-3 cell (1,1,1), one level-2 child left
+    entirely void (B3 fires: its present children are all full), and in cell
+    (2,2,2) a single void voxel (unique either way)."""
+    vox = np.ones((16, 16, 16), np.int32)
+    vox[4:6, 4:6, 4:6] = 0
+    vox[9, 10, 11] = 0
+    return vox
+
+
+@pytest.mark.parametrize("kind,seed", [("designed", 0), ("random", 1), ("random", 2)])
+def test_level3_classification_b3_by_coordinates(oracle, kind, seed):
+    if kind == "designed":
+        vox = b3_mask()
+    else:
+        rng = np.random.default_rng(seed)
+        vox = np.ones((16, 16, 24), np.int32)
+        # void 2x2x2 blocks on the level-2 grid and scattered void voxels
+        blocks = rng.random((8, 8, 12)) < 0.08
+        vox[np.kron(blocks, np.ones((2, 2, 2), bool)).astype(bool)] = 0
+        vox[rng.random(vox.shape) < 0.01] = 0
+        vox[0, 0, 0] = 1
+    bbox = np.array([[0, 0, 0], [vox.shape[1], vox.shape[0], vox.shape[2]]], float)
+    exact_ids, ref_ids = level3_classes_by_coordinates(vox, 4)
+    b3 = exact_ids - ref_ids
+    assert ref_ids <= exact_ids
+    if kind == "designed":
+        assert b3 == {1 + 1 + 4 * 1 + 16 * 1}     # cell (1,1,1) of the 4^3 level-3 grid
+    assert len(b3) >= 1
+    for exact in (False, True):
+        m = oracle.build(vox, bbox, num_levels=4, exact_galerkin=exact)
+        lev3 = m.levels[2]
+        assert lev3.storingState == 2
+        got = set(int(v) for v in lev3.uniqueElements)
+        assert got == (exact_ids if exact else ref_ids)
+        # a B3 element is treated as universal: 4 Ke0 (SetupSolver.m:105) in
+        # the reference, its Galerkin product (not 4 Ke0) when exact
+        st = np.zeros(lev3.numElements + 1, dtype=np.int64)
+        st[lev3.uniqueElements] = np.arange(1, lev3.uniqueElements.size + 1)
+        Ke3 = 4.0 * m.Ke0
+        assert np.abs(lev3.Ks[0] - Ke3).max() < 1e-13
+        for e in b3:
+            K = lev3.Ks[st[e]]
+            if exact:
+                assert st[e] > 0 and np.abs(K - Ke3).max() > 1e-3
+            else:
+                assert st[e] == 0
+
+
+# ---- B2: error propagation of the reference V-cycle (Vcycle.m:1-36) ---------
+
+@pytest.mark.parametrize("L,shape", [(2, (4, 4, 6)), (3, (8, 4, 8))])
+def test_reference_vcycle_operator_identity(oracle, L, shape):
+    """The reference cycle (x_l = S r; r_l = r - A S r; coarse correction;
+    post-smoothing with the stale r_l, Vcycle.m:12-33 with JacobiSmoother.m)
+    is the linear map M = Z B_1 with
+        B_l = 2 S_l - S_l A_l S_l + P_l B_{l+1} R_l (I - A_l S_l),  B_L = C,
+    S_l = omega D_l^-1, R_l = P_l^T, C = K_ff^-1 of the coarsest level (zero
+    on fixed DOFs), A_l the assembled operator with fixed rows zeroed
+    (AtX.m:57) and Z zeroing the level-1 fixed DOFs (Vcycle.m:35).  Every
+    matrix here comes from an independent assembly (scipy sparse from the
+    element matrices, a dense inverse of the coarsest K_ff, the diagonal of
+    the assembly), not from the oracle's cycle code; a post-smoothing with
+    the updated residual, a dropped pre-smoothed x or a wrong level's
+    diagonal changes M by O(1)."""
+    m = holey_brick(oracle, L=L, shape=shape, seed=3)
+    n = [lev.numDOFs for lev in m.levels]
+
+    def unit(k, size):
+        v = np.zeros(size)
+        v[k] = 1.0
+        return v
+
+    A, S = [], []
+    for ii in range(1, L + 1):
+        lev = m.levels[ii - 1]
+        K = (oracle.assemble_level1(m) if ii == 1 else assemble_level(m, ii)).toarray()
+        Kz = K.copy()
+        Kz[lev.fixedDOFs - 1, :] = 0.0
+        A.append(Kz)
+        S.append(m.omega * np.diag(1.0 / K.diagonal()))
+    P = [None] + [np.stack([oracle.interpolate(m, unit(k, n[ii - 1]), ii) for k in range(n[ii - 1])], axis=1)
+                  for ii in range(2, L + 1)]
+    Kc = assemble_level(m, L).toarray()
+    fr = m.levels[-1].freeDOFs - 1
+    C = np.zeros_like(Kc)
+    C[np.ix_(fr, fr)] = np.linalg.inv(Kc[np.ix_(fr, fr)])
+    B = C
+    for ii in range(L - 1, 0, -1):
+        Sl, Al, Pl = S[ii - 1], A[ii - 1], P[ii]
+        I = np.eye(n[ii - 1])
+        B = 2 * Sl - Sl @ Al @ Sl + Pl @ B @ Pl.T @ (I - Al @ Sl)
+    Z = np.ones(n[0])
+    Z[m.levels[0].fixedDOFs - 1] = 0.0
+    M_formula = Z[:, None] * B
+    M = np.stack([oracle.vcycle(m, unit(j, n[0])) for j in range(n[0])], axis=1)
+    assert np.abs(M - M_formula).max() <= 1e-10 * np.abs(M_formula).max()
+    # the transpose relation the formula uses: restriction is P^T
+    rng = np.random.default_rng(0)
+    for ii in range(2, L + 1):
+        rf = rng.standard_normal(n[ii - 2])
+        assert np.abs(oracle.restrict(m, rf, ii) - P[ii - 1].T @ rf).max() < 1e-13 * np.abs(rf).max() * 8
