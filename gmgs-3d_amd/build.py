@@ -16,7 +16,7 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libgmgs.so"
 SOURCES = ["fe_host.cpp", "mesh.hip", "ops.hip", "ops_zm.hip", "setup.hip", "dense.hip",
-           "capi.cpp"]
+           "capi.cpp", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GMGS_OFFLOAD_ARCH", "gfx950")

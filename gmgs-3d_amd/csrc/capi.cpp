@@ -1571,6 +1571,45 @@ void check_level(gmgs_ctx* c, int level) {
 
 }  // namespace
 
+// AtX of one level on device-resident compact vectors, with the two padded
+// level vectors it needs taken from caller-provided device scratch (the
+// entry point that allocates nothing passes its file-scope device array):
+// scratch(device, &bytes) returns that device's scratch address and size.
+// Nothing here allocates: the operator data belong to the context, x and y to
+// the caller.
+namespace gmgs {
+int atx_device_scratch(gmgs_ctx* c, int level, const double* x, double* y,
+                       void* (*scratch)(int device, size_t* bytes)) {
+  if (!c || c->lev.empty() || !x || !y || !scratch) return GMGS_ERR_ARGUMENT;
+  return guarded(c, [&]() -> int {
+    GMGS_HIP(hipSetDevice(c->device));
+    check_level(c, level);
+    if (!c->bc) throw Status(GMGS_ERR_STATE, "boundary conditions not set (fixed DOFs unknown)");
+    hipStream_t s = c->stream;
+    Level& l = c->lev[level - 1];
+    const long long nn = l.mesh.nn, N = l.mesh.g.N;
+    size_t have = 0;
+    double* base = static_cast<double*>(scratch(c->device, &have));
+    if (!base) throw Status(GMGS_ERR_DEVICE, "device scratch array not found on this device");
+    const size_t need = sizeof(double) * 6 * (size_t)N;
+    if (need > have)
+      throw Status(GMGS_ERR_ARGUMENT, "level " + std::to_string(level) + " needs " + std::to_string(need)
+                                          + " bytes of scratch (two padded level vectors), more than the "
+                                          + std::to_string(have) + " available");
+    double* din = base;
+    double* dout = base + 3 * N;
+    // void and halo entries of the input must read as zero; the output is
+    // read back only at solid nodes, which the operator writes
+    launch_fill<double>(din, 0.0, 3 * N, s);
+    launch_scatter_compact(l.mesh.node_pidx.as<long long>(), nn, x, din, N, s);
+    apply_level<double, double>(c, level - 1, AP_AX, din, 0.0, dout, nullptr, nullptr, nullptr);
+    launch_gather_compact<double>(l.mesh.node_pidx.as<long long>(), nn, dout, N, y, s);
+    GMGS_HIP(hipStreamSynchronize(s));
+    return GMGS_OK;
+  });
+}
+}  // namespace gmgs
+
 // ====================================================================== C ABI
 extern "C" {
 

@@ -125,6 +125,8 @@ def load_library() -> C.CDLL:
     lib.gmgs_kernel_timing_read.restype = C.c_int
     lib.gmgs_time_operator.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_double)]
     lib.gmgs_time_operator.restype = C.c_int
+    lib.gmgs_atx_device_noalloc.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
+    lib.gmgs_atx_device_noalloc.restype = C.c_int
     lib.gmgs_last_error.argtypes = [P]
     lib.gmgs_last_error.restype = C.c_char_p
     lib.gmgs_destroy.argtypes = [P]
@@ -143,7 +145,7 @@ def exported_symbols():
             "gmgs_vcycle", "gmgs_smooth", "gmgs_restrict", "gmgs_interpolate", "gmgs_query_size",
             "gmgs_query", "gmgs_last_error", "gmgs_destroy", "gmgs_kernel_timing",
             "gmgs_kernel_timing_read", "gmgs_batch_solve", "gmgs_copy_probe",
-            "gmgs_time_operator"]
+            "gmgs_time_operator", "gmgs_atx_device_noalloc"]
 
 
 def _dptr(a: np.ndarray):

@@ -180,6 +180,21 @@ int gmgs_adjoint(gmgs_ctx* c, const double* g, const double* U, double* lambda, 
 int gmgs_batch_solve(gmgs_ctx** ctxs, int n, double** U, gmgs_solve_info* infos);
 
 int gmgs_atx(gmgs_ctx* c, int level, const double* x, double* y);
+
+/* gmgs_atx on device-resident vectors, allocating nothing (no counterpart in
+ * the reference; for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * x_dev and y_dev are caller-owned device buffers of 3 * numNodes(level)
+ * doubles, node-interleaved in compact node order, on the context's device.
+ * The two padded level vectors the operator works on (48 bytes per padded
+ * node of the level) come from a file-scope device array of
+ * GMGS_NOALLOC_SCRATCH_MIB MiB that each GPU holds until the process ends;
+ * a larger level returns GMGS_ERR_ARGUMENT.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (two contexts' streams, two host
+ * threads) race.  Runs on the context's stream and returns when y_dev is
+ * written.  Needs gmgs_set_bc (the fixed DOFs). */
+#define GMGS_NOALLOC_SCRATCH_MIB 64
+int gmgs_atx_device_noalloc(gmgs_ctx* c, int level, const double* x_dev, double* y_dev);
 int gmgs_vcycle(gmgs_ctx* c, const double* r, double* z);
 int gmgs_smooth(gmgs_ctx* c, int level, const double* r, double* dx, double* r_out);
 int gmgs_restrict(gmgs_ctx* c, int level, const double* r_fine, double* r_coarse);
