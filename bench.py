@@ -60,23 +60,33 @@ def read_traffic(name: str, tx: int, zc: int):
     return None
 
 
-def cpu_baseline(seed, iters):
-    """Oracle FP64 PCG (reference algorithm) on the host, on the 169x147x256
-    ellipsoid problem of the same generator: set-up untimed, then ``iters``
-    timed iterations.  Returns (seconds per iteration per DOF, DOFs, seconds
-    per iteration)."""
+def cpu_baseline(seed, runs, budget_s=60.0):
+    """The reference algorithm's FP64 solve on the host cores: the oracle's
+    numpy set-up of the 169x147x256 ellipsoid problem of the same generator
+    (timed apart), then oracle/cpu_baseline.cpp (C++/OpenMP, all threads
+    OMP_NUM_THREADS allows) solving it to rtol 1e-6, up to ``runs`` times
+    within ``budget_s``; the median solve time is the sample."""
     sys.path.insert(0, str(REPO))
+    from oracle import cpu_baseline as cb
     from oracle import gmgs_oracle as oracle
     pkg = sys.modules["gmgs3d_amd"]
+    if not cb.LIB.exists():
+        cb.build()
     problem = pkg.workloads.ellipsoids_c2(seed)
+    t0 = time.perf_counter()
     m = oracle.build(problem.vox, problem.bbox, num_levels=problem.num_levels, omega=problem.omega)
     oracle.apply_bc(m, problem.loads, problem.fixed)
     oracle.factor_coarsest(m)
-    t0 = time.perf_counter()
-    oracle.cg_solver(m, m.F, 1e-30, iters - 1)      # exactly `iters` iterations
-    dt = (time.perf_counter() - t0) / iters
-    ndofs = m.levels[0].numDOFs
-    return dt / ndofs, ndofs, dt, problem.name
+    prep = cb.Prepared(m)
+    setup_s = time.perf_counter() - t0
+    times, its = [], 0
+    t_start = time.perf_counter()
+    while len(times) < runs and (not times or time.perf_counter() - t_start + times[-1] < budget_s):
+        _, its, conv, rel, sec = prep.solve(m.F, 1e-6, 3000)
+        times.append(sec)
+        log(f"[bench] cpu baseline solve {len(times)}: {its} its, relres {rel:.2e}, {sec:.2f} s")
+    return dict(ndofs=m.levels[0].numDOFs, its=its, seconds=float(np.median(times)), runs=len(times),
+                setup_s=setup_s, threads=cb.threads(), cpu=cb.cpu_model(), name=problem.name)
 
 
 def timed_solves(solver, steps, tag):
@@ -165,7 +175,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c2", "c3", "c4"])
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-it", type=int, default=3000)
     ap.add_argument("--matvec-reps", type=int, default=20)
@@ -284,15 +294,15 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline and cmp is not None:
-        log(f"[bench] cpu baseline: oracle set-up + {args.cpu_iters} timed iterations")
-        per_dof, sdofs, dt, sname = cpu_baseline(args.seed, args.cpu_iters)
-        t_cpu = per_dof * ndofs * info64.its
-        cpu = {"value": round(ndofs / t_cpu / 1e6, 5), "unit": "MDOF/s", "cores": 1, "kind": "port",
-               "sample": (f"numpy FP64 oracle (reference algorithm, numpy single-threaded) on "
-                          f"{sname} ({sdofs} DOFs, same generator): set-up untimed, "
-                          f"{args.cpu_iters} PCG iterations timed ({dt:.2f} s each); the per-DOF "
-                          f"iteration cost is scaled to this workload's {ndofs} DOFs and the "
-                          f"{info64.its} iterations its FP64 solve needs")}
+        log("[bench] cpu baseline: oracle set-up, then the C++/OpenMP FP64 reference-algorithm solve")
+        cb = cpu_baseline(args.seed, args.cpu_runs)
+        cpu = {"value": round(cb["ndofs"] / cb["seconds"] / 1e6, 4), "unit": "MDOF/s",
+               "cores": cb["threads"], "kind": "port",
+               "sample": (f"full FP64 solve of {cb['name']} ({cb['ndofs']} DOFs, 5 levels, same generator) "
+                          f"to rtol 1e-6 with the reference algorithm (Jacobi step, kernel/Vcycle.m cycle, "
+                          f"Fletcher-Reeves), oracle/cpu_baseline.cpp on {cb['threads']} OpenMP threads of "
+                          f"'{cb['cpu']}': {cb['its']} iterations, median of {cb['runs']} solve(s) "
+                          f"{cb['seconds']:.2f} s; numpy set-up {cb['setup_s']:.1f} s not in the value")}
     line = {
         "metric": "MDOF/s to rel. residual 1e-6, mixed-prec vs FP64 V-cycle; matvec HBM GB/s",
         "value": round(value, 3),

@@ -3,7 +3,7 @@
 The compute path is libgmgs.so (hand-written HIP for gfx950, C ABI in
 include/gmgs.h); this package is its Python host side.
 """
-from . import workloads  # noqa: F401
+from . import export, voxelize, workloads  # noqa: F401
 from .gmgs import (GMGS3D, GMGSError, NotConverged, Solver, default_params,  # noqa: F401
                    batch_solve, copy_probe,
                    load_library, library_path, exported_symbols,

@@ -419,11 +419,24 @@ class GMGS3D:
         self.iterationHist_ = np.zeros((0, 3))
         self.solver: Optional[Solver] = None
 
-    def CreateVoxilizedModel(self, nx, ny, nz):
-        """Cuboid branch of common/CreateVoxilizedModel.m:27-32."""
-        self.nelx_, self.nely_, self.nelz_ = int(nx), int(ny), int(nz)
-        self.voxelizedVolume_ = np.ones((ny, nx, nz), dtype=np.int32)
-        self.boundingBox_ = np.array([[0.0, 0.0, 0.0], [nx, ny, nz]])
+    def CreateVoxilizedModel(self, *args):
+        """common/CreateVoxilizedModel.m: (triFaceModelFile, finestResolutionControl)
+        voxelises a .ply / .obj surface mesh (:11-26), (nx, ny, nz) builds a
+        solid cuboid (:27-32)."""
+        if len(args) == 2:
+            from .voxelize import create_voxelized_model
+            self.voxelizedVolume_, self.boundingBox_ = create_voxelized_model(args[0], args[1])
+            self.nely_, self.nelx_, self.nelz_ = self.voxelizedVolume_.shape
+        elif len(args) == 3:
+            nx, ny, nz = (int(v) for v in args)
+            self.nelx_, self.nely_, self.nelz_ = nx, ny, nz
+            self.voxelizedVolume_ = np.ones((ny, nx, nz), dtype=np.int32)
+            self.boundingBox_ = np.array([[0.0, 0.0, 0.0], [nx, ny, nz]])
+        else:
+            raise ValueError("Wrong input for voxelization!")
+        if self.solver is not None:
+            self.solver.close()
+            self.solver = None
         return self.voxelizedVolume_
 
     def DiscretizeDesignDomain(self):
@@ -456,6 +469,28 @@ class GMGS3D:
             return None
         self.cartesianStressField_ = self.solver.stress(self.U_)
         return self.cartesianStressField_
+
+    # on-disk formats (common/ExportDataset_3DTSV.m, common/WrapVoxelFEAmodel.m)
+    def _finest_maps(self):
+        res = self.solver.query("RES", 1)
+        return (res, self.solver.query("ELE_MAP_BACK", 1), self.solver.query("NOD_MAP_BACK", 1))
+
+    def ExportDataset_3DTSV(self, fileName="dataset_TSV.carti"):
+        """The .carti stress dataset of the last GetCartesianStress."""
+        from .export import export_dataset_3dtsv
+        if self.cartesianStressField_ is None:
+            warnings.warn("No Cartesian Stress Available!")
+            return None
+        res, emb, nmb = self._finest_maps()
+        return export_dataset_3dtsv(fileName, res, self.boundingBox_, emb, nmb, self.fixingCond_,
+                                    self.loadingCond_, self.cartesianStressField_)
+
+    def WrapVoxelFEAmodel(self, fileName="wrappedVoxelFEAmodel.txt"):
+        from .export import wrap_voxel_fea_model
+        if self.solver is None:
+            self.DiscretizeDesignDomain()
+        res, emb, nmb = self._finest_maps()
+        return wrap_voxel_fea_model(fileName, res, emb, nmb, self.fixingCond_, self.loadingCond_)
 
     # operator handles
     def AtX(self, x, level=1):

@@ -366,17 +366,25 @@ def test_simple_kernels_match_oracle(case, oracle, pkg):
     s2.close()
 
 
-def test_c1_box_reference_config(oracle, pkg):
-    """BASELINE config 0: 40x40x64 box, 4 levels, omega 0.6, rtol 1e-3, FP64."""
+@pytest.fixture(scope="module")
+def c1_oracle(oracle, pkg):
+    """The oracle's solve of the 40x40x64 box (coarsestResolutionControl_ 1000
+    -> 4 levels, omega 0.6, rtol 1e-3, FP64) and its stress field."""
     p = pkg.workloads.box_c1()
-    s = pkg.Solver(p.vox, p.bbox, coarsest_resolution_control=p.coarsest_control,
-                   jacobi_weight=p.omega, tol=p.tol, max_it=p.max_it)
-    s.set_bc(p.fixed, p.loads)
-    U, info, hist = s.solve()
     m = oracle.build(p.vox, p.bbox, control=p.coarsest_control, omega=p.omega)
     oracle.apply_bc(m, p.loads, p.fixed)
     oracle.factor_coarsest(m)
     res = oracle.cg_solver(m, m.F, p.tol, p.max_it)
+    return p, m, res, oracle.cartesian_stress(m, res.x)
+
+
+def test_c1_box_reference_config(c1_oracle, pkg):
+    """BASELINE config 0: 40x40x64 box, 4 levels, omega 0.6, rtol 1e-3, FP64."""
+    p, m, res, _ = c1_oracle
+    s = pkg.Solver(p.vox, p.bbox, coarsest_resolution_control=p.coarsest_control,
+                   jacobi_weight=p.omega, tol=p.tol, max_it=p.max_it)
+    s.set_bc(p.fixed, p.loads)
+    U, info, hist = s.solve()
     e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
     log(f"  C1: dofs {U.size} its gpu {info.its} oracle {res.its} err {e:.2e} "
         f"t {info.solve_seconds * 1e3:.2f} ms")
@@ -385,8 +393,59 @@ def test_c1_box_reference_config(oracle, pkg):
     k = min(hist.shape[0], res.hist.shape[0])
     assert np.allclose(hist[:k, 2], res.hist[:k, 2], rtol=1e-6)
     S = s.stress(U)
-    assert rel(S, oracle.cartesian_stress(m, U)) < 1e-12
+    assert rel(S, oracle_stress_of(m, U)) < 1e-12
     s.close()
+
+
+def oracle_stress_of(m, U):
+    from oracle import gmgs_oracle
+    return gmgs_oracle.cartesian_stress(m, U)
+
+
+def test_gmgs3d_mirror_c1(c1_oracle, pkg, tmp_path):
+    """The reference's entry points through the GMGS3D mirror (globals under
+    the reference's names): CreateVoxilizedModel(40, 40, 64) (cuboid branch,
+    CreateVoxilizedModel.m:27-32), the C1 boundary conditions, GetDeformation
+    (GetDeformation.m) and GetCartesianStress (GetCartesianStress.m) against
+    the oracle: U within 1e-6, iterations within 1, iterationHist_ relres,
+    stress 1e-6; a second GetDeformation with changed loads re-applies the
+    BCs (no stale set-up, GetDeformation.m:12)."""
+    p, m, res, S_ref = c1_oracle
+    g = pkg.GMGS3D()
+    g.CreateVoxilizedModel(40, 40, 64)
+    g.coarsestResolutionControl_ = 1000.0
+    g.weightFactorJacobi_ = 0.6
+    g.tol_ = 1e-3
+    g.fixingCond_ = p.fixed
+    g.loadingCond_ = p.loads
+    with pytest.warns(UserWarning, match="No Deformation Field"):
+        assert g.GetCartesianStress() is None          # GetCartesianStress.m:6-9
+    U = g.GetDeformation()
+    S = g.GetCartesianStress()
+    e = np.linalg.norm(U - res.x) / np.linalg.norm(res.x)
+    log(f"  GMGS3D mirror: its {g.iterationHist_.shape[0] + 1} oracle {res.its} err {e:.2e}")
+    assert g.solver.num_levels == 4 and U.size == 327795
+    assert e < 1e-6 and abs(g.iterationHist_.shape[0] - res.hist.shape[0]) <= 1
+    k = min(g.iterationHist_.shape[0], res.hist.shape[0])
+    assert np.allclose(g.iterationHist_[:k, 2], res.hist[:k, 2], rtol=1e-6)
+    assert np.abs(S - S_ref).max() < 1e-6 * np.abs(S_ref).max()
+    assert np.array_equal(g.boundingBox_, np.array([[0, 0, 0], [40, 40, 64.0]]))
+    # the .carti dataset (ExportDataset_3DTSV.m) holds this stress field
+    lines = g.ExportDataset_3DTSV(tmp_path / "c1.carti").read_text().splitlines()
+    assert lines[:5] == ["DATASET CARTESIAN_GRID ", "Resolution: 40 40 64",
+                         "LowerBound: 0.000000 0.000000 0.000000", "UpperBound: 40.000000 40.000000 64.000000",
+                         "ELEMENTS 102400 int "]
+    k = lines.index("Cartesian Stress: 109265")
+    Sf = np.loadtxt(lines[k + 1:])
+    assert Sf.shape == (109265, 6) and np.abs(Sf - S).max() <= 1e-6 * np.abs(S).max()
+    assert lines[k - 1682] == "Fixed Nodes: 1681"
+    wrapped = g.WrapVoxelFEAmodel(tmp_path / "wrapped.txt").read_text().splitlines()
+    assert wrapped[2] == "valid elements: 102400" and wrapped[3] == "1"
+    g.loadingCond_ = p.loads.copy()
+    g.loadingCond_[:, 1:] *= 2.0
+    U2 = g.GetDeformation()
+    assert np.linalg.norm(U2 - 2 * U) < 1e-5 * np.linalg.norm(2 * U)
+    g.solver.close()
 
 
 def test_determinism_graphs_and_autotune(pkg):
