@@ -2,19 +2,23 @@
 """Benchmark: multigrid-PCG throughput on one MI355X.
 
 One step = one full solve of the synthetic ellipsoid-union elasticity problem
-(337x294x512 voxels, 6 levels, by default) to relative residual 1e-6 with the
-mixed-precision solver (FP32 symmetric V(2,2) cycle inside an FP64 flexible
-PCG): b resident in HBM, set-up excluded, the solution left on the device.
-The reference algorithm in FP64 (the baseline of the speed-up), the same
-symmetric cycle in FP64 and the reference cycle in FP32 are timed the same
-way beside it.  The level-1 FP64
-operators (matvec q = A p and the Jacobi step) are then timed with HIP events
-on the solver's stream over back-to-back launches, and one instrumented solve
-gives the per-kernel breakdown.
+(337x294x512 voxels, 6 levels, 56,994,411 DOFs, by default) to relative
+residual 1e-6 with the mixed-precision solver: an FP32 symmetric cycle with
+Chebyshev sweeps (degree 1 on level 1, 2 on level 2, 3 below) on exact
+Galerkin coarse operators inside an FP64 flexible PCG; b resident in HBM,
+set-up excluded, the solution left on the device.  The reference algorithm in
+FP64 (the baseline of the speed-up), the same cycle in FP64 and two other
+FP32 cycles are timed the same way beside it.  The level-1 FP64 operators
+(the direction update fused with the matvec, the matvec and the Jacobi step)
+are then timed with HIP events on the solver's stream over back-to-back
+launches, one instrumented solve gives the per-kernel breakdown, the CPU
+baseline solves the 169x147x256 problem with the reference algorithm on the
+host cores (oracle/cpu_baseline.cpp), and the 64-problem batch is timed
+against one-by-one solves (``c4_batch``).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4]
 
---config c4 times the 64-problem batch (one gmgs_batch_solve per step).
+--config c4 makes the 64-problem batch the benchmark line.
 """
 from __future__ import annotations
 
@@ -102,9 +106,10 @@ def timed_solves(solver, steps, tag):
     return float(np.mean(times)), info
 
 
-def run_batch(args, pkg):
+def batch_figures(args, pkg, steps):
     """The 64-problem batch (uneven ellipsoid-union solids, default level rule):
-    one gmgs_batch_solve per step, against the same problems solved one by one."""
+    ``steps`` gmgs_batch_solve calls after one warm-up, against the same
+    problems solved one by one.  Returns the measured figures."""
     t0 = time.perf_counter()
     probs = pkg.workloads.batch_c4(args.seed, 64)
     log(f"[bench] generated 64 problems in {time.perf_counter() - t0:.1f}s")
@@ -128,7 +133,7 @@ def run_batch(args, pkg):
     worst = max(float(np.abs(U - R).max() / max(np.abs(R).max(), 1e-300))
                 for (U, _), R in zip(out, ref))
     times = []
-    for k in range(args.steps):
+    for k in range(steps):
         t0 = time.perf_counter()
         res = pkg.batch_solve(solvers, copy=False)
         times.append((time.perf_counter() - t0) * 1e3)
@@ -141,6 +146,16 @@ def run_batch(args, pkg):
         s.solve(raise_on_nonconvergence=False, copy=False)
     ms_seq = (time.perf_counter() - t0) * 1e3
     log(f"[bench] sequential: {ms_seq:.1f} ms; batch {ms_batch:.1f} ms")
+    for s in solvers:
+        s.close()
+    return dict(total=total, dofs=dofs, ms_batch=ms_batch, ms_seq=ms_seq, worst=worst, setup_s=setup_s)
+
+
+def run_batch(args, pkg):
+    """--config c4: the batch as the benchmark line."""
+    b = batch_figures(args, pkg, args.steps)
+    total, dofs, ms_batch, ms_seq, worst, setup_s = (b["total"], b["dofs"], b["ms_batch"], b["ms_seq"],
+                                                     b["worst"], b["setup_s"])
     line = {
         "metric": "MDOF/s to rel. residual 1e-6, mixed-prec vs FP64 V-cycle; matvec HBM GB/s",
         "value": round(total / (ms_batch * 1e-3) / 1e6, 3),
@@ -164,8 +179,6 @@ def run_batch(args, pkg):
                    "setup_s": round(setup_s, 3)},
     }
     print(json.dumps(line), flush=True)
-    for s in solvers:
-        s.close()
 
 
 def main():
@@ -181,6 +194,11 @@ def main():
     ap.add_argument("--matvec-reps", type=int, default=20)
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the comparison solvers (counter-collection runs)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-problem batch figures of the default line")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--coarse-degree", type=int, default=3,
+                    help="Chebyshev degree of the headline cycle on levels 3 and below")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -208,10 +226,10 @@ def main():
     base_flags = pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE
     sym_flags = base_flags | pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP
     cheb_flags = sym_flags | pkg.FLAG_CHEBYSHEV | pkg.FLAG_EXACT_GALERKIN
-    cheb_sweeps = dict(vcycle_sweeps=1, vcycle_sweeps_level2=2, vcycle_sweeps_coarse=4)
+    cheb_sweeps = dict(vcycle_sweeps=1, vcycle_sweeps_level2=2, vcycle_sweeps_coarse=args.coarse_degree)
 
     # headline: FP32 symmetric cycle with Chebyshev sweeps (degree 1 on level 1,
-    # 2 on level 2, 4 below) on exact Galerkin coarse operators, inside the FP64
+    # 2 on level 2, 3 below) on exact Galerkin coarse operators, inside the FP64
     # flexible PCG
     t0 = time.perf_counter()
     s = pkg.Solver(prob.vox, prob.bbox, precision=pkg.GMGS_PREC_MIXED, flags=cheb_flags,
@@ -303,6 +321,16 @@ def main():
                           f"Fletcher-Reeves), oracle/cpu_baseline.cpp on {cb['threads']} OpenMP threads of "
                           f"'{cb['cpu']}': {cb['its']} iterations, median of {cb['runs']} solve(s) "
                           f"{cb['seconds']:.2f} s; numpy set-up {cb['setup_s']:.1f} s not in the value")}
+    batch = None
+    if not args.no_batch and not args.headline_only:
+        log("[bench] the 64-problem batch")
+        b = batch_figures(args, pkg, args.batch_steps)
+        batch = {"workload": f"batch64s{args.seed}", "problems": 64, "dofs": b["total"],
+                 "dofs_min": int(min(b["dofs"])), "dofs_max": int(max(b["dofs"])),
+                 "ms_batch": round(b["ms_batch"], 3), "ms_one_by_one": round(b["ms_seq"], 3),
+                 "mdofs_batch": round(b["total"] / (b["ms_batch"] * 1e-3) / 1e6, 3),
+                 "speedup_vs_one_by_one": round(b["ms_seq"] / b["ms_batch"], 3),
+                 "batch_vs_individual_max_rel_dev": b["worst"], "setup_s": round(b["setup_s"], 3)}
     line = {
         "metric": "MDOF/s to rel. residual 1e-6, mixed-prec vs FP64 V-cycle; matvec HBM GB/s",
         "value": round(value, 3),
@@ -318,8 +346,9 @@ def main():
         "data": "synthetic (seeded ellipsoid-union voxel solid)",
         "config": {"workload": prob.name, "dofs": ndofs, "levels": s.num_levels,
                    "rtol": 1e-6,
-                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, 2 on level 2, 4 below), "
-                             "exact Galerkin coarse operators, no Jacobi step, FP64 flexible PCG"),
+                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, 2 on level 2, "
+                             f"{args.coarse_degree} below), exact Galerkin coarse operators, no Jacobi step, "
+                             "FP64 flexible PCG"),
                    "fp64": "reference algorithm: kernel/Vcycle.m V-cycle, Jacobi step, FP64 PCG",
                    "its_mixed": info.its, "converged_mixed": bool(info.converged),
                    "true_relres_mixed": info.true_relres,
@@ -342,6 +371,7 @@ def main():
                               "alg_bytes_per_launch": jac_bytes, "avg_launch_ms": round(jac_ms, 5)},
         "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
         "cpu_baseline": cpu,
+        "c4_batch": batch,
     }
     print(json.dumps(line), flush=True)
     s.close()

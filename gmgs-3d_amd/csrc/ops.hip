@@ -712,10 +712,23 @@ __global__ __launch_bounds__(256) void k_add_alpha_p(Grid g, const uint32_t* __r
   }
 }
 
+// Padded index of this lane's node in tile tr (0, a halo entry, for a lane
+// beyond the grid's x extent)
+__device__ inline long long lane_index(const Grid& g, const TileRec& tr, bool& ok) {
+  const int lane = threadIdx.x & 63;
+  const int yp = tr.y0 + (lane & 15), xp = tr.x0 + (lane >> 4);
+  ok = xp < g.px;
+  return ok ? yp + (long long)g.py * (xp + (long long)g.px * tr.z) : 0;
+}
+
 // x = (x + omega r / d) + alpha p with the iteration's Jacobi increment
 // taken from r before it is replaced (no increment when invd is null; no x
 // update at all when x is null); r = r_a - alpha q (r_a may be r itself);
-// partials of r.r
+// partials of r.r.  Two tiles per pass of the fixed grid, the loads of both
+// issued together and independently of the flags (void and halo entries hold
+// zero and read as zero): only the stores and the norm look at the flags.
+// Each lane visits its tiles in the same order as with one tile per pass, so
+// the partial sums are unchanged.
 __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __restrict__ flags,
                                                    const TileRec* __restrict__ tiles, long long nt,
                                                    double* __restrict__ x, const double* __restrict__ p,
@@ -728,28 +741,49 @@ __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __res
   if (done && *done) return;
   double d = 0.0;
   const double alpha = sc->alpha;
-  GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
-    if (!ln_.active) continue;
-    const long long N = g.N, n = ln_.n;
-    // every load before the first store (r_a may be r itself)
-    double rav[3], qv[3], xv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, uv[3] = {0, 0, 0};
+  const long long N = g.N;
+  const long long stride = (long long)gridDim.x * 4;
+  for (long long t0 = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); t0 < nt; t0 += 2 * stride) {
+    const long long t1 = t0 + stride;
+    const bool has1 = t1 < nt;
+    const TileRec tr0 = tiles[t0];
+    const TileRec tr1 = tiles[has1 ? t1 : t0];
+    bool ok0, ok1;
+    const long long n0 = lane_index(g, tr0, ok0);
+    long long n1 = lane_index(g, tr1, ok1);
+    ok1 = ok1 && has1;
+    if (!ok1) n1 = 0;
+    const long long nn[2] = {n0, n1};
+    const uint32_t fl[2] = {flags[n0], flags[n1]};
+    // every load of both tiles before the first store (r_a may be r itself)
+    double rav[2][3], qv[2][3], xv[2][3], pv[2][3], uv[2][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      rav[c] = ra[c * N + n];
-      qv[c] = q[c * N + n];
-      if (x) {
-        xv[c] = x[c * N + n];
-        pv[c] = p[c * N + n];
-        if (invd) uv[c] = omega * invd[c * N + n] * r[c * N + n];
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const long long m = c * N + nn[k];
+        rav[k][c] = ra[m];
+        qv[k][c] = q[m];
+        xv[k][c] = pv[k][c] = uv[k][c] = 0.0;
+        if (x) {
+          xv[k][c] = x[m];
+          pv[k][c] = p[m];
+          if (invd) uv[k][c] = omega * invd[m] * r[m];
+        }
       }
-    }
+    const bool act[2] = {ok0 && (fl[0] & F_ELEM_MASK) != 0, ok1 && (fl[1] & F_ELEM_MASK) != 0};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (x && invd) x[c * N + n] = (xv[c] + uv[c]) + alpha * pv[c];
-      else if (x) x[c * N + n] = xv[c] + alpha * pv[c];
-      const double rv = rav[c] - alpha * qv[c];
-      r[c * N + n] = rv;
-      d += rv * rv;
+    for (int k = 0; k < 2; ++k) {
+      if (!act[k]) continue;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const long long m = c * N + nn[k];
+        if (x && invd) x[m] = (xv[k][c] + uv[k][c]) + alpha * pv[k][c];
+        else if (x) x[m] = xv[k][c] + alpha * pv[k][c];
+        const double rv = rav[k][c] - alpha * qv[k][c];
+        r[m] = rv;
+        d += rv * rv;
+      }
     }
   }
   const double b = block_sum(d);

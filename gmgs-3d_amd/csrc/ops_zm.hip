@@ -25,6 +25,7 @@
 #include "ops_zm.h"
 #include "mesh.h"
 
+
 namespace gmgs {
 
 namespace {
@@ -310,50 +311,58 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
     fN = flags_at(zz + 2);
     // neighbour (dy, dx) = (a-1, b-1), component c, read from the staged plane
 #define GMGS_U(a, b, c) sB[c][lx + (b)][ly + (a)]
-    // class sums of the in-plane neighbourhood (shared by the three outputs)
-    T C0[3], Cy[3], Cx[3], Cxy[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      C0[c] = GMGS_U(1, 1, c);
-      Cy[c] = GMGS_U(0, 1, c) + GMGS_U(2, 1, c);
-      Cx[c] = GMGS_U(1, 0, c) + GMGS_U(1, 2, c);
-      Cxy[c] = (GMGS_U(0, 0, c) + GMGS_U(2, 2, c)) + (GMGS_U(0, 2, c) + GMGS_U(2, 0, c));
-    }
-    const T Dxy_x = (GMGS_U(2, 2, 0) + GMGS_U(0, 0, 0)) - (GMGS_U(2, 0, 0) + GMGS_U(0, 2, 0));
-    const T Dxy_y = (GMGS_U(2, 2, 1) + GMGS_U(0, 0, 1)) - (GMGS_U(2, 0, 1) + GMGS_U(0, 2, 1));
-    const T Dx0_x = GMGS_U(1, 2, 0) - GMGS_U(1, 0, 0);
-    const T Dx1_x = (GMGS_U(0, 2, 0) - GMGS_U(0, 0, 0)) + (GMGS_U(2, 2, 0) - GMGS_U(2, 0, 0));
-    const T Dx0_z = GMGS_U(1, 2, 2) - GMGS_U(1, 0, 2);
-    const T Dx1_z = (GMGS_U(0, 2, 2) - GMGS_U(0, 0, 2)) + (GMGS_U(2, 2, 2) - GMGS_U(2, 0, 2));
-    const T Dy0_y = GMGS_U(2, 1, 1) - GMGS_U(0, 1, 1);
-    const T Dy1_y = (GMGS_U(2, 0, 1) - GMGS_U(0, 0, 1)) + (GMGS_U(2, 2, 1) - GMGS_U(0, 2, 1));
-    const T Dy0_z = GMGS_U(2, 1, 2) - GMGS_U(0, 1, 2);
-    const T Dy1_z = (GMGS_U(2, 0, 2) - GMGS_U(0, 0, 2)) + (GMGS_U(2, 2, 2) - GMGS_U(0, 2, 2));
-#undef GMGS_U
+    // One displacement component at a time: its nine in-plane values, their
+    // class sums and differences, and their contributions to the three
+    // outputs (z-1, z, z+1) are formed before the next component's values are
+    // read, so only nine staged values are live at once (the scheduling
+    // barrier keeps the next component's LDS reads behind this one's
+    // arithmetic).  The class coefficients: diagonal blocks by
+    // |dy| + 2|dx| (cf[8 c + 4 q + 0..3], q = 1 off the output's plane),
+    // the xy coupling of the two in-plane diagonals (cf[24 + q]), the xz and
+    // yz couplings of the off-plane outputs (cf[26..27], cf[28..29], odd in
+    // dz).
     {
       const cptr<T> C = opaque(op.cls);
       T cf[30];
 #pragma unroll
       for (int i = 0; i < 30; ++i) cf[i] = C[i];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int oz = 1 - k, q = oz != 0 ? 1 : 0;
-        T* acc = k == 0 ? aM : (k == 1 ? a0 : aP);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          acc[c] += cf[8 * c + 4 * q] * C0[c] + cf[8 * c + 1 + 4 * q] * Cy[c]
-                    + cf[8 * c + 2 + 4 * q] * Cx[c] + cf[8 * c + 3 + 4 * q] * Cxy[c];
-        acc[0] += cf[24 + q] * Dxy_y;
-        acc[1] += cf[24 + q] * Dxy_x;
-        if (oz != 0) {
-          const T sgn = (T)oz;
-          acc[0] += sgn * (cf[26] * Dx0_z + cf[27] * Dx1_z);
-          acc[2] += sgn * (cf[26] * Dx0_x + cf[27] * Dx1_x);
-          acc[1] += sgn * (cf[28] * Dy0_z + cf[29] * Dy1_z);
-          acc[2] += sgn * (cf[28] * Dy0_y + cf[29] * Dy1_y);
+      for (int c = 0; c < 3; ++c) {
+        const T u00 = GMGS_U(0, 0, c), u01 = GMGS_U(0, 1, c), u02 = GMGS_U(0, 2, c);
+        const T u10 = GMGS_U(1, 0, c), u11 = GMGS_U(1, 1, c), u12 = GMGS_U(1, 2, c);
+        const T u20 = GMGS_U(2, 0, c), u21 = GMGS_U(2, 1, c), u22 = GMGS_U(2, 2, c);
+        const T cy = u01 + u21, cx = u10 + u12, cxy = (u00 + u22) + (u02 + u20);
+        aM[c] += cf[8 * c + 4] * u11 + cf[8 * c + 5] * cy + cf[8 * c + 6] * cx + cf[8 * c + 7] * cxy;
+        a0[c] += cf[8 * c + 0] * u11 + cf[8 * c + 1] * cy + cf[8 * c + 2] * cx + cf[8 * c + 3] * cxy;
+        aP[c] += cf[8 * c + 4] * u11 + cf[8 * c + 5] * cy + cf[8 * c + 6] * cx + cf[8 * c + 7] * cxy;
+        if (c < 2) {
+          // in-plane diagonal difference: x feeds the y output, y the x output
+          const T dxy = (u22 + u00) - (u20 + u02);
+          const int o = 1 - c;
+          aM[o] += cf[25] * dxy;
+          a0[o] += cf[24] * dxy;
+          aP[o] += cf[25] * dxy;
         }
+        if (c != 1) {
+          // x differences: of x into the z outputs, of z into the x outputs
+          const T dx0 = u12 - u10, dx1 = (u02 - u00) + (u22 - u20);
+          const T t = cf[26] * dx0 + cf[27] * dx1;
+          const int o = c == 0 ? 2 : 0;
+          aM[o] += t;
+          aP[o] -= t;
+        }
+        if (c != 0) {
+          // y differences: of y into the z outputs, of z into the y outputs
+          const T dy0 = u21 - u01, dy1 = (u20 - u00) + (u22 - u02);
+          const T t = cf[28] * dy0 + cf[29] * dy1;
+          const int o = c == 1 ? 2 : 1;
+          aM[o] += t;
+          aP[o] -= t;
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
+#undef GMGS_U
     // write output zz-1
     const int zo_ = zz - 1;
     if (lane_ok && zo_ >= z0 && zo_ < z1 && (fM & OUT_MASK)) {
