@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--coarse-degree", type=int, default=3,
                     help="Chebyshev degree of the headline cycle on levels 3 and below")
+    ap.add_argument("--level2-degree", type=int, default=2,
+                    help="Chebyshev degree of the headline cycle on level 2")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -226,7 +228,8 @@ def main():
     base_flags = pkg.FLAG_NO_INDEX_TABLES | pkg.FLAG_AUTOTUNE
     sym_flags = base_flags | pkg.FLAG_SYM_VCYCLE | pkg.FLAG_NO_JACOBI_STEP
     cheb_flags = sym_flags | pkg.FLAG_CHEBYSHEV | pkg.FLAG_EXACT_GALERKIN
-    cheb_sweeps = dict(vcycle_sweeps=1, vcycle_sweeps_level2=2, vcycle_sweeps_coarse=args.coarse_degree)
+    cheb_sweeps = dict(vcycle_sweeps=1, vcycle_sweeps_level2=args.level2_degree,
+                       vcycle_sweeps_coarse=args.coarse_degree)
 
     # headline: FP32 symmetric cycle with Chebyshev sweeps (degree 1 on level 1,
     # 2 on level 2, 3 below) on exact Galerkin coarse operators, inside the FP64
@@ -346,8 +349,8 @@ def main():
         "data": "synthetic (seeded ellipsoid-union voxel solid)",
         "config": {"workload": prob.name, "dofs": ndofs, "levels": s.num_levels,
                    "rtol": 1e-6,
-                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, 2 on level 2, "
-                             f"{args.coarse_degree} below), exact Galerkin coarse operators, no Jacobi step, "
+                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, "
+                             f"{args.level2_degree} on level 2, {args.coarse_degree} below), exact Galerkin coarse operators, no Jacobi step, "
                              "FP64 flexible PCG"),
                    "fp64": "reference algorithm: kernel/Vcycle.m V-cycle, Jacobi step, FP64 PCG",
                    "its_mixed": info.its, "converged_mixed": bool(info.converged),
