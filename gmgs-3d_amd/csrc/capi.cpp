@@ -10,6 +10,7 @@
 #include <mutex>
 #include <tuple>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "common.h"
@@ -293,6 +294,44 @@ std::vector<T> download(const DevBuf& d, size_t n, hipStream_t s) {
   return h;
 }
 
+// Equal rows of the FP64 assembled rows [q][row] (stride 'rows'): uid[row]
+// numbers the distinct rows in order of first appearance, rep[u] is the first
+// row of distinct row u.  Rows are grouped by two 64-bit fingerprints and the
+// grouping is then checked bitwise on the device; should two different rows
+// ever share both fingerprints, every row stays its own.
+void distinct_rows(const double* canon, long long rows, std::vector<int>& uid, std::vector<long long>& rep,
+                   hipStream_t s) {
+  using Key = std::pair<unsigned long long, unsigned long long>;
+  struct KeyHash {
+    size_t operator()(const Key& k) const { return (size_t)(k.first ^ (k.second * 0x9E3779B97F4A7C15ull)); }
+  };
+  DevBuf dh;
+  dh.alloc(sizeof(unsigned long long) * 2 * (size_t)rows);
+  launch_row_hash(canon, rows, dh.as<unsigned long long>(), s);
+  const std::vector<unsigned long long> h = download<unsigned long long>(dh, 2 * (size_t)rows, s);
+  std::unordered_map<Key, int, KeyHash> first;
+  first.reserve((size_t)rows);
+  uid.assign((size_t)rows, 0);
+  rep.clear();
+  for (long long r = 0; r < rows; ++r) {
+    const auto it = first.emplace(Key(h[2 * r], h[2 * r + 1]), (int)rep.size());
+    if (it.second) rep.push_back(r);
+    uid[(size_t)r] = it.first->second;
+  }
+  DevBuf du, drep, dbad;
+  upload(du, uid.data(), uid.size(), s);
+  upload(drep, rep.data(), rep.size(), s);
+  dbad.alloc_zero(sizeof(int), s);
+  launch_row_verify(canon, rows, du.as<int>(), drep.as<long long>(), dbad.as<int>(), s);
+  if (download<int>(dbad, 1, s)[0] != 0) {
+    rep.resize((size_t)rows);
+    for (long long r = 0; r < rows; ++r) {
+      uid[(size_t)r] = (int)r;
+      rep[(size_t)r] = r;
+    }
+  }
+}
+
 void to_f32(const DevBuf& a, DevBuf& b, long long n, hipStream_t s) {
   b.alloc(sizeof(float) * (n > 0 ? n : 1));
   launch_convert<double, float>(a.as<double>(), b.as<float>(), n, s);
@@ -557,24 +596,36 @@ void build_operators(gmgs_ctx* c) {
     }
     const long long stride = rows + 1;
     const long long asc_n = (long long)kStencilCoefs * stride;
-    if (!ar.empty() && (double)asc_n * sizeof(double) <= kAscBudgetBytes &&
-        16.0 * (double)stride < 4294967296.0 && 3 * l.mesh.g.N < (1LL << 31)) {
+    if (!ar.empty() && (double)asc_n * sizeof(double) <= kAscBudgetBytes && 3 * l.mesh.g.N < (1LL << 31)) {
       for (int& v : ar)
         if (v < 0) v = (int)(stride - 1);
-      upload(l.arow, ar.data(), ar.size(), s);
-      l.as_stride = stride;
+      DevBuf slots;
+      upload(slots, ar.data(), ar.size(), s);
       DevBuf canon;
       canon.alloc_zero(sizeof(double) * asc_n, s);
-      launch_assemble_stencil(make_op<double>(c, li), l.tiles.as<TileRec>(), l.ntiles, l.arow.as<int>(), stride,
+      launch_assemble_stencil(make_op<double>(c, li), l.tiles.as<TileRec>(), l.ntiles, slots.as<int>(), stride,
                               canon.as<double>(), s);
+      // equal rows stored once: the nodes' rows become indices of distinct rows
+      std::vector<int> uid;
+      std::vector<long long> rep;
+      distinct_rows(canon.as<double>(), stride, uid, rep, s);
+      const long long nu = (long long)rep.size();
+      DevBuf drep;
+      upload(drep, rep.data(), rep.size(), s);
       if (c->mixed) {
-        l.asc32.alloc_zero(sizeof(float) * stencil_group_elems(stride, sizeof(float)), s);
-        launch_relayout_stencil<float>(canon.as<double>(), stride, l.asc32.as<float>(), s);
+        l.asc32.alloc_zero(sizeof(float) * (size_t)(nu * stencil_row_elems(sizeof(float))), s);
+        launch_relayout_rows<float>(canon.as<double>(), stride, drep.as<long long>(), nu, l.asc32.as<float>(), s);
       } else {
-        l.asc64.alloc_zero(sizeof(double) * stencil_group_elems(stride, sizeof(double)), s);
-        launch_relayout_stencil<double>(canon.as<double>(), stride, l.asc64.as<double>(), s);
+        l.asc64.alloc_zero(sizeof(double) * (size_t)(nu * stencil_row_elems(sizeof(double))), s);
+        launch_relayout_rows<double>(canon.as<double>(), stride, drep.as<long long>(), nu, l.asc64.as<double>(),
+                                     s);
       }
-      // tiles holding a row of their own (not F_FAST), and those rows
+      l.as_stride = nu;
+      std::vector<int> au(ar.size());
+      for (size_t i = 0; i < ar.size(); ++i) au[i] = uid[(size_t)ar[i]];
+      upload(l.arow, au.data(), au.size(), s);
+      // tiles holding a row of their own (a solid node that is not F_FAST),
+      // and those tiles' rows
       std::vector<TileRec> mt;
       std::vector<int> mr;
       for (long long t = 0; t < l.ntiles; ++t) {
@@ -582,7 +633,7 @@ void build_operators(gmgs_ctx* c) {
         for (int lane = 0; lane < 64; ++lane) any = any || ar[t * 64 + lane] != (int)(stride - 1);
         if (!any) continue;
         mt.push_back(th[t]);
-        mr.insert(mr.end(), ar.begin() + t * 64, ar.begin() + (t + 1) * 64);
+        mr.insert(mr.end(), au.begin() + t * 64, au.begin() + (t + 1) * 64);
       }
       l.nmtiles = (long long)mt.size();
       if (!mt.empty()) {

@@ -22,9 +22,9 @@ struct LevelOp {
   const T* emod;             // per padded cell modulus factor (level-1 SIMP) or null
   const T* invd;             // inverse diagonal, SoA 3N (zero off the solid nodes)
   const int* colz;           // per padded column (y + py x): solid-node z range lo | hi << 16, or null
-  const T* asc;              // assembled stencil rows in 16-byte groups [q / G][row][G], or null
+  const T* asc;              // assembled stencil rows, one after another (stencil_row_elems each), or null
   const int* arow;           // per tile slot (64 tile + lane): the node's row of asc
-  long long as_stride;       // rows of asc (the last one: the universal stencil)
+  long long as_stride;       // rows of asc
   const T* corner;           // canonical row block of Ke (72): every block is a reflection of one
   int zero_fixed;            // SMOOTH: fixed components of the output are 0 instead of the input's
 };
@@ -89,14 +89,22 @@ constexpr int kStencilCoefs = 243;
 __host__ __device__ constexpr int stencil_plane_groups(int elem_bytes) {
   return (81 + 16 / elem_bytes - 1) / (16 / elem_bytes);
 }
-// the grouped layout [q' / G][row][G] (G = 16 / sizeof(T), z-planes padded to
-// whole groups) of the level operator's assembled rows, from the FP64 rows
-// [q][row]; dst zeroed, stencil_group_elems(rows, sizeof(T)) elements
-template <typename T>
-void launch_relayout_stencil(const double* src, long long rows, T* dst, hipStream_t s);
-inline long long stencil_group_elems(long long rows, int elem_bytes) {
-  return 3LL * stencil_plane_groups(elem_bytes) * (16 / elem_bytes) * rows;
+// elements of one assembled row in the layout of k_apply_as: three z-planes
+// of 81 coefficients, each padded to whole 16-byte groups
+inline long long stencil_row_elems(int elem_bytes) {
+  return 3LL * stencil_plane_groups(elem_bytes) * (16 / elem_bytes);
 }
+// rows rep[0..nu) of the FP64 rows [q][row] (stride 'rows') in that layout,
+// one after another; dst zeroed, nu * stencil_row_elems(sizeof(T)) elements
+template <typename T>
+void launch_relayout_rows(const double* src, long long rows, const long long* rep, long long nu, T* dst,
+                          hipStream_t s);
+// two 64-bit fingerprints per FP64 row [q][row] (h[2 row], h[2 row + 1])
+void launch_row_hash(const double* src, long long rows, unsigned long long* h, hipStream_t s);
+// *bad = 1 (*bad zeroed by the caller) if a row's coefficients differ
+// bitwise from those of row rep[uid[row]]
+void launch_row_verify(const double* src, long long rows, const int* uid, const long long* rep, int* bad,
+                       hipStream_t s);
 
 template <typename T, typename TR>
 void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode,

@@ -174,11 +174,12 @@ __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __r
 // 16 x 4 x 1 node tile, four tiles per block.  The tile's 18 x 6 x 3
 // neighbourhood of the input (u = omega r / d for the smoother) is staged in
 // LDS by the wave; each lane then multiplies its 27 neighbours by its row of
-// the operator, whose 243 coefficients are stored in 16-byte groups
-// ([q' / G][row][G], G = 16 / sizeof(T); each z-plane's 81 coefficients
-// padded to whole groups, q' = (q / 81) QP + q % 81) so a lane reads its row
-// with coalesced 16-byte loads, one z-plane at a time.  F_FAST lanes share the
-// universal row.
+// the operator.  Rows are stored one after another, each z-plane's 81
+// coefficients padded to whole 16-byte groups (q' = (q / 81) QP + q % 81), so
+// a lane reads its row with consecutive 16-byte loads, one z-plane at a time.
+// Equal rows are stored once (nodes with the same neighbourhood of elements
+// share them), which keeps the rows of a level in the last-level cache.
+// F_FAST lanes share the universal row.
 // Coefficients read through a constant-address-space pointer kept opaque to
 // loop-invariant code motion: scalar loads into SGPR operands where they are
 // used, instead of values hoisted into vector registers.
@@ -364,20 +365,16 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
 #undef GMGS_AU
       }
     } else {
-      // the node's assembled row (F_FAST nodes: the universal row): group q0
-      // at a uniform base (q0 rows groups) plus one 32-bit byte offset (16
-      // rows < 2^32, checked at set-up); a wave whose nodes are all F_FAST
-      // skips these loads
-      const unsigned boff = (unsigned)arow * 16u;
-      const char* __restrict__ base = reinterpret_cast<const char*>(op.asc);
-      const long long gbytes = op.as_stride * 16;
+      // the node's assembled row (F_FAST nodes: the universal row), read
+      // group after group; a wave whose nodes are all F_FAST skips these loads
+      const char* __restrict__ base =
+          reinterpret_cast<const char*>(op.asc) + (long long)arow * (3 * NGP * 16);
 #pragma unroll 1
       for (int kz = 0; kz < 3; ++kz) {
         const T* __restrict__ sp = &sU[wave][0][SY * SX * kz + (ly + 1) + SY * (lx + 1)];
 #pragma unroll
         for (int g0 = 0; g0 < NGP; ++g0) {
-          const CoefGroup<T> cg =
-              *reinterpret_cast<const CoefGroup<T>*>(base + (long long)(kz * NGP + g0) * gbytes + boff);
+          const CoefGroup<T> cg = *reinterpret_cast<const CoefGroup<T>*>(base + (kz * NGP + g0) * 16);
 #pragma unroll
           for (int k = 0; k < G; ++k) {
             const int r = g0 * G + k;                   // coefficient within the z-plane
@@ -410,17 +407,50 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
   }
 }
 
-// canonical FP64 rows [q][row] -> the grouped layout [q / G][row][G] of
-// k_apply_as in T (coefficients 243.. of the last group zero)
+// canonical FP64 rows [q][row] (stride 'rows') -> the row-major layout
+// [u][q'] of k_apply_as in T for the rows rep[0..nu): q' = (q / 81) QP +
+// q % 81, QP = G x (16-byte groups per z-plane), 3 QP elements per row (the
+// padding of each z-plane zero)
 template <typename T>
-__global__ void k_relayout_stencil(const double* __restrict__ src, long long rows, T* __restrict__ dst) {
+__global__ void k_relayout_rows(const double* __restrict__ src, long long rows,
+                                const long long* __restrict__ rep, long long nu, T* __restrict__ dst) {
   constexpr int G = 16 / (int)sizeof(T), QP = G * stencil_plane_groups((int)sizeof(T));
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)kStencilCoefs * rows) return;
-  const int q = (int)(i / rows);
-  const long long row = i % rows;
+  if (i >= (long long)kStencilCoefs * nu) return;
+  const int q = (int)(i % kStencilCoefs);
+  const long long u = i / kStencilCoefs;
   const int qp = (q / 81) * QP + q % 81;
-  dst[((long long)(qp / G) * rows + row) * G + qp % G] = (T)src[i];
+  dst[u * (3 * QP) + qp] = (T)src[(long long)q * rows + rep[u]];
+}
+
+// two 64-bit fingerprints of every FP64 row [q][row]: the bit patterns of its
+// 243 coefficients through two multiply-xorshift chains with different
+// constants
+__global__ void k_row_hash(const double* __restrict__ src, long long rows,
+                           unsigned long long* __restrict__ h) {
+  const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  unsigned long long a = 0x243F6A8885A308D3ull, b = 0x13198A2E03707344ull;
+  for (int q = 0; q < kStencilCoefs; ++q) {
+    const unsigned long long v = (unsigned long long)__double_as_longlong(src[(long long)q * rows + row]);
+    a = (a ^ v) * 0x9E3779B97F4A7C15ull;
+    a ^= a >> 29;
+    b = (b + v) * 0xC2B2AE3D27D4EB4Full;
+    b ^= b >> 31;
+  }
+  h[2 * row] = a;
+  h[2 * row + 1] = b;
+}
+
+// *bad = 1 if some row's coefficients differ bitwise from its representative's
+__global__ void k_row_verify(const double* __restrict__ src, long long rows, const int* __restrict__ uid,
+                             const long long* __restrict__ rep, int* __restrict__ bad) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)kStencilCoefs * rows) return;
+  const long long row = i % rows;
+  const long long q = i / rows;
+  const long long r = rep[uid[row]];
+  if (__double_as_longlong(src[q * rows + row]) != __double_as_longlong(src[q * rows + r])) atomicOr(bad, 1);
 }
 
 // the row of a solid node in the level operator as a 27-point stencil: the
@@ -1222,14 +1252,31 @@ template int launch_apply_as_rows<float, double>(const LevelOp<float>&, const Ti
                                                  const int*, hipStream_t);
 
 template <typename T>
-void launch_relayout_stencil(const double* src, long long rows, T* dst, hipStream_t s) {
-  const long long n = (long long)kStencilCoefs * rows;
+void launch_relayout_rows(const double* src, long long rows, const long long* rep, long long nu, T* dst,
+                          hipStream_t s) {
+  const long long n = (long long)kStencilCoefs * nu;
   if (n <= 0) return;
-  k_relayout_stencil<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, rows, dst);
+  k_relayout_rows<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, rows, rep, nu, dst);
   GMGS_LAUNCH_CHECK();
 }
-template void launch_relayout_stencil<double>(const double*, long long, double*, hipStream_t);
-template void launch_relayout_stencil<float>(const double*, long long, float*, hipStream_t);
+template void launch_relayout_rows<double>(const double*, long long, const long long*, long long, double*,
+                                           hipStream_t);
+template void launch_relayout_rows<float>(const double*, long long, const long long*, long long, float*,
+                                          hipStream_t);
+
+void launch_row_hash(const double* src, long long rows, unsigned long long* h, hipStream_t s) {
+  if (rows <= 0) return;
+  k_row_hash<<<(unsigned)((rows + 255) / 256), 256, 0, s>>>(src, rows, h);
+  GMGS_LAUNCH_CHECK();
+}
+
+void launch_row_verify(const double* src, long long rows, const int* uid, const long long* rep, int* bad,
+                       hipStream_t s) {
+  const long long n = (long long)kStencilCoefs * rows;
+  if (n <= 0) return;
+  k_row_verify<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, rows, uid, rep, bad);
+  GMGS_LAUNCH_CHECK();
+}
 
 void launch_assemble_stencil(const LevelOp<double>& op, const TileRec* tiles, long long ntiles,
                              const int* arow, long long stride, double* asc, hipStream_t s) {
