@@ -61,10 +61,10 @@ struct Level {
   long long as_stride = 0;
   bool as_built = false, as = false;
   // split operator: the z-marching kernel on the F_FAST nodes and the
-  // assembled rows on the others, over the tiles that hold them (mtiles, with
-  // their rows per tile and lane in marow)
-  DevBuf mtiles, marow;
-  long long nmtiles = 0;
+  // assembled rows on the others, grouped in row boxes (rboxes, with their
+  // nodes and rows in rentries)
+  DevBuf rboxes, rentries;
+  long long nrbox = 0;
   bool split_ok = false, split = false;
   int col_tx = 8, col_zc = 16;
   // level 1 of a mixed context: the FP64 matvec's own column tiles (the
@@ -248,10 +248,8 @@ int apply_level(gmgs_ctx* c, int li, int mode, const TR* in, T omega, T* out, TR
   if (l.split) {
     int np = launch_apply_zm<T, TR>(op, l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc, mode, in, omega,
                                     in2, out, partial, done, c->stream, nullptr, true);
-    LevelOp<T> opr = op;
-    opr.arow = l.marow.as<int>();
-    np += launch_apply_as_rows<T, TR>(opr, l.mtiles.as<TileRec>(), l.nmtiles, mode, in, omega, in2, out,
-                                      partial ? partial + np : nullptr, done, c->stream);
+    np += launch_apply_rows<T, TR>(op, l.rboxes.as<RowTile>(), l.rentries.as<RowEntry>(), l.nrbox, mode, in,
+                                   omega, in2, out, partial ? partial + np : nullptr, done, c->stream);
     if (xacc)
       launch_jacobi_acc<T, TR>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles,
                                op.invd, in, omega, xacc, c->stream);
@@ -571,9 +569,9 @@ void build_operators(gmgs_ctx* c) {
     // the universal row last; used where the z-marching kernel is not
     l.as = l.as_built = false;
     l.split = l.split_ok = false;
-    l.mtiles.release();
-    l.marow.release();
-    l.nmtiles = 0;
+    l.rboxes.release();
+    l.rentries.release();
+    l.nrbox = 0;
     l.asc64.release();
     l.asc32.release();
     l.arow.release();
@@ -624,26 +622,46 @@ void build_operators(gmgs_ctx* c) {
       std::vector<int> au(ar.size());
       for (size_t i = 0; i < ar.size(); ++i) au[i] = uid[(size_t)ar[i]];
       upload(l.arow, au.data(), au.size(), s);
-      // tiles holding a row of their own (a solid node that is not F_FAST),
-      // and those tiles' rows
-      std::vector<TileRec> mt;
-      std::vector<int> mr;
-      for (long long t = 0; t < l.ntiles; ++t) {
-        bool any = false;
-        for (int lane = 0; lane < 64; ++lane) any = any || ar[t * 64 + lane] != (int)(stride - 1);
-        if (!any) continue;
-        mt.push_back(th[t]);
-        mr.insert(mr.end(), au.begin() + t * 64, au.begin() + (t + 1) * 64);
-      }
-      l.nmtiles = (long long)mt.size();
-      if (!mt.empty()) {
-        upload(l.mtiles, mt.data(), mt.size(), s);
-        upload(l.marow, mr.data(), mr.size(), s);
+      // row boxes of the split operator: the solid nodes that are not
+      // F_FAST, by box of 16 x 4 x kRowBoxZ nodes (boxes in order of their
+      // first node in the tile list, nodes in tile and lane order), up to 64
+      // per record
+      std::map<std::tuple<int, int, int>, int> box_of;
+      std::vector<RowTile> bhead;
+      std::vector<std::vector<RowEntry>> bents;
+      for (long long t = 0; t < l.ntiles; ++t)
+        for (int lane = 0; lane < 64; ++lane) {
+          if (ar[t * 64 + lane] == (int)(stride - 1)) continue;
+          const int zb = (th[t].z - 1) / kRowBoxZ, z0 = 1 + zb * kRowBoxZ;
+          const auto key = std::make_tuple(zb, th[t].x0, th[t].y0);
+          auto it = box_of.find(key);
+          if (it == box_of.end()) {
+            it = box_of.emplace(key, (int)bhead.size()).first;
+            bhead.push_back(RowTile{th[t].y0, th[t].x0, z0, 0});
+            bents.emplace_back();
+          }
+          const int pos = (lane & 15) | ((lane >> 4) << 4) | ((th[t].z - z0) << 6);
+          bents[(size_t)it->second].push_back(RowEntry{pos, au[t * 64 + lane]});
+        }
+      std::vector<RowTile> rb;
+      std::vector<RowEntry> re;
+      for (size_t b = 0; b < bhead.size(); ++b)
+        for (size_t k0 = 0; k0 < bents[b].size(); k0 += 64) {
+          const size_t k1 = std::min(bents[b].size(), k0 + 64);
+          RowTile r = bhead[b];
+          r.n = (int)(k1 - k0);
+          rb.push_back(r);
+          for (size_t k = k0; k < k0 + 64; ++k) re.push_back(k < k1 ? bents[b][k] : RowEntry{-1, 0});
+        }
+      l.nrbox = (long long)rb.size();
+      if (!rb.empty()) {
+        upload(l.rboxes, rb.data(), rb.size(), s);
+        upload(l.rentries, re.data(), re.size(), s);
       }
       GMGS_HIP(hipStreamSynchronize(s));
       l.as_built = true;
       // the split operator needs the z-marching kernel's tiles
-      l.split_ok = l.zm && l.nmtiles > 0;
+      l.split_ok = l.zm && l.nrbox > 0;
       l.split = l.split_ok && c->prefer_split;
       l.as = !l.split && (!l.zm || c->prefer_as || c->prefer_split);
       if (c->prefer_as || c->prefer_split) l.zm = false;
@@ -736,10 +754,8 @@ float time_split(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx
   auto run = [&]() {
     launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, AP_SMOOTH, in, (T)0.5, nullptr, out, nullptr, nullptr, s,
                            nullptr, true);
-    LevelOp<T> opr = op;
-    opr.arow = l.marow.as<int>();
-    launch_apply_as_rows<T, TR>(opr, l.mtiles.as<TileRec>(), l.nmtiles, AP_SMOOTH, in, (T)0.5, nullptr, out,
-                                nullptr, nullptr, s);
+    launch_apply_rows<T, TR>(op, l.rboxes.as<RowTile>(), l.rentries.as<RowEntry>(), l.nrbox, AP_SMOOTH, in,
+                             (T)0.5, nullptr, out, nullptr, nullptr, s);
   };
   HipEvent a, b;
   for (int w = 0; w < 2; ++w) run();
@@ -760,7 +776,7 @@ long long partial_capacity(gmgs_ctx* c) {
   long long n = c->nparts;
   for (const Level& l : c->lev) {
     n = std::max<long long>(n, std::max<long long>(apply_blocks(l.ntiles), std::max(l.ncol, l.ncol64)));
-    if (l.split_ok) n = std::max<long long>(n, l.ncol + (long long)apply_blocks(l.nmtiles));
+    if (l.split_ok) n = std::max<long long>(n, l.ncol + (long long)apply_blocks(l.nrbox));
   }
   return n;
 }

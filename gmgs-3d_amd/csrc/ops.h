@@ -202,13 +202,25 @@ void launch_xpby(const Grid& g, const uint32_t* flags, const TileRec* tiles, lon
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                 const double* a, const double* b, double* partial, hipStream_t s);
 
-// the assembled rows on the solid nodes that are not F_FAST, over the tiles
-// that hold such nodes (op.arow: their rows, per tile and lane): AX, SMOOTH
-// or RESID (out = in2 - A in) as launch_apply; returns the number of dot
-// partials written (one per block of four tiles)
+// Row boxes of the split operator's second half: the solid nodes that are
+// not F_FAST, grouped by box of 16 (y) x 4 (x) x kRowBoxZ (z) nodes, up to 64
+// per record.  Record r's box starts at padded (y0, x0, z0); entry 64 r + lane
+// gives that lane's node in the box (pos = ly | lx << 4 | lz << 6, -1 for
+// none) and its assembled row.
+constexpr int kRowBoxZ = 4;
+struct RowTile {
+  int y0, x0, z0, n;
+};
+struct RowEntry {
+  int pos, row;
+};
+// the assembled rows of the nodes the row boxes list: AX, SMOOTH or RESID
+// (out = in2 - A in) as launch_apply; returns the number of dot partials
+// written (one per block of four records: AX p.Ap, RESID |in2 - A in|^2)
 template <typename T, typename TR>
-int launch_apply_as_rows(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode, const TR* in,
-                         T omega, const T* in2, T* out, double* partial, const int* done, hipStream_t s);
+int launch_apply_rows(const LevelOp<T>& op, const RowTile* boxes, const RowEntry* entries, long long nbox,
+                      int mode, const TR* in, T omega, const T* in2, T* out, double* partial, const int* done,
+                      hipStream_t s);
 
 // start of a solve from x0 = 0: x = 0, r = b, partials of b.b, and p, p2,
 // z0, z1 = 0 where given (null: not written)

@@ -198,9 +198,7 @@ struct alignas(16) CoefGroup {
   T v[16 / sizeof(T)];
 };
 
-// ROWS_ONLY: only the solid nodes that are not F_FAST are computed and written
-// (the split operator's second half, over the tiles that hold such nodes)
-template <typename T, typename TR, int MODE, bool ROWS_ONLY>
+template <typename T, typename TR, int MODE>
 __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* __restrict__ tiles,
                                                   long long ntiles, const TR* __restrict__ in, T omega,
                                                   const T* __restrict__ in2, T* __restrict__ out,
@@ -316,14 +314,14 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if ((fl & F_ELEM_MASK) && !(ROWS_ONLY && (fl & F_FAST))) {
+  if (fl & F_ELEM_MASK) {
     T a[3] = {0, 0, 0};
     // F_FAST nodes (all eight elements present and universal): the universal
     // stencil from its 30 reflection-class coefficients, read as scalar
     // operands, over the three staged planes (input plane z - 1 + zz, so
     // oz = zz - 1 is the input's z offset from the output)
     const bool ucls = op.cls != nullptr;
-    if (!ROWS_ONLY && ucls && (fl & F_FAST)) {
+    if (ucls && (fl & F_FAST)) {
       const cptr<T> C = opaque(op.cls);
       T cf[30];
 #pragma unroll
@@ -396,6 +394,121 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
         if (partial) dsum += (double)cen[c] * (double)ac;
       } else if (MODE == AP_RESID) {
         out[c * N + n] = (T)cen[c] - ac;
+      } else {
+        out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : (T)cen[c] - ac;
+      }
+    }
+  }
+  if (partial) {
+    const double b = block_sum(dsum);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+  }
+}
+
+// The split operator's second half: one wave per row box.  The wave stages
+// the box's neighbourhood (18 x 6 x (kRowBoxZ + 2) nodes, u = omega r / d for
+// the smoother) in LDS, 16 loads per lane at a time with out-of-grid positions
+// reading index 0 (a zero halo entry), and each lane with an entry multiplies
+// its node's 27 neighbours by the node's assembled row, in the row's
+// coefficient order (the sums of k_apply_as).
+template <typename T, typename TR, int MODE>
+__global__ __launch_bounds__(256) void k_apply_rows(LevelOp<T> op, const RowTile* __restrict__ boxes,
+                                                    const RowEntry* __restrict__ entries, long long nbox,
+                                                    const TR* __restrict__ in, T omega,
+                                                    const T* __restrict__ in2, T* __restrict__ out,
+                                                    double* __restrict__ partial, const int* __restrict__ done) {
+  if (done && *done) return;
+  constexpr int SY = 18, SX = 6, SZ = kRowBoxZ + 2, NP = SY * SX * SZ, NS = 3 * NP;
+  constexpr int G = 16 / (int)sizeof(T), NGP = stencil_plane_groups((int)sizeof(T));
+  constexpr int BATCH = 16;
+  __shared__ T sU[4][NS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long r = (long long)blockIdx.x * 4 + wave;
+  const Grid g = op.g;
+  const long long N = g.N;
+  double dsum = 0.0;
+  RowTile bx{0, 0, 0, 0};
+  RowEntry en{-1, 0};
+  long long n = 0;
+  uint32_t fl = 0;
+  TR cen[3] = {0, 0, 0};                 // the node's in (SMOOTH, AX dot) or in2 (RESID)
+  if (r < nbox) {
+    bx = boxes[r];
+    en = entries[r * 64 + lane];
+    if (en.pos >= 0) {
+      n = (bx.y0 + (en.pos & 15)) + g.py * ((bx.x0 + ((en.pos >> 4) & 3)) + g.px * (long long)(bx.z0 + (en.pos >> 6)));
+      fl = op.flags[n];
+      if (MODE == AP_RESID) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cen[c] = (TR)in2[c * N + n];
+      } else if (MODE == AP_SMOOTH || partial) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cen[c] = in[c * N + n];
+      }
+    }
+    // 32-bit indices: 3 N < 2^31 (checked at set-up)
+    const int gy0 = bx.y0 - 1, gx0 = bx.x0 - 1, gz0 = bx.z0 - 1;
+    const int isx = (int)g.py, isz = (int)(g.py * g.px), iN = (int)N;
+    for (int b0 = 0; b0 < NS; b0 += 64 * BATCH) {
+      TR v[BATCH];
+      T d[MODE == AP_SMOOTH ? BATCH : 1];
+#pragma unroll
+      for (int k = 0; k < BATCH; ++k) {
+        const int e = b0 + k * 64 + lane;
+        const int c = e / NP, rem = e % NP, zz = rem / (SY * SX), xx = (rem / SY) % SX, yy = rem % SY;
+        const int y = gy0 + yy, x = gx0 + xx, z = gz0 + zz;
+        const bool ok = e < NS && y >= 0 && y < g.py && x >= 0 && x < g.px && z >= 0 && z < g.pz;
+        const unsigned m = ok ? (unsigned)(y + isx * x + isz * z + iN * c) : 0u;
+        v[k] = in[m];
+        if (MODE == AP_SMOOTH) d[k] = op.invd[m];
+      }
+#pragma unroll
+      for (int k = 0; k < BATCH; ++k) {
+        const int e = b0 + k * 64 + lane;
+        if (e < NS) {
+          T a = (T)v[k];
+          if (MODE == AP_SMOOTH) a = omega * d[k] * a;
+          sU[wave][e] = a;
+        }
+      }
+    }
+  }
+  // each wave stages and reads only its own box
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (en.pos >= 0) {
+    const int ly = en.pos & 15, lx = (en.pos >> 4) & 3, lz = en.pos >> 6;
+    T a[3] = {0, 0, 0};
+    const char* __restrict__ base = reinterpret_cast<const char*>(op.asc) + (long long)en.row * (3 * NGP * 16);
+#pragma unroll 1
+    for (int kz = 0; kz < 3; ++kz) {
+      const T* __restrict__ sp = &sU[wave][SY * SX * (lz + kz) + SY * (lx + 1) + (ly + 1)];
+#pragma unroll
+      for (int g0 = 0; g0 < NGP; ++g0) {
+        const CoefGroup<T> cg = *reinterpret_cast<const CoefGroup<T>*>(base + (kz * NGP + g0) * 16);
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+          const int q = g0 * G + k;                     // coefficient within the z-plane
+          if (q < 81) {
+            const int j = q / 9, cc = (q / 3) % 3, dd = q % 3;
+            const int dy = j % 3 - 1, dx = j / 3 - 1;
+            a[cc] += cg.v[k] * sp[dd * NP + dy + SY * dx];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const bool fx = (fl & (1u << (F_FIX_SHIFT + c))) != 0;
+      const T ac = fx ? (T)0 : a[c];          // AtX output zeroed on fixed DOFs (kernel/AtX.m:57)
+      if (MODE == AP_AX) {
+        out[c * N + n] = ac;
+        if (partial) dsum += (double)cen[c] * (double)ac;
+      } else if (MODE == AP_RESID) {
+        const T v = (T)cen[c] - ac;
+        out[c * N + n] = v;
+        if (partial) dsum += (double)v * (double)v;
       } else {
         out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : (T)cen[c] - ac;
       }
@@ -1201,10 +1314,10 @@ void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, 
   const bool eop = op.eop != nullptr, emod = op.emod != nullptr;
   if (op.asc && !xacc && mode != AP_RESID) {
     if (mode == AP_AX)
-      k_apply_as<T, TR, AP_AX, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, nullptr, out, partial,
+      k_apply_as<T, TR, AP_AX><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, nullptr, out, partial,
                                                           done);
     else
-      k_apply_as<T, TR, AP_SMOOTH, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, nullptr, out,
+      k_apply_as<T, TR, AP_SMOOTH><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, nullptr, out,
                                                               partial, done);
     GMGS_LAUNCH_CHECK();
     return;
@@ -1225,31 +1338,35 @@ void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, 
 }
 
 template <typename T, typename TR>
-int launch_apply_as_rows(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, int mode, const TR* in,
-                         T omega, const T* in2, T* out, double* partial, const int* done, hipStream_t s) {
-  if (ntiles <= 0) return 0;
-  if (!op.asc || !op.arow) throw std::invalid_argument("apply_as_rows: no assembled rows");
-  const unsigned nb = apply_blocks(ntiles);
+int launch_apply_rows(const LevelOp<T>& op, const RowTile* boxes, const RowEntry* entries, long long nbox,
+                      int mode, const TR* in, T omega, const T* in2, T* out, double* partial, const int* done,
+                      hipStream_t s) {
+  if (nbox <= 0) return 0;
+  if (!op.asc) throw std::invalid_argument("apply_rows: no assembled rows");
+  if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("apply_rows: grid too large");
+  const unsigned nb = apply_blocks(nbox);
   if (mode == AP_AX)
-    k_apply_as<T, TR, AP_AX, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, in2, out, partial, done);
+    k_apply_rows<T, TR, AP_AX><<<nb, 256, 0, s>>>(op, boxes, entries, nbox, in, omega, in2, out, partial, done);
   else if (mode == AP_RESID)
-    k_apply_as<T, TR, AP_RESID, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, in2, out, partial, done);
+    k_apply_rows<T, TR, AP_RESID><<<nb, 256, 0, s>>>(op, boxes, entries, nbox, in, omega, in2, out, partial,
+                                                     done);
   else if (mode == AP_SMOOTH)
-    k_apply_as<T, TR, AP_SMOOTH, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, in2, out, partial, done);
+    k_apply_rows<T, TR, AP_SMOOTH><<<nb, 256, 0, s>>>(op, boxes, entries, nbox, in, omega, in2, out, partial,
+                                                      done);
   else
-    throw std::invalid_argument("apply_as_rows: mode");
+    throw std::invalid_argument("apply_rows: mode");
   GMGS_LAUNCH_CHECK();
   return (int)nb;
 }
-template int launch_apply_as_rows<double, double>(const LevelOp<double>&, const TileRec*, long long, int,
-                                                  const double*, double, const double*, double*, double*,
-                                                  const int*, hipStream_t);
-template int launch_apply_as_rows<float, float>(const LevelOp<float>&, const TileRec*, long long, int,
-                                                const float*, float, const float*, float*, double*, const int*,
-                                                hipStream_t);
-template int launch_apply_as_rows<float, double>(const LevelOp<float>&, const TileRec*, long long, int,
-                                                 const double*, float, const float*, float*, double*,
-                                                 const int*, hipStream_t);
+template int launch_apply_rows<double, double>(const LevelOp<double>&, const RowTile*, const RowEntry*,
+                                               long long, int, const double*, double, const double*, double*,
+                                               double*, const int*, hipStream_t);
+template int launch_apply_rows<float, float>(const LevelOp<float>&, const RowTile*, const RowEntry*, long long,
+                                             int, const float*, float, const float*, float*, double*,
+                                             const int*, hipStream_t);
+template int launch_apply_rows<float, double>(const LevelOp<float>&, const RowTile*, const RowEntry*,
+                                              long long, int, const double*, float, const float*, float*,
+                                              double*, const int*, hipStream_t);
 
 template <typename T>
 void launch_relayout_rows(const double* src, long long rows, const long long* rep, long long nu, T* dst,
