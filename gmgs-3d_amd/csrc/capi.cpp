@@ -623,7 +623,7 @@ void build_operators(gmgs_ctx* c) {
       for (size_t i = 0; i < ar.size(); ++i) au[i] = uid[(size_t)ar[i]];
       upload(l.arow, au.data(), au.size(), s);
       // row boxes of the split operator: the solid nodes that are not
-      // F_FAST, by box of 16 x 4 x kRowBoxZ nodes (boxes in order of their
+      // F_FAST, by box of 16 x kRowBoxX x kRowBoxZ nodes (boxes in order of their
       // first node in the tile list, nodes in tile and lane order), up to 64
       // per record
       std::map<std::tuple<int, int, int>, int> box_of;
@@ -633,14 +633,15 @@ void build_operators(gmgs_ctx* c) {
         for (int lane = 0; lane < 64; ++lane) {
           if (ar[t * 64 + lane] == (int)(stride - 1)) continue;
           const int zb = (th[t].z - 1) / kRowBoxZ, z0 = 1 + zb * kRowBoxZ;
-          const auto key = std::make_tuple(zb, th[t].x0, th[t].y0);
+          const int x0 = th[t].x0 - th[t].x0 % kRowBoxX;
+          const auto key = std::make_tuple(zb, x0, th[t].y0);
           auto it = box_of.find(key);
           if (it == box_of.end()) {
             it = box_of.emplace(key, (int)bhead.size()).first;
-            bhead.push_back(RowTile{th[t].y0, th[t].x0, z0, 0});
+            bhead.push_back(RowTile{th[t].y0, x0, z0, 0});
             bents.emplace_back();
           }
-          const int pos = (lane & 15) | ((lane >> 4) << 4) | ((th[t].z - z0) << 6);
+          const int pos = (lane & 15) | ((th[t].x0 - x0 + (lane >> 4)) << 4) | ((th[t].z - z0) << 7);
           bents[(size_t)it->second].push_back(RowEntry{pos, au[t * 64 + lane]});
         }
       std::vector<RowTile> rb;

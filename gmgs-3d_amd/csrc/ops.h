@@ -203,11 +203,12 @@ void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long
                 const double* a, const double* b, double* partial, hipStream_t s);
 
 // Row boxes of the split operator's second half: the solid nodes that are
-// not F_FAST, grouped by box of 16 (y) x 4 (x) x kRowBoxZ (z) nodes, up to 64
-// per record.  Record r's box starts at padded (y0, x0, z0); entry 64 r + lane
-// gives that lane's node in the box (pos = ly | lx << 4 | lz << 6, -1 for
-// none) and its assembled row.
-constexpr int kRowBoxZ = 4;
+// not F_FAST, grouped by box of 16 (y) x kRowBoxX (x) x kRowBoxZ (z) nodes,
+// up to 64 per record.  Record r's box starts at padded (y0, x0, z0); entry
+// 64 r + lane gives that lane's node in the box (pos = ly | lx << 4 | lz << 7,
+// -1 for none) and its assembled row.
+constexpr int kRowBoxX = 4, kRowBoxZ = 4;
+static_assert(kRowBoxX == 4 || kRowBoxX == 8, "row boxes hold whole 16 x 4 tiles in x, lx in 3 bits");
 struct RowTile {
   int y0, x0, z0, n;
 };

@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
 }
 
 // The split operator's second half: one wave per row box.  The wave stages
-// the box's neighbourhood (18 x 6 x (kRowBoxZ + 2) nodes, u = omega r / d for
+// the box's neighbourhood (18 x (kRowBoxX + 2) x (kRowBoxZ + 2) nodes, u = omega r / d for
 // the smoother) in LDS, 16 loads per lane at a time with out-of-grid positions
 // reading index 0 (a zero halo entry), and each lane with an entry multiplies
 // its node's 27 neighbours by the node's assembled row, in the row's
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void k_apply_rows(LevelOp<T> op, const RowTile
                                                     const T* __restrict__ in2, T* __restrict__ out,
                                                     double* __restrict__ partial, const int* __restrict__ done) {
   if (done && *done) return;
-  constexpr int SY = 18, SX = 6, SZ = kRowBoxZ + 2, NP = SY * SX * SZ, NS = 3 * NP;
+  constexpr int SY = 18, SX = kRowBoxX + 2, SZ = kRowBoxZ + 2, NP = SY * SX * SZ, NS = 3 * NP;
   constexpr int G = 16 / (int)sizeof(T), NGP = stencil_plane_groups((int)sizeof(T));
   constexpr int BATCH = 16;
   __shared__ T sU[4][NS];
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void k_apply_rows(LevelOp<T> op, const RowTile
     bx = boxes[r];
     en = entries[r * 64 + lane];
     if (en.pos >= 0) {
-      n = (bx.y0 + (en.pos & 15)) + g.py * ((bx.x0 + ((en.pos >> 4) & 3)) + g.px * (long long)(bx.z0 + (en.pos >> 6)));
+      n = (bx.y0 + (en.pos & 15)) + g.py * ((bx.x0 + ((en.pos >> 4) & 7)) + g.px * (long long)(bx.z0 + (en.pos >> 7)));
       fl = op.flags[n];
       if (MODE == AP_RESID) {
 #pragma unroll
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(256) void k_apply_rows(LevelOp<T> op, const RowTile
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (en.pos >= 0) {
-    const int ly = en.pos & 15, lx = (en.pos >> 4) & 3, lz = en.pos >> 6;
+    const int ly = en.pos & 15, lx = (en.pos >> 4) & 7, lz = en.pos >> 7;
     T a[3] = {0, 0, 0};
     const char* __restrict__ base = reinterpret_cast<const char*>(op.asc) + (long long)en.row * (3 * NGP * 16);
 #pragma unroll 1
