@@ -1595,8 +1595,7 @@ void set_bc_impl(gmgs_ctx* c, const int32_t* fixed, int64_t k, const double* loa
   inverse_from_cholesky(c->A.as<double>(), c->W.as<double>(), c->Kinv.as<double>(), c->nfp, s);
   c->W.release();
   c->gv_b.alloc_zero(sizeof(double) * c->nfp, s);
-  const long long nch = (c->nfp + 255) / 256;
-  c->gv_part.alloc_zero(sizeof(double) * nch * c->nfp, s);
+  c->gv_part.alloc_zero(sizeof(double) * coarsest_partial_elems(c->nfp), s);
   GMGS_HIP(hipStreamSynchronize(s));
   smoother_weights(c);
   c->bc = true;

@@ -24,7 +24,9 @@ int cholesky(double* A, long long nfp, hipStream_t s);
 // Kinv = inv(A) from the factor L in A (uses W as workspace for L^-1)
 void inverse_from_cholesky(const double* L, double* W, double* Kinv, long long nfp, hipStream_t s);
 
-// x_L(free) = Kinv * r_L(free), x_L(fixed) = 0
+// x_L(free) = Kinv * r_L(free), x_L(fixed) = 0; bbuf holds nfp doubles,
+// partial coarsest_partial_elems(nfp)
+long long coarsest_partial_elems(long long nfp);
 template <typename T>
 void coarsest_apply(const double* Kinv, long long nfp, long long nfree, const long long* code,
                     const T* r, T* x, long long N, double* bbuf, double* partial, const int* done,

@@ -276,7 +276,9 @@ __global__ void k_gather_free(const long long* __restrict__ code, long long nfre
 }
 
 constexpr int GV_ROWS = 256;
-constexpr int GV_CHUNK = 256;
+// rows of K_ff^-1 per partial: short enough that the product's serial sums
+// (one per block and row) leave hundreds of blocks in flight
+constexpr int GV_CHUNK = 64;
 
 // partial[chunk][i] = sum_{k in chunk} Kinv[k][i] b[k]   (Kinv symmetric)
 __global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ Kinv, long long nfp,
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ Kinv, l
   __shared__ double bs[GV_CHUNK];
   const long long i = (long long)blockIdx.x * GV_ROWS + threadIdx.x;
   const long long k0 = (long long)blockIdx.y * GV_CHUNK;
-  bs[threadIdx.x] = (k0 + threadIdx.x < nfp) ? b[k0 + threadIdx.x] : 0.0;
+  if (threadIdx.x < GV_CHUNK) bs[threadIdx.x] = (k0 + threadIdx.x < nfp) ? b[k0 + threadIdx.x] : 0.0;
   __syncthreads();
   if (i >= nfp) return;
   double acc = 0.0;
@@ -309,6 +311,8 @@ __global__ void k_scatter_free(const long long* __restrict__ code, long long nfr
 }
 
 }  // namespace
+
+long long coarsest_partial_elems(long long nfp) { return (nfp + GV_CHUNK - 1) / GV_CHUNK * nfp; }
 
 long long coarsest_free_dofs(const Grid& g, const long long* node_pidx, long long nn,
                              const uint32_t* flags, DevBuf& code, DevBuf& dofmap, hipStream_t s) {
