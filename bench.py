@@ -120,7 +120,8 @@ def batch_figures(args, pkg, steps):
     for p in probs:
         s = pkg.Solver(p.vox, p.bbox, num_levels=p.num_levels, jacobi_weight=p.omega, tol=1e-6,
                        max_it=args.max_it, precision=pkg.GMGS_PREC_MIXED, flags=flags,
-                       vcycle_sweeps=1, vcycle_sweeps_level2=2, vcycle_sweeps_coarse=4)
+                       vcycle_sweeps=1, vcycle_sweeps_level2=args.level2_degree,
+                       vcycle_sweeps_coarse=args.batch_coarse_degree)
         s.set_bc(p.fixed, p.loads)
         solvers.append(s)
     setup_s = time.perf_counter() - t0
@@ -171,7 +172,8 @@ def run_batch(args, pkg):
         "data": "synthetic (64 seeded uneven ellipsoid-union solids)",
         "config": {"workload": f"batch64s{args.seed}", "problems": 64, "dofs": total,
                    "dofs_min": int(min(dofs)), "dofs_max": int(max(dofs)), "rtol": 1e-6,
-                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, 2 on level 2, 4 below), "
+                   "mixed": ("FP32 symmetric cycle, Chebyshev sweeps (degree 1 on level 1, "
+                             f"{args.level2_degree} on level 2, {args.batch_coarse_degree} below), "
                              "exact Galerkin coarse operators, no Jacobi step, FP64 flexible PCG"),
                    "sequential_ms": round(ms_seq, 3),
                    "speedup_vs_sequential": round(ms_seq / ms_batch, 3),
@@ -199,6 +201,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--coarse-degree", type=int, default=3,
                     help="Chebyshev degree of the headline cycle on levels 3 and below")
+    ap.add_argument("--batch-coarse-degree", type=int, default=4,
+                    help="Chebyshev degree of the batch's cycle on levels 3 and below")
     ap.add_argument("--level2-degree", type=int, default=2,
                     help="Chebyshev degree of the headline cycle on level 2")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
