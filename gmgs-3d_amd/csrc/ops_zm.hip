@@ -97,38 +97,39 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   const bool lane_ok = xp < g.px;
   const long long colidx = yp + sx * xp;
   const int z0 = t.z0, z1 = min(t.z0 + zc, g.pz - 1);   // outputs [z0, z1)
-  int own_lo = 1, own_hi = 0;                            // own column's solid range
-  if (lane_ok) {
-    const int r = op.colz[colidx];
-    own_lo = r & 0xffff;
-    own_hi = r >> 16;
-  }
-
-  // this thread's share of a staged plane: global offset (without z) and
+  // This thread's share of a staged plane: global offset (without z) and
   // that column's solid z range (lo in the low 16 bits, hi - lo above); a
   // position outside the grid or on a column without solid nodes gets the
-  // range lo = 0xffff, hi - lo = 0, which no plane index matches
-  int soff[PER];
+  // range lo = 0xffff, hi - lo = 0, which no plane index matches.  The
+  // column ranges (one per staged value and this lane's own) are all loaded
+  // before the first is used: one round trip at the start of the tile, not
+  // one per value.  Column 0 is a halo column without solid nodes (lo =
+  // 0xffff > hi = 0), so a position outside the grid loads from there and
+  // comes out empty, branch-free.
+  int scol[PER];
   int zr[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int e = tid + i * NT;
-    soff[i] = 0;
-    zr[i] = 0xffff;
-    if (e < NSTAGE) {
-      const int c = e / (SX * SY), rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
-      const int gy = t.y0 - 1 + yy, gx = t.x0 - 1 + xx;
-      if (gy >= 0 && gy < g.py && gx >= 0 && gx < g.px) {
-        const long long col = gy + sx * gx;
-        const int r = op.colz[col];
-        const int lo = r & 0xffff, hi = r >> 16;
-        if (hi >= lo) {
-          soff[i] = (int)(col + c * N);
-          zr[i] = lo | ((hi - lo) << 16);
-        }
-      }
-    }
+    const int rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
+    const int gy = t.y0 - 1 + yy, gx = t.x0 - 1 + xx;
+    const bool in = e < NSTAGE && gy >= 0 && gy < g.py && gx >= 0 && gx < g.px;
+    scol[i] = in ? gy + (int)sx * gx : 0;
+    zr[i] = op.colz[scol[i]];
   }
+  const int own_r = op.colz[lane_ok ? (int)colidx : 0];
+  const int own_lo = own_r & 0xffff, own_hi = own_r >> 16;
+  int soff[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + i * NT;
+    const int c = e / (SX * SY);
+    const int lo = zr[i] & 0xffff, hi = zr[i] >> 16;
+    const bool solid = hi >= lo;
+    soff[i] = solid ? scol[i] + c * (int)N : 0;
+    zr[i] = solid ? lo | ((hi - lo) << 16) : 0xffff;
+  }
+
   // raw loads only (no arithmetic on them until they are staged, two planes
   // later): the input and, for the smoother, the inverse diagonal (AX_P: z).  Index 0 of
   // every padded array is a corner halo entry and holds 0, so a skipped
