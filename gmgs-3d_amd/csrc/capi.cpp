@@ -16,6 +16,7 @@
 #include "common.h"
 #include "dense.h"
 #include "fe_host.h"
+#include "launch.h"
 #include "mesh.h"
 #include "ops.h"
 #include "ops_zm.h"
@@ -80,6 +81,13 @@ struct Level {
 };
 
 }  // namespace
+
+namespace gmgs {
+Recorder*& current_recorder() {
+  static thread_local Recorder* r = nullptr;
+  return r;
+}
+}  // namespace gmgs
 
 struct gmgs_ctx {
   gmgs_params prm{};
@@ -1260,10 +1268,65 @@ void finish_deferred_x(gmgs_ctx* c, hipStream_t s) {
 // Batch graphs (gmgs_batch_solve): one iteration pair of each context of a
 // list, forked from and joined into one stream, plus a gather of their done
 // flags; kept for the list until one of its contexts changes.
+//
+// Contexts whose iteration pairs make the same sequence of kernel launches
+// (the same kernels in the same order, any grid sizes) form a group: each
+// step of the group is ONE launch over all its contexts (k_multi), whose
+// workgroups each do the work of one context's own launch, so every context's
+// results are bitwise those of its own solve.  A context whose pair holds a
+// kernel that cannot be merged keeps its own launches on a branch of its own.
 struct BatchGraph {
   hipGraphExec_t ge = nullptr;
   DevBuf dptrs, dflags;   // the contexts' done flags (device pointers) and their gather
+  std::vector<DevBuf> keep;   // argument structs and block maps of the merged launches
+  int groups = 0;             // branches of the graph
+  long long launches = 0;     // kernel launches per iteration pair
 };
+
+// The launches of one iteration pair of c, recorded (inside a stream capture
+// that is then discarded, so nothing runs).  False when the pair holds a
+// node that did not go through launch() (it could not be merged).
+bool record_iteration_pair(gmgs_ctx* c, Recorder& rec) {
+  GMGS_HIP(hipStreamSynchronize(c->stream));
+  struct Install {
+    explicit Install(Recorder* r) { current_recorder() = r; }
+    ~Install() { current_recorder() = nullptr; }
+  } install(&rec);
+  hipGraph_t g = nullptr;
+  GMGS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  try {
+    for (int k = 0; k < 2; ++k) {
+      if (c->mixed) pcg_iteration<float>(c, k);
+      else pcg_iteration<double>(c, k);
+    }
+  } catch (...) {
+    if (hipStreamEndCapture(c->stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  GMGS_HIP(hipStreamEndCapture(c->stream, &g));
+  size_t nn = 0;
+  hipError_t e = hipGraphGetNodes(g, nullptr, &nn);
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (e == hipSuccess && nn) e = hipGraphGetNodes(g, nodes.data(), &nn);
+  size_t kernels = 0;
+  bool other = false;
+  for (size_t i = 0; e == hipSuccess && i < nn; ++i) {
+    hipGraphNodeType t;
+    e = hipGraphNodeGetType(nodes[i], &t);
+    if (t == hipGraphNodeTypeKernel) ++kernels;
+    else other = true;
+  }
+  (void)hipGraphDestroy(g);
+  GMGS_HIP(e);
+  return !other && kernels == rec.recs.size();
+}
+
+bool same_launches(const Recorder& a, const Recorder& b) {
+  if (a.recs.size() != b.recs.size()) return false;
+  for (size_t i = 0; i < a.recs.size(); ++i)
+    if (*a.recs[i].type != *b.recs[i].type) return false;
+  return true;
+}
 using BatchKey = std::vector<gmgs_ctx*>;
 
 std::map<BatchKey, BatchGraph>& batch_cache() {
@@ -1968,37 +2031,79 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
       GMGS_HIP(hipStreamWaitEvent(s0, ev[i], 0));
       maxk = std::max(maxk, cs[i]->prm.max_it + 1);
     }
-    // the graph of one iteration pair of every context of 'act' (fork/join
-    // capture), from the cache or captured now
+    // the graph of one iteration pair of every context of 'act' (merged
+    // launches per group, fork/join across groups), from the cache or built now
     auto graph_for = [&](const BatchKey& act) -> BatchGraph& {
       std::lock_guard<std::mutex> lk(batch_mutex());
       auto it = batch_cache().find(act);
       if (it != batch_cache().end()) return it->second;
-      BatchGraph& bg = batch_cache()[act];
       const int m = (int)act.size();
+      // record every context's pair and group equal sequences
+      std::vector<Recorder> recs(m);
+      std::vector<int> mergeable(m);
+      for (int i = 0; i < m; ++i) mergeable[i] = record_iteration_pair(act[i], recs[i]) ? 1 : 0;
+      std::vector<std::vector<int>> groups;
+      for (int i = 0; i < m; ++i) {
+        bool placed = false;
+        for (auto& gr : groups)
+          if (mergeable[i] && mergeable[gr[0]] && same_launches(recs[gr[0]], recs[i])) {
+            gr.push_back(i);
+            placed = true;
+            break;
+          }
+        if (!placed) groups.push_back({i});
+      }
+      BatchGraph bg;
       std::vector<const int*> hp(m);
       for (int i = 0; i < m; ++i) hp[i] = &act[i]->sc.as<PcgScalars>()->done;
       upload(bg.dptrs, hp.data(), (size_t)m, s0);
       bg.dflags.alloc_zero(sizeof(int) * m, s0);
+      // argument structs and block maps of every merged step, before the capture
+      std::vector<std::vector<LaunchRec::Multi>> steps(groups.size());
+      for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const auto& gr = groups[gi];
+        if (!mergeable[gr[0]]) continue;
+        const size_t ns = recs[gr[0]].recs.size();
+        for (size_t k = 0; k < ns; ++k) {
+          std::vector<const LaunchRec*> rk;
+          for (int i : gr) rk.push_back(&recs[i].recs[k]);
+          steps[gi].push_back(rk[0]->prepare(rk, bg.keep, s0));
+        }
+      }
       GMGS_HIP(hipStreamSynchronize(s0));
       hipGraph_t g = nullptr;
       GMGS_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
       GMGS_HIP(hipEventRecord(ev[n], s0));
-      for (int i = 0; i < m; ++i) {
-        gmgs_ctx* c = act[i];
+      for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const auto& gr = groups[gi];
+        gmgs_ctx* c = act[gr[0]];
         GMGS_HIP(hipStreamWaitEvent(c->stream, ev[n], 0));
-        for (int k = 0; k < 2; ++k) {
-          if (c->mixed) pcg_iteration<float>(c, k);
-          else pcg_iteration<double>(c, k);
+        if (mergeable[gr[0]]) {
+          for (const LaunchRec::Multi& mu : steps[gi])
+            if (mu.total > 0) {
+              mu.go(mu, c->stream);
+              ++bg.launches;
+            }
+        } else {
+          for (int k = 0; k < 2; ++k) {
+            if (c->mixed) pcg_iteration<float>(c, k);
+            else pcg_iteration<double>(c, k);
+          }
+          bg.launches += (long long)recs[gr[0]].recs.size();
         }
-        GMGS_HIP(hipEventRecord(ev[i % n], c->stream));
-        GMGS_HIP(hipStreamWaitEvent(s0, ev[i % n], 0));
+        GMGS_HIP(hipEventRecord(ev[gi % n], c->stream));
+        GMGS_HIP(hipStreamWaitEvent(s0, ev[gi % n], 0));
       }
       launch_gather_flags(bg.dptrs.as<const int*>(), m, bg.dflags.as<int>(), s0);
       GMGS_HIP(hipStreamEndCapture(s0, &g));
-      GMGS_HIP(hipGraphInstantiate(&bg.ge, g, nullptr, nullptr, 0));
-      GMGS_HIP(hipGraphDestroy(g));
-      return bg;
+      const hipError_t e = hipGraphInstantiate(&bg.ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        bg.ge = nullptr;
+        GMGS_HIP(e);
+      }
+      bg.groups = (int)groups.size();
+      return batch_cache()[act] = std::move(bg);
     };
     BatchKey act(cs, cs + n);
     BatchGraph* bg = &graph_for(act);

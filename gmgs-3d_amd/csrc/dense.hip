@@ -1,5 +1,6 @@
 // Coarsest-level direct solve on the device.
 #include "dense.h"
+#include "launch.h"
 #include "mesh.h"
 
 namespace gmgs {
@@ -266,14 +267,30 @@ __global__ __launch_bounds__(256) void k_wtw(const double* __restrict__ W, doubl
 }
 
 template <typename T>
-__global__ void k_gather_free(const long long* __restrict__ code, long long nfree, long long nfp,
-                              const T* __restrict__ r, double* __restrict__ b,
-                              const int* __restrict__ done) {
+struct KGatherFree {
+  static constexpr int kThreads = BS;
+  const long long* code;
+  long long nfree;
+  long long nfp;
+  const T* r;
+  double* b;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    const long long* __restrict__ code = this->code;
+    long long nfree = this->nfree;
+    long long nfp = this->nfp;
+    const T* __restrict__ r = this->r;
+    double* __restrict__ b = this->b;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long i = (long long)block_ * blockDim.x + threadIdx.x;
   if (i >= nfp) return;
   b[i] = i < nfree ? (double)r[code[i]] : 0.0;
-}
+  }
+};
 
 constexpr int GV_ROWS = 256;
 // rows of K_ff^-1 per partial: short enough that the product's serial sums
@@ -281,34 +298,67 @@ constexpr int GV_ROWS = 256;
 constexpr int GV_CHUNK = 64;
 
 // partial[chunk][i] = sum_{k in chunk} Kinv[k][i] b[k]   (Kinv symmetric)
-__global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ Kinv, long long nfp,
-                                              const double* __restrict__ b,
-                                              double* __restrict__ partial,
-                                              const int* __restrict__ done) {
+struct KGemv {
+  static constexpr int kThreads = 256;
+  const double* Kinv;
+  long long nfp;
+  const double* b;
+  double* partial;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    const double* __restrict__ Kinv = this->Kinv;
+    long long nfp = this->nfp;
+    const double* __restrict__ b = this->b;
+    double* __restrict__ partial = this->partial;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   __shared__ double bs[GV_CHUNK];
-  const long long i = (long long)blockIdx.x * GV_ROWS + threadIdx.x;
-  const long long k0 = (long long)blockIdx.y * GV_CHUNK;
+  const long long nrb = (nfp + GV_ROWS - 1) / GV_ROWS;   // row blocks; block = row block + nrb chunk
+  const long long i = (long long)(block_ % nrb) * GV_ROWS + threadIdx.x;
+  const long long chunk = (long long)block_ / nrb;
+  const long long k0 = chunk * GV_CHUNK;
   if (threadIdx.x < GV_CHUNK) bs[threadIdx.x] = (k0 + threadIdx.x < nfp) ? b[k0 + threadIdx.x] : 0.0;
   __syncthreads();
   if (i >= nfp) return;
   double acc = 0.0;
   const long long kend = (k0 + GV_CHUNK < nfp) ? GV_CHUNK : nfp - k0;
   for (long long k = 0; k < kend; ++k) acc += Kinv[(k0 + k) * nfp + i] * bs[k];
-  partial[(long long)blockIdx.y * nfp + i] = acc;
-}
+  partial[chunk * nfp + i] = acc;
+  }
+};
 
 template <typename T>
-__global__ void k_scatter_free(const long long* __restrict__ code, long long nfree, long long nfp,
-                               int nchunks, const double* __restrict__ partial, T* __restrict__ x,
-                               const int* __restrict__ done) {
+struct KScatterFree {
+  static constexpr int kThreads = BS;
+  const long long* code;
+  long long nfree;
+  long long nfp;
+  int nchunks;
+  const double* partial;
+  T* x;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    const long long* __restrict__ code = this->code;
+    long long nfree = this->nfree;
+    long long nfp = this->nfp;
+    int nchunks = this->nchunks;
+    const double* __restrict__ partial = this->partial;
+    T* __restrict__ x = this->x;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long i = (long long)block_ * blockDim.x + threadIdx.x;
   if (i >= nfree) return;
   double v = 0.0;
   for (int c = 0; c < nchunks; ++c) v += partial[(long long)c * nfp + i];
   x[code[i]] = (T)v;
-}
+  }
+};
 
 }  // namespace
 
@@ -381,13 +431,10 @@ void coarsest_apply(const double* Kinv, long long nfp, long long nfree, const lo
                     const T* r, T* x, long long N, double* bbuf, double* partial, const int* done,
                     hipStream_t s) {
   // x is all-zero except at free DOFs: the caller keeps fixed/void entries zero
-  k_gather_free<T><<<grid1d(nfp, BS), BS, 0, s>>>(code, nfree, nfp, r, bbuf, done);
-  GMGS_LAUNCH_CHECK();
-  dim3 grid((unsigned)((nfp + GV_ROWS - 1) / GV_ROWS), (unsigned)((nfp + GV_CHUNK - 1) / GV_CHUNK));
-  k_gemv<<<grid, 256, 0, s>>>(Kinv, nfp, bbuf, partial, done);
-  GMGS_LAUNCH_CHECK();
-  k_scatter_free<T><<<grid1d(nfree, BS), BS, 0, s>>>(code, nfree, nfp, (int)grid.y, partial, x, done);
-  GMGS_LAUNCH_CHECK();
+  launch(KGatherFree<T>{code, nfree, nfp, r, bbuf, done}, grid1d(nfp, BS), s);
+  const long long nrb = (nfp + GV_ROWS - 1) / GV_ROWS, nch = (nfp + GV_CHUNK - 1) / GV_CHUNK;
+  launch(KGemv{Kinv, nfp, bbuf, partial, done}, nrb * nch, s);
+  launch(KScatterFree<T>{code, nfree, nfp, (int)nch, partial, x, done}, grid1d(nfree, BS), s);
   (void)N;
 }
 

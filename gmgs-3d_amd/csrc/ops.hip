@@ -9,6 +9,7 @@
 //   restrict, prolong: span-2 trilinear transfers (RestrictResidual.m,
 //            InterpolationDeviation.m) written on the structured grid.
 #include "ops.h"
+#include "launch.h"
 
 namespace gmgs {
 
@@ -57,16 +58,17 @@ __device__ inline LaneNode lane_node_at(const Grid& g, const uint32_t* __restric
   return r;
 }
 
-// one tile per wave, four per block, tile = 4 blockIdx + wave
+// one tile per wave, four per block, tile = 4 block + wave
 __device__ inline LaneNode lane_node(const Grid& g, const uint32_t* __restrict__ flags,
-                                     const TileRec* __restrict__ tiles, long long ntiles) {
-  return lane_node_at(g, flags, tiles, ntiles, (long long)blockIdx.x * 4 + (threadIdx.x >> 6));
+                                     const TileRec* __restrict__ tiles, long long ntiles, int block) {
+  return lane_node_at(g, flags, tiles, ntiles, (long long)block * 4 + (threadIdx.x >> 6));
 }
 
 // tile loop of the pointwise kernels: a fixed grid walks the tile list, so a
-// kernel that reduces leaves one partial per block, gridDim.x in all
+// kernel that reduces leaves one partial per block, nblocks_ in all (block_,
+// nblocks_: the kernel body's block and block count)
 #define GMGS_TILE_LOOP(g, flags, tiles, nt, ln_)                                              \
-  for (long long tb_ = (long long)blockIdx.x * 4; tb_ < (nt); tb_ += (long long)gridDim.x * 4) \
+  for (long long tb_ = (long long)block_ * 4; tb_ < (nt); tb_ += (long long)nblocks_ * 4) \
     if (const LaneNode ln_ = lane_node_at(g, flags, tiles, nt, tb_ + (threadIdx.x >> 6)); true)
 
 template <typename T, typename TR, int MODE>
@@ -82,13 +84,32 @@ struct InputU {
 };
 
 template <typename T, typename TR, int MODE, bool HAS_EOP, bool HAS_EMOD>
-__global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __restrict__ tiles,
-                                               long long ntiles, const TR* __restrict__ in, T omega,
-                                               T* __restrict__ out, TR* __restrict__ xacc,
-                                               double* __restrict__ partial,
-                                               const int* __restrict__ done) {
+struct KApply {
+  static constexpr int kThreads = 256;
+  LevelOp<T> op;
+  const TileRec* tiles;
+  long long ntiles;
+  const TR* in;
+  T omega;
+  T* out;
+  TR* xacc;
+  double* partial;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    LevelOp<T> op = this->op;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long ntiles = this->ntiles;
+    const TR* __restrict__ in = this->in;
+    T omega = this->omega;
+    T* __restrict__ out = this->out;
+    TR* __restrict__ xacc = this->xacc;
+    double* __restrict__ partial = this->partial;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
-  const LaneNode ln_ = lane_node(op.g, op.flags, tiles, ntiles);
+  const LaneNode ln_ = lane_node(op.g, op.flags, tiles, ntiles, block_);
   double dsum = 0.0;
   if (ln_.active) {
     const long long N = op.g.N;
@@ -166,9 +187,10 @@ __global__ __launch_bounds__(256) void k_apply(LevelOp<T> op, const TileRec* __r
   }
   if (partial) {
     const double b = block_sum(dsum);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    if (threadIdx.x == 0) partial[block_] = b;
   }
-}
+  }
+};
 
 // Level operator from assembled stencil rows (coarse levels).  One wave per
 // 16 x 4 x 1 node tile, four tiles per block.  The tile's 18 x 6 x 3
@@ -188,7 +210,12 @@ using cptr = const __attribute__((address_space(4))) T*;
 
 template <typename T>
 __device__ inline cptr<T> opaque(const T* p) {
-  cptr<T> q = (cptr<T>)p;
+  // (the pointer is uniform: read from the first lane so that it sits in
+  // scalar registers however it was loaded)
+  const unsigned long long u = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u),
+                 hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  cptr<T> q = (cptr<T>)(((unsigned long long)hi << 32) | lo);
   asm volatile("" : "+s"(q));
   return q;
 }
@@ -199,17 +226,36 @@ struct alignas(16) CoefGroup {
 };
 
 template <typename T, typename TR, int MODE>
-__global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* __restrict__ tiles,
-                                                  long long ntiles, const TR* __restrict__ in, T omega,
-                                                  const T* __restrict__ in2, T* __restrict__ out,
-                                                  double* __restrict__ partial,
-                                                  const int* __restrict__ done) {
+struct KApplyAs {
+  static constexpr int kThreads = 256;
+  LevelOp<T> op;
+  const TileRec* tiles;
+  long long ntiles;
+  const TR* in;
+  T omega;
+  const T* in2;
+  T* out;
+  double* partial;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    LevelOp<T> op = this->op;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long ntiles = this->ntiles;
+    const TR* __restrict__ in = this->in;
+    T omega = this->omega;
+    const T* __restrict__ in2 = this->in2;
+    T* __restrict__ out = this->out;
+    double* __restrict__ partial = this->partial;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   constexpr int SY = 18, SX = 6, NS = SY * SX * 3;   // staged nodes per tile
   constexpr int G = 16 / (int)sizeof(T), NGP = stencil_plane_groups((int)sizeof(T));
   __shared__ T sU[4][3][NS];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long long t = (long long)blockIdx.x * 4 + wave;
+  const long long t = (long long)block_ * 4 + wave;
   const Grid g = op.g;
   const long long N = g.N, sx = g.py, sz = (long long)g.py * g.px;
   double dsum = 0.0;
@@ -401,9 +447,10 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
   }
   if (partial) {
     const double b = block_sum(dsum);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    if (threadIdx.x == 0) partial[block_] = b;
   }
-}
+  }
+};
 
 // The split operator's second half: one wave per row box.  The wave stages
 // the box's neighbourhood (18 x (kRowBoxX + 2) x (kRowBoxZ + 2) nodes, u = omega r / d for
@@ -412,18 +459,39 @@ __global__ __launch_bounds__(256) void k_apply_as(LevelOp<T> op, const TileRec* 
 // its node's 27 neighbours by the node's assembled row, in the row's
 // coefficient order (the sums of k_apply_as).
 template <typename T, typename TR, int MODE>
-__global__ __launch_bounds__(256) void k_apply_rows(LevelOp<T> op, const RowTile* __restrict__ boxes,
-                                                    const RowEntry* __restrict__ entries, long long nbox,
-                                                    const TR* __restrict__ in, T omega,
-                                                    const T* __restrict__ in2, T* __restrict__ out,
-                                                    double* __restrict__ partial, const int* __restrict__ done) {
+struct KApplyRows {
+  static constexpr int kThreads = 256;
+  LevelOp<T> op;
+  const RowTile* boxes;
+  const RowEntry* entries;
+  long long nbox;
+  const TR* in;
+  T omega;
+  const T* in2;
+  T* out;
+  double* partial;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    LevelOp<T> op = this->op;
+    const RowTile* __restrict__ boxes = this->boxes;
+    const RowEntry* __restrict__ entries = this->entries;
+    long long nbox = this->nbox;
+    const TR* __restrict__ in = this->in;
+    T omega = this->omega;
+    const T* __restrict__ in2 = this->in2;
+    T* __restrict__ out = this->out;
+    double* __restrict__ partial = this->partial;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   constexpr int SY = 18, SX = kRowBoxX + 2, SZ = kRowBoxZ + 2, NP = SY * SX * SZ, NS = 3 * NP;
   constexpr int G = 16 / (int)sizeof(T), NGP = stencil_plane_groups((int)sizeof(T));
   constexpr int BATCH = 16;
   __shared__ T sU[4][NS];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long long r = (long long)blockIdx.x * 4 + wave;
+  const long long r = (long long)block_ * 4 + wave;
   const Grid g = op.g;
   const long long N = g.N;
   double dsum = 0.0;
@@ -516,9 +584,10 @@ __global__ __launch_bounds__(256) void k_apply_rows(LevelOp<T> op, const RowTile
   }
   if (partial) {
     const double b = block_sum(dsum);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    if (threadIdx.x == 0) partial[block_] = b;
   }
-}
+  }
+};
 
 // canonical FP64 rows [q][row] (stride 'rows') -> the row-major layout
 // [u][q'] of k_apply_as in T for the rows rep[0..nu): q' = (q / 81) QP +
@@ -570,15 +639,29 @@ __global__ void k_row_verify(const double* __restrict__ src, long long rows, con
 // same element sum as k_apply (universal or stored element matrices, SIMP
 // scale), accumulated per neighbour instead of multiplied by the input
 template <bool HAS_EOP, bool HAS_EMOD>
-__global__ __launch_bounds__(256) void k_assemble_stencil(LevelOp<double> op,
-                                                          const TileRec* __restrict__ tiles,
-                                                          long long ntiles, const int* __restrict__ arow,
-                                                          long long stride, double* __restrict__ asc) {
-  if (blockIdx.x == 0 && threadIdx.x < kStencilCoefs)
+struct KAssembleStencil {
+  static constexpr int kThreads = 256;
+  LevelOp<double> op;
+  const TileRec* tiles;
+  long long ntiles;
+  const int* arow;
+  long long stride;
+  double* asc;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    LevelOp<double> op = this->op;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long ntiles = this->ntiles;
+    const int* __restrict__ arow = this->arow;
+    long long stride = this->stride;
+    double* __restrict__ asc = this->asc;
+    (void)block_;
+    (void)nblocks_;
+
+  if (block_ == 0 && threadIdx.x < kStencilCoefs)
     asc[(long long)threadIdx.x * stride + stride - 1] = op.stencil[threadIdx.x];
-  const LaneNode ln_ = lane_node(op.g, op.flags, tiles, ntiles);
+  const LaneNode ln_ = lane_node(op.g, op.flags, tiles, ntiles, block_);
   if (!ln_.active || (ln_.fl & F_FAST)) return;
-  const long long slot = arow[((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63)];
+  const long long slot = arow[((long long)block_ * 4 + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63)];
   const long long n = ln_.n;
   const long long sx = op.g.py, sz = (long long)op.g.py * op.g.px;
   for (int p = 0; p < 8; ++p) {
@@ -600,15 +683,34 @@ __global__ __launch_bounds__(256) void k_assemble_stencil(LevelOp<double> op,
           asc[(long long)(9 * o + 3 * c + d) * stride + slot] += sc * K[(3 * ln + c) * 24 + 3 * m + d];
     }
   }
-}
+  }
+};
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_restrict(Grid gc, const uint32_t* __restrict__ flags,
-                                                  const TileRec* __restrict__ tiles, long long nt,
-                                                  Grid gf, const T* __restrict__ rf,
-                                                  T* __restrict__ rc, const int* __restrict__ done) {
+struct KRestrict {
+  static constexpr int kThreads = 256;
+  Grid gc;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  Grid gf;
+  const T* rf;
+  T* rc;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid gc = this->gc;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    Grid gf = this->gf;
+    const T* __restrict__ rf = this->rf;
+    T* __restrict__ rc = this->rc;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
-  const LaneNode ln_ = lane_node(gc, flags, tiles, nt);
+  const LaneNode ln_ = lane_node(gc, flags, tiles, nt, block_);
   if (!ln_.active) return;
   const long long Nf = gf.N;
   const long long sx = gf.py, sz = (long long)gf.py * gf.px;
@@ -627,14 +729,46 @@ __global__ __launch_bounds__(256) void k_restrict(Grid gc, const uint32_t* __res
   rc[ln_.n] = a0;
   rc[N + ln_.n] = a1;
   rc[2 * N + ln_.n] = a2;
-}
+  }
+};
 
 template <typename T, typename TR, typename TZ>
-__global__ __launch_bounds__(256) void k_prolong_smooth(
-    Grid gf, const uint32_t* __restrict__ flags, const TileRec* __restrict__ tiles, long long nt,
-    Grid gc, const T* __restrict__ xc, const T* __restrict__ invd, const TR* __restrict__ r,
-    const T* __restrict__ rp, T omega, TZ* __restrict__ x, int fix, const TZ* __restrict__ zprev,
-    double* __restrict__ partial, const int* __restrict__ done) {
+struct KProlongSmooth {
+  static constexpr int kThreads = 256;
+  Grid gf;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  Grid gc;
+  const T* xc;
+  const T* invd;
+  const TR* r;
+  const T* rp;
+  T omega;
+  TZ* x;
+  int fix;
+  const TZ* zprev;
+  double* partial;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid gf = this->gf;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    Grid gc = this->gc;
+    const T* __restrict__ xc = this->xc;
+    const T* __restrict__ invd = this->invd;
+    const TR* __restrict__ r = this->r;
+    const T* __restrict__ rp = this->rp;
+    T omega = this->omega;
+    TZ* __restrict__ x = this->x;
+    int fix = this->fix;
+    const TZ* __restrict__ zprev = this->zprev;
+    double* __restrict__ partial = this->partial;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   double d0 = 0.0, d1 = 0.0;
   GMGS_TILE_LOOP(gf, flags, tiles, nt, ln_) {
@@ -673,19 +807,40 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(
     const double b0 = block_sum(d0);
     const double b1 = block_sum(d1);
     if (threadIdx.x == 0) {
-      partial[blockIdx.x] = b0;
-      partial[gridDim.x + blockIdx.x] = b1;
+      partial[block_] = b0;
+      partial[nblocks_ + block_] = b1;
     }
   }
-}
+  }
+};
 
 // e = P xc on the fine level's solid nodes (the prolongation of
 // kernel/InterpolationDeviation.m:3-17 on the padded grids)
 template <typename T>
-__global__ __launch_bounds__(256) void k_prolong(Grid gf, const uint32_t* __restrict__ flags,
-                                                 const TileRec* __restrict__ tiles, long long nt,
-                                                 Grid gc, const T* __restrict__ xc, T* __restrict__ e,
-                                                 int mask, const int* __restrict__ done) {
+struct KProlong {
+  static constexpr int kThreads = 256;
+  Grid gf;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  Grid gc;
+  const T* xc;
+  T* e;
+  int mask;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid gf = this->gf;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    Grid gc = this->gc;
+    const T* __restrict__ xc = this->xc;
+    T* __restrict__ e = this->e;
+    int mask = this->mask;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   GMGS_TILE_LOOP(gf, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -710,12 +865,28 @@ __global__ __launch_bounds__(256) void k_prolong(Grid gf, const uint32_t* __rest
 #pragma unroll
     for (int c = 0; c < 3; ++c) e[c * N + n] = (mask && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) ? (T)0 : p[c];
   }
-}
+  }
+};
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_mask_fixed(Grid g, const uint32_t* __restrict__ flags,
-                                                    const TileRec* __restrict__ tiles, long long nt,
-                                                    const T* __restrict__ src, T* __restrict__ dst) {
+struct KMaskFixed {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const T* src;
+  T* dst;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const T* __restrict__ src = this->src;
+    T* __restrict__ dst = this->dst;
+    (void)block_;
+    (void)nblocks_;
+
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -723,24 +894,66 @@ __global__ __launch_bounds__(256) void k_mask_fixed(Grid g, const uint32_t* __re
     for (int c = 0; c < 3; ++c)
       dst[c * N + ln_.n] = (ln_.fl & (1u << (F_FIX_SHIFT + c))) ? (T)0 : src[c * N + ln_.n];
   }
-}
+  }
+};
 
 // x = (sum_s w_s (R_s + Q_s)) / d + e on the solid nodes, R_0 = r0, s < NU;
 // zero fixed components if 'fix'; partials of r0.x and r0.zprev.  The terms
 // are separate restrict arguments so the loads of all three components can
 // be issued ahead of the stores.
 template <typename T, typename TR, typename TZ, int NU>
-__global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restrict__ flags,
-                                                 const TileRec* __restrict__ tiles, long long nt,
-                                                 const T* __restrict__ invd, T w0, T w1, T w2, T w3,
-                                                 const TR* __restrict__ r0, const T* __restrict__ q0,
-                                                 const T* __restrict__ r1, const T* __restrict__ q1,
-                                                 const T* __restrict__ r2, const T* __restrict__ q2,
-                                                 const T* __restrict__ r3, const T* __restrict__ q3,
-                                                 const T* __restrict__ e, TZ* __restrict__ x, int fix,
-                                                 const TZ* __restrict__ zprev,
-                                                 double* __restrict__ partial,
-                                                 const int* __restrict__ done) {
+struct KCombine {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const T* invd;
+  T w0;
+  T w1;
+  T w2;
+  T w3;
+  const TR* r0;
+  const T* q0;
+  const T* r1;
+  const T* q1;
+  const T* r2;
+  const T* q2;
+  const T* r3;
+  const T* q3;
+  const T* e;
+  TZ* x;
+  int fix;
+  const TZ* zprev;
+  double* partial;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const T* __restrict__ invd = this->invd;
+    T w0 = this->w0;
+    T w1 = this->w1;
+    T w2 = this->w2;
+    T w3 = this->w3;
+    const TR* __restrict__ r0 = this->r0;
+    const T* __restrict__ q0 = this->q0;
+    const T* __restrict__ r1 = this->r1;
+    const T* __restrict__ q1 = this->q1;
+    const T* __restrict__ r2 = this->r2;
+    const T* __restrict__ q2 = this->q2;
+    const T* __restrict__ r3 = this->r3;
+    const T* __restrict__ q3 = this->q3;
+    const T* __restrict__ e = this->e;
+    TZ* __restrict__ x = this->x;
+    int fix = this->fix;
+    const TZ* __restrict__ zprev = this->zprev;
+    double* __restrict__ partial = this->partial;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   double d0 = 0.0, d1 = 0.0;
   const long long N = g.N;
@@ -772,18 +985,35 @@ __global__ __launch_bounds__(256) void k_combine(Grid g, const uint32_t* __restr
     const double b0 = block_sum(d0);
     const double b1 = block_sum(d1);
     if (threadIdx.x == 0) {
-      partial[blockIdx.x] = b0;
-      partial[gridDim.x + blockIdx.x] = b1;
+      partial[block_] = b0;
+      partial[nblocks_ + block_] = b1;
     }
   }
-}
+  }
+};
 
 // out = a - out on the solid nodes
 template <typename T>
-__global__ __launch_bounds__(256) void k_rsub(Grid g, const uint32_t* __restrict__ flags,
-                                              const TileRec* __restrict__ tiles, long long nt,
-                                              const T* __restrict__ a, T* __restrict__ out,
-                                              const int* __restrict__ done) {
+struct KRsub {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const T* a;
+  T* out;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const T* __restrict__ a = this->a;
+    T* __restrict__ out = this->out;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
@@ -797,15 +1027,34 @@ __global__ __launch_bounds__(256) void k_rsub(Grid g, const uint32_t* __restrict
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c * N + ln_.n] = av[c] - ov[c];
   }
-}
+  }
+};
 
 template <typename TZ>
-__global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __restrict__ flags,
-                                                  const TileRec* __restrict__ tiles, long long nt,
-                                                  const TZ* __restrict__ z, double* __restrict__ p,
-                                                  double* __restrict__ x,
-                                                  const PcgScalars* __restrict__ sc,
-                                                  const int* __restrict__ done) {
+struct KUpdateP {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const TZ* z;
+  double* p;
+  double* x;
+  const PcgScalars* sc;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const TZ* __restrict__ z = this->z;
+    double* __restrict__ p = this->p;
+    double* __restrict__ x = this->x;
+    const PcgScalars* __restrict__ sc = this->sc;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   const double beta = sc->beta, alpha = sc->alpha;
   const long long N = g.N;
@@ -828,14 +1077,32 @@ __global__ __launch_bounds__(256) void k_update_p(Grid g, const uint32_t* __rest
       p[c * N + n] = (double)zv[c] + beta * po[c];
     }
   }
-}
+  }
+};
 
 // x += alpha p (the last iteration's deferred x update; runs after the loop)
-__global__ __launch_bounds__(256) void k_add_alpha_p(Grid g, const uint32_t* __restrict__ flags,
-                                                     const TileRec* __restrict__ tiles, long long nt,
-                                                     double* __restrict__ x, const double* __restrict__ p,
-                                                     const double* __restrict__ p_odd,
-                                                     const PcgScalars* __restrict__ sc) {
+struct KAddAlphaP {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  double* x;
+  const double* p;
+  const double* p_odd;
+  const PcgScalars* sc;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    double* __restrict__ x = this->x;
+    const double* __restrict__ p = this->p;
+    const double* __restrict__ p_odd = this->p_odd;
+    const PcgScalars* __restrict__ sc = this->sc;
+    (void)block_;
+    (void)nblocks_;
+
   const double alpha = sc->alpha;
   // directions alternating between two buffers: iteration k (its = k + 1
   // after it) wrote p[k & 1]
@@ -853,7 +1120,8 @@ __global__ __launch_bounds__(256) void k_add_alpha_p(Grid g, const uint32_t* __r
 #pragma unroll
     for (int c = 0; c < 3; ++c) x[c * N + n] = xo[c] + alpha * po[c];
   }
-}
+  }
+};
 
 // Padded index of this lane's node in tile tr (0, a halo entry, for a lane
 // beyond the grid's x extent)
@@ -872,21 +1140,46 @@ __device__ inline long long lane_index(const Grid& g, const TileRec& tr, bool& o
 // zero and read as zero): only the stores and the norm look at the flags.
 // Each lane visits its tiles in the same order as with one tile per pass, so
 // the partial sums are unchanged.
-__global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __restrict__ flags,
-                                                   const TileRec* __restrict__ tiles, long long nt,
-                                                   double* __restrict__ x, const double* __restrict__ p,
-                                                   const double* ra, const double* __restrict__ q,
-                                                   double* r, const double* __restrict__ invd,
-                                                   double omega,
-                                                   const PcgScalars* __restrict__ sc,
-                                                   double* __restrict__ partial,
-                                                   const int* __restrict__ done) {
+struct KUpdateXr {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  double* x;
+  const double* p;
+  const double* ra;
+  const double* q;
+  double* r;
+  const double* invd;
+  double omega;
+  const PcgScalars* sc;
+  double* partial;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    double* __restrict__ x = this->x;
+    const double* __restrict__ p = this->p;
+    const double* ra = this->ra;
+    const double* __restrict__ q = this->q;
+    double* r = this->r;
+    const double* __restrict__ invd = this->invd;
+    double omega = this->omega;
+    const PcgScalars* __restrict__ sc = this->sc;
+    double* __restrict__ partial = this->partial;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   double d = 0.0;
   const double alpha = sc->alpha;
   const long long N = g.N;
-  const long long stride = (long long)gridDim.x * 4;
-  for (long long t0 = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); t0 < nt; t0 += 2 * stride) {
+  const long long stride = (long long)nblocks_ * 4;
+  for (long long t0 = (long long)block_ * 4 + (threadIdx.x >> 6); t0 < nt; t0 += 2 * stride) {
     const long long t1 = t0 + stride;
     const bool has1 = t1 < nt;
     const TileRec tr0 = tiles[t0];
@@ -930,15 +1223,34 @@ __global__ __launch_bounds__(256) void k_update_xr(Grid g, const uint32_t* __res
     }
   }
   const double b = block_sum(d);
-  if (threadIdx.x == 0) partial[blockIdx.x] = b;
-}
+  if (threadIdx.x == 0) partial[block_] = b;
+  }
+};
 
 // x += omega r / d on the solid nodes
 template <typename T, typename TR>
-__global__ __launch_bounds__(256) void k_jacobi_acc(Grid g, const uint32_t* __restrict__ flags,
-                                                    const TileRec* __restrict__ tiles, long long nt,
-                                                    const T* __restrict__ invd, const TR* __restrict__ r,
-                                                    T omega, TR* __restrict__ x) {
+struct KJacobiAcc {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const T* invd;
+  const TR* r;
+  T omega;
+  TR* x;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const T* __restrict__ invd = this->invd;
+    const TR* __restrict__ r = this->r;
+    T omega = this->omega;
+    TR* __restrict__ x = this->x;
+    (void)block_;
+    (void)nblocks_;
+
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -946,15 +1258,29 @@ __global__ __launch_bounds__(256) void k_jacobi_acc(Grid g, const uint32_t* __re
 #pragma unroll
     for (int c = 0; c < 3; ++c) x[c * N + n] += (TR)(omega * invd[c * N + n] * (T)r[c * N + n]);
   }
-}
+  }
+};
 
 // v = values in [-1, 1) from a fixed integer hash of (padded index, component)
 // on the free components of the solid nodes (the lowbias32 mixer of
 // C. Wellons' hash-prospector: xorshift-multiply rounds)
 template <typename T>
-__global__ __launch_bounds__(256) void k_seed(Grid g, const uint32_t* __restrict__ flags,
-                                              const TileRec* __restrict__ tiles, long long nt,
-                                              T* __restrict__ v) {
+struct KSeed {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  T* v;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    T* __restrict__ v = this->v;
+    (void)block_;
+    (void)nblocks_;
+
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -970,15 +1296,37 @@ __global__ __launch_bounds__(256) void k_seed(Grid g, const uint32_t* __restrict
       v[c * N + ln_.n] = (ln_.fl & (1u << (F_FIX_SHIFT + c))) ? (T)0 : val;
     }
   }
-}
+  }
+};
 
 // r -= alpha q (q null: r unchanged), z = invd r; partials of r.z
 template <typename T>
-__global__ __launch_bounds__(256) void k_jcg_step(Grid g, const uint32_t* __restrict__ flags,
-                                                  const TileRec* __restrict__ tiles, long long nt,
-                                                  const T* __restrict__ invd, const T* __restrict__ q,
-                                                  double alpha, T* __restrict__ r, T* __restrict__ z,
-                                                  double* __restrict__ partial) {
+struct KJcgStep {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const T* invd;
+  const T* q;
+  double alpha;
+  T* r;
+  T* z;
+  double* partial;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const T* __restrict__ invd = this->invd;
+    const T* __restrict__ q = this->q;
+    double alpha = this->alpha;
+    T* __restrict__ r = this->r;
+    T* __restrict__ z = this->z;
+    double* __restrict__ partial = this->partial;
+    (void)block_;
+    (void)nblocks_;
+
   double d = 0.0;
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
@@ -997,14 +1345,32 @@ __global__ __launch_bounds__(256) void k_jcg_step(Grid g, const uint32_t* __rest
     }
   }
   const double b = block_sum(d);
-  if (threadIdx.x == 0) partial[blockIdx.x] = b;
-}
+  if (threadIdx.x == 0) partial[block_] = b;
+  }
+};
 
 // p = z + beta p (beta = 0: p = z)
 template <typename T>
-__global__ __launch_bounds__(256) void k_xpby(Grid g, const uint32_t* __restrict__ flags,
-                                              const TileRec* __restrict__ tiles, long long nt,
-                                              const T* __restrict__ z, double beta, T* __restrict__ p) {
+struct KXpby {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const T* z;
+  double beta;
+  T* p;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const T* __restrict__ z = this->z;
+    double beta = this->beta;
+    T* __restrict__ p = this->p;
+    (void)block_;
+    (void)nblocks_;
+
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -1014,13 +1380,29 @@ __global__ __launch_bounds__(256) void k_xpby(Grid g, const uint32_t* __restrict
       p[m] = beta == 0.0 ? z[m] : (T)((double)z[m] + beta * (double)p[m]);
     }
   }
-}
+  }
+};
 
-__global__ __launch_bounds__(256) void k_dot(Grid g, const uint32_t* __restrict__ flags,
-                                             const TileRec* __restrict__ tiles, long long nt,
-                                             const double* __restrict__ a,
-                                             const double* __restrict__ b,
-                                             double* __restrict__ partial) {
+struct KDot {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const double* a;
+  const double* b;
+  double* partial;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const double* __restrict__ a = this->a;
+    const double* __restrict__ b = this->b;
+    double* __restrict__ partial = this->partial;
+    (void)block_;
+    (void)nblocks_;
+
   double d = 0.0;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -1029,20 +1411,44 @@ __global__ __launch_bounds__(256) void k_dot(Grid g, const uint32_t* __restrict_
     for (int c = 0; c < 3; ++c) d += a[c * N + n] * b[c * N + n];
   }
   const double s = block_sum(d);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
+  if (threadIdx.x == 0) partial[block_] = s;
+  }
+};
 
 // start of a solve (CG_solver.m:11-13 with x0 = 0): x = p = 0, z0 = z1 = 0,
 // r = b, partials of b.b; solid nodes only (void entries are never written
 // and stay zero)
 template <typename TZ>
-__global__ __launch_bounds__(256) void k_pcg_start(Grid g, const uint32_t* __restrict__ flags,
-                                                   const TileRec* __restrict__ tiles, long long nt,
-                                                   const double* __restrict__ b, double* __restrict__ x,
-                                                   double* __restrict__ p, double* __restrict__ p2,
-                                                   double* __restrict__ r,
-                                                   TZ* __restrict__ z0, TZ* __restrict__ z1,
-                                                   double* __restrict__ partial) {
+struct KPcgStart {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const double* b;
+  double* x;
+  double* p;
+  double* p2;
+  double* r;
+  TZ* z0;
+  TZ* z1;
+  double* partial;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const double* __restrict__ b = this->b;
+    double* __restrict__ x = this->x;
+    double* __restrict__ p = this->p;
+    double* __restrict__ p2 = this->p2;
+    double* __restrict__ r = this->r;
+    TZ* __restrict__ z0 = this->z0;
+    TZ* __restrict__ z1 = this->z1;
+    double* __restrict__ partial = this->partial;
+    (void)block_;
+    (void)nblocks_;
+
   double d = 0.0;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -1061,15 +1467,31 @@ __global__ __launch_bounds__(256) void k_pcg_start(Grid g, const uint32_t* __res
     }
   }
   const double s = block_sum(d);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
+  if (threadIdx.x == 0) partial[block_] = s;
+  }
+};
 
 // partials of |b - q|^2 (the true residual of x when q = A x)
-__global__ __launch_bounds__(256) void k_resid_sq(Grid g, const uint32_t* __restrict__ flags,
-                                                  const TileRec* __restrict__ tiles, long long nt,
-                                                  const double* __restrict__ b,
-                                                  const double* __restrict__ q,
-                                                  double* __restrict__ partial) {
+struct KResidSq {
+  static constexpr int kThreads = 256;
+  Grid g;
+  const uint32_t* flags;
+  const TileRec* tiles;
+  long long nt;
+  const double* b;
+  const double* q;
+  double* partial;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    Grid g = this->g;
+    const uint32_t* __restrict__ flags = this->flags;
+    const TileRec* __restrict__ tiles = this->tiles;
+    long long nt = this->nt;
+    const double* __restrict__ b = this->b;
+    const double* __restrict__ q = this->q;
+    double* __restrict__ partial = this->partial;
+    (void)block_;
+    (void)nblocks_;
+
   double d = 0.0;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
@@ -1081,8 +1503,9 @@ __global__ __launch_bounds__(256) void k_resid_sq(Grid g, const uint32_t* __rest
     }
   }
   const double s = block_sum(d);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
+  if (threadIdx.x == 0) partial[block_] = s;
+  }
+};
 
 // one block of 256 threads; fixed-order sums (thread t adds part[t],
 // part[t + 256], ... in turn; the loads are issued eight at a time so the
@@ -1110,11 +1533,26 @@ __device__ inline double reduce_parts(const double* __restrict__ part, int n) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void k_finalize(int which, const double* __restrict__ part,
-                                                  int nparts, PcgScalars* __restrict__ sc,
-                                                  double* __restrict__ hist_rel,
-                                                  unsigned long long* __restrict__ hist_t,
-                                                  const int* __restrict__ done) {
+struct KFinalize {
+  static constexpr int kThreads = 256;
+  int which;
+  const double* part;
+  int nparts;
+  PcgScalars* sc;
+  double* hist_rel;
+  unsigned long long* hist_t;
+  const int* done;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    int which = this->which;
+    const double* __restrict__ part = this->part;
+    int nparts = this->nparts;
+    PcgScalars* __restrict__ sc = this->sc;
+    double* __restrict__ hist_rel = this->hist_rel;
+    unsigned long long* __restrict__ hist_t = this->hist_t;
+    const int* __restrict__ done = this->done;
+    (void)block_;
+    (void)nblocks_;
+
   if (done && *done) return;
   if (which == 0) {
     const double s = reduce_parts(part, nparts);
@@ -1161,7 +1599,8 @@ __global__ __launch_bounds__(256) void k_finalize(int which, const double* __res
       sc->rz_prev = sc->rz;
     }
   }
-}
+  }
+};
 
 template <typename TS, typename TD>
 __global__ void k_convert(const TS* __restrict__ a, TD* __restrict__ b, long long n) {
@@ -1314,16 +1753,16 @@ void launch_apply(const LevelOp<T>& op, const TileRec* tiles, long long ntiles, 
   const bool eop = op.eop != nullptr, emod = op.emod != nullptr;
   if (op.asc && !xacc && mode != AP_RESID) {
     if (mode == AP_AX)
-      k_apply_as<T, TR, AP_AX><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, nullptr, out, partial,
-                                                          done);
+      launch(KApplyAs<T, TR, AP_AX>{op, tiles, ntiles, in, omega, nullptr, out, partial,
+                                                          done}, nb, s);
     else
-      k_apply_as<T, TR, AP_SMOOTH><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, nullptr, out,
-                                                              partial, done);
+      launch(KApplyAs<T, TR, AP_SMOOTH>{op, tiles, ntiles, in, omega, nullptr, out,
+                                                              partial, done}, nb, s);
     GMGS_LAUNCH_CHECK();
     return;
   }
-#define GMGS_APPLY(MODE, E, M)                                                                 \
-  k_apply<T, TR, MODE, E, M><<<nb, 256, 0, s>>>(op, tiles, ntiles, in, omega, out, xacc, partial, done)
+#define GMGS_APPLY(MODE, E, M) \
+  launch(KApply<T, TR, MODE, E, M>{op, tiles, ntiles, in, omega, out, xacc, partial, done}, nb, s)
   if (mode == AP_AX) {
     if (eop) GMGS_APPLY(AP_AX, true, false);
     else if (emod) GMGS_APPLY(AP_AX, false, true);
@@ -1346,13 +1785,13 @@ int launch_apply_rows(const LevelOp<T>& op, const RowTile* boxes, const RowEntry
   if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("apply_rows: grid too large");
   const unsigned nb = apply_blocks(nbox);
   if (mode == AP_AX)
-    k_apply_rows<T, TR, AP_AX><<<nb, 256, 0, s>>>(op, boxes, entries, nbox, in, omega, in2, out, partial, done);
+    launch(KApplyRows<T, TR, AP_AX>{op, boxes, entries, nbox, in, omega, in2, out, partial, done}, nb, s);
   else if (mode == AP_RESID)
-    k_apply_rows<T, TR, AP_RESID><<<nb, 256, 0, s>>>(op, boxes, entries, nbox, in, omega, in2, out, partial,
-                                                     done);
+    launch(KApplyRows<T, TR, AP_RESID>{op, boxes, entries, nbox, in, omega, in2, out, partial,
+                                                     done}, nb, s);
   else if (mode == AP_SMOOTH)
-    k_apply_rows<T, TR, AP_SMOOTH><<<nb, 256, 0, s>>>(op, boxes, entries, nbox, in, omega, in2, out, partial,
-                                                      done);
+    launch(KApplyRows<T, TR, AP_SMOOTH>{op, boxes, entries, nbox, in, omega, in2, out, partial,
+                                                      done}, nb, s);
   else
     throw std::invalid_argument("apply_rows: mode");
   GMGS_LAUNCH_CHECK();
@@ -1400,9 +1839,9 @@ void launch_assemble_stencil(const LevelOp<double>& op, const TileRec* tiles, lo
   if (ntiles <= 0) return;
   const unsigned nb = apply_blocks(ntiles);
   if (op.eop && op.emod) throw std::invalid_argument("assemble_stencil: eop and emod together");
-  if (op.eop) k_assemble_stencil<true, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, arow, stride, asc);
-  else if (op.emod) k_assemble_stencil<false, true><<<nb, 256, 0, s>>>(op, tiles, ntiles, arow, stride, asc);
-  else k_assemble_stencil<false, false><<<nb, 256, 0, s>>>(op, tiles, ntiles, arow, stride, asc);
+  if (op.eop) launch(KAssembleStencil<true, false>{op, tiles, ntiles, arow, stride, asc}, nb, s);
+  else if (op.emod) launch(KAssembleStencil<false, true>{op, tiles, ntiles, arow, stride, asc}, nb, s);
+  else launch(KAssembleStencil<false, false>{op, tiles, ntiles, arow, stride, asc}, nb, s);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1410,7 +1849,7 @@ template <typename T>
 void launch_restrict(const Grid& gc, const uint32_t* flags_c, const TileRec* tiles_c, long long nt,
                      const Grid& gf, const T* rf, T* rc, const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_restrict<T><<<apply_blocks(nt), 256, 0, s>>>(gc, flags_c, tiles_c, nt, gf, rf, rc, done);
+  launch(KRestrict<T>{gc, flags_c, tiles_c, nt, gf, rf, rc, done}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1420,9 +1859,9 @@ void launch_prolong_smooth(const Grid& gf, const uint32_t* flags_f, const TileRe
                            T omega, TZ* x, bool fix, const TZ* zprev, double* partial,
                            const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_prolong_smooth<T, TR, TZ><<<grid_for(nt, partial), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, invd, r,
+  launch(KProlongSmooth<T, TR, TZ>{gf, flags_f, tiles, nt, gc, xc, invd, r,
                                                                rp, omega, x, fix ? 1 : 0, zprev,
-                                                               partial, done);
+                                                               partial, done}, grid_for(nt, partial), s);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1430,7 +1869,7 @@ template <typename T>
 void launch_rsub(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, const T* a,
                  T* out, const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_rsub<T><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, out, done);
+  launch(KRsub<T>{g, flags, tiles, nt, a, out, done}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_rsub<double>(const Grid&, const uint32_t*, const TileRec*, long long, const double*,
@@ -1442,7 +1881,7 @@ template <typename T>
 void launch_prolong(const Grid& gf, const uint32_t* flags_f, const TileRec* tiles, long long nt,
                     const Grid& gc, const T* xc, T* e, bool mask, const int* done, hipStream_t s) {
   if (nt <= 0) return;
-  k_prolong<T><<<apply_blocks(nt), 256, 0, s>>>(gf, flags_f, tiles, nt, gc, xc, e, mask ? 1 : 0, done);
+  launch(KProlong<T>{gf, flags_f, tiles, nt, gc, xc, e, mask ? 1 : 0, done}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_prolong<double>(const Grid&, const uint32_t*, const TileRec*, long long, const Grid&,
@@ -1454,7 +1893,7 @@ template <typename T>
 void launch_mask_fixed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                        const T* src, T* dst, hipStream_t s) {
   if (nt <= 0) return;
-  k_mask_fixed<T><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, src, dst);
+  launch(KMaskFixed<T>{g, flags, tiles, nt, src, dst}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_mask_fixed<double>(const Grid&, const uint32_t*, const TileRec*, long long,
@@ -1470,9 +1909,10 @@ void launch_combine(const Grid& g, const uint32_t* flags, const TileRec* tiles, 
   const unsigned nb = grid_for(nt, partial);
   const int f = fix ? 1 : 0;
 #define GMGS_COMBINE(NU)                                                                            \
-  k_combine<T, TR, TZ, NU><<<nb, 256, 0, s>>>(g, flags, tiles, nt, invd, cs.w[0], cs.w[1], cs.w[2],  \
-                                              cs.w[3], r0, cs.Q[0], cs.R[1], cs.Q[1], cs.R[2],       \
-                                              cs.Q[2], cs.R[3], cs.Q[3], e, x, f, zprev, partial, done)
+  launch(KCombine<T, TR, TZ, NU>{g, flags, tiles, nt, invd, cs.w[0], cs.w[1], cs.w[2],         \
+                                 cs.w[3], r0, cs.Q[0], cs.R[1], cs.Q[1], cs.R[2], cs.Q[2], cs.R[3], \
+                                 cs.Q[3], e, x, f, zprev, partial, done},                           \
+         nb, s)
   switch (cs.nu) {
     case 1: GMGS_COMBINE(1); break;
     case 2: GMGS_COMBINE(2); break;
@@ -1496,14 +1936,14 @@ template <typename TZ>
 void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                      const TZ* z, double* p, const PcgScalars* sc, const int* done, hipStream_t s,
                      double* x) {
-  k_update_p<TZ><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, p, x, sc, done);
+  launch(KUpdateP<TZ>{g, flags, tiles, nt, z, p, x, sc, done}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 
 void launch_add_alpha_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, double* x,
                         const double* p, const double* p_odd, const PcgScalars* sc, hipStream_t s) {
   if (nt <= 0) return;
-  k_add_alpha_p<<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, p_odd, sc);
+  launch(KAddAlphaP{g, flags, tiles, nt, x, p, p_odd, sc}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1511,8 +1951,8 @@ void launch_update_xr(const Grid& g, const uint32_t* flags, const TileRec* tiles
                       double* x, const double* p, const double* ra, const double* q, double* r,
                       const double* invd, double omega, const PcgScalars* sc, double* partial,
                       const int* done, hipStream_t s) {
-  k_update_xr<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, x, p, ra, q, r, invd, omega, sc,
-                                                   partial, done);
+  launch(KUpdateXr{g, flags, tiles, nt, x, p, ra, q, r, invd, omega, sc,
+                                                   partial, done}, pointwise_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1520,7 +1960,7 @@ template <typename T, typename TR>
 void launch_jacobi_acc(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                        const T* invd, const TR* r, T omega, TR* x, hipStream_t s) {
   if (nt <= 0) return;
-  k_jacobi_acc<T, TR><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, r, omega, x);
+  launch(KJacobiAcc<T, TR>{g, flags, tiles, nt, invd, r, omega, x}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_jacobi_acc<double, double>(const Grid&, const uint32_t*, const TileRec*, long long,
@@ -1535,7 +1975,7 @@ template <typename T>
 void launch_seed(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, T* v,
                  hipStream_t s) {
   if (nt <= 0) return;
-  k_seed<T><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, v);
+  launch(KSeed<T>{g, flags, tiles, nt, v}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 template <typename T>
@@ -1543,14 +1983,14 @@ void launch_jcg_step(const Grid& g, const uint32_t* flags, const TileRec* tiles,
                      const T* invd, const T* q, double alpha, T* r, T* z, double* partial,
                      hipStream_t s) {
   if (nt <= 0) return;
-  k_jcg_step<T><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, invd, q, alpha, r, z, partial);
+  launch(KJcgStep<T>{g, flags, tiles, nt, invd, q, alpha, r, z, partial}, pointwise_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 template <typename T>
 void launch_xpby(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, const T* z,
                  double beta, T* p, hipStream_t s) {
   if (nt <= 0) return;
-  k_xpby<T><<<apply_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, z, beta, p);
+  launch(KXpby<T>{g, flags, tiles, nt, z, beta, p}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 #define GMGS_INST_LANCZOS(T)                                                                          \
@@ -1565,7 +2005,7 @@ GMGS_INST_LANCZOS(float)
 
 void launch_dot(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                 const double* a, const double* b, double* partial, hipStream_t s) {
-  k_dot<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, a, b, partial);
+  launch(KDot{g, flags, tiles, nt, a, b, partial}, pointwise_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 
@@ -1573,7 +2013,7 @@ template <typename TZ>
 void launch_pcg_start(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                       const double* b, double* x, double* p, double* p2, double* r, TZ* z0, TZ* z1,
                       double* partial, hipStream_t s) {
-  k_pcg_start<TZ><<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, b, x, p, p2, r, z0, z1, partial);
+  launch(KPcgStart<TZ>{g, flags, tiles, nt, b, x, p, p2, r, z0, z1, partial}, pointwise_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 template void launch_pcg_start<double>(const Grid&, const uint32_t*, const TileRec*, long long,
@@ -1585,14 +2025,14 @@ template void launch_pcg_start<float>(const Grid&, const uint32_t*, const TileRe
 
 void launch_resid_sq(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt,
                      const double* b, const double* q, double* partial, hipStream_t s) {
-  k_resid_sq<<<pointwise_blocks(nt), 256, 0, s>>>(g, flags, tiles, nt, b, q, partial);
+  launch(KResidSq{g, flags, tiles, nt, b, q, partial}, pointwise_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 
 void launch_finalize(int which, const double* partial, int nparts, PcgScalars* sc,
                      double* hist_relres, unsigned long long* hist_t, const int* done,
                      hipStream_t s) {
-  k_finalize<<<1, 256, 0, s>>>(which, partial, nparts, sc, hist_relres, hist_t, done);
+  launch(KFinalize{which, partial, nparts, sc, hist_relres, hist_t, done}, 1, s);
   GMGS_LAUNCH_CHECK();
 }
 

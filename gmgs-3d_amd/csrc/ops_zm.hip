@@ -23,6 +23,7 @@
 
 #include "ops.h"
 #include "ops_zm.h"
+#include "launch.h"
 #include "mesh.h"
 
 
@@ -48,7 +49,12 @@ __device__ inline void wave_lds_sync() {
 
 template <typename T>
 __device__ inline cptr<T> opaque(const T* p) {
-  cptr<T> q = (cptr<T>)p;
+  // (the pointer is uniform: read from the first lane so that it sits in
+  // scalar registers however it was loaded)
+  const unsigned long long u = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u),
+                 hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  cptr<T> q = (cptr<T>)(((unsigned long long)hi << 32) | lo);
   asm volatile("" : "+s"(q));
   return q;
 }
@@ -57,11 +63,32 @@ __device__ inline cptr<T> opaque(const T* p) {
 // also stored elements, 2 = F_FAST nodes only (no corrections; the other
 // solid nodes are left to the assembled-row kernel)
 template <typename T, typename TR, int MODE, int E, int TX>
-__global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTile* __restrict__ tiles,
-                                                      int zc, const TR* __restrict__ in, T omega,
-                                                      const T* __restrict__ in2, T* __restrict__ out,
-                                                      double* __restrict__ partial,
-                                                      const int* __restrict__ done, CycleOut<T> co) {
+struct KApplyZm {
+  static constexpr int kThreads = 16 * TX;
+  LevelOp<T> op;
+  const ColTile* tiles;
+  int zc;
+  const TR* in;
+  T omega;
+  const T* in2;
+  T* out;
+  double* partial;
+  const int* done;
+  CycleOut<T> co;
+  __device__ __forceinline__ void run(int block_, int nblocks_) const {
+    LevelOp<T> op = this->op;
+    const ColTile* __restrict__ tiles = this->tiles;
+    int zc = this->zc;
+    const TR* __restrict__ in = this->in;
+    T omega = this->omega;
+    const T* __restrict__ in2 = this->in2;
+    T* __restrict__ out = this->out;
+    double* __restrict__ partial = this->partial;
+    const int* __restrict__ done = this->done;
+    CycleOut<T> co = this->co;
+    (void)block_;
+    (void)nblocks_;
+
   constexpr int TY = 16, NT = TY * TX, SY = TY + 2, SX = TX + 2;
   constexpr int NSTAGE = 3 * SX * SY;
   constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
@@ -86,7 +113,7 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
   const int wv = (int)(threadIdx.x >> 6);
   // XCD-contiguous tile order: blocks b = 8 j + k (k = XCD) of the first
   // 8 floor(n / 8) blocks take tile k floor(n / 8) + j; the rest keep their own
-  const int nb = (int)gridDim.x, q8 = nb >> 3, bid = (int)blockIdx.x;
+  const int nb = (int)nblocks_, q8 = nb >> 3, bid = (int)block_;
   const int ti = bid < (q8 << 3) ? (bid & 7) * q8 + (bid >> 3) : bid;
   const ColTile t = tiles[ti];
   const int tid = threadIdx.x, ly = tid % TY, lx = tid / TY, lane = tid & 63;
@@ -455,11 +482,12 @@ __global__ __launch_bounds__(16 * TX) void k_apply_zm(LevelOp<T> op, const ColTi
         r += sh[0][w];
         if (TWO) r2 += sh[TWO ? 1 : 0][w];
       }
-      partial[blockIdx.x] = r;
-      if (TWO) partial[gridDim.x + blockIdx.x] = r2;
+      partial[block_] = r;
+      if (TWO) partial[nblocks_ + block_] = r2;
     }
   }
-}
+  }
+};
 
 __global__ void k_coltile_mark(Grid g, const uint32_t* __restrict__ flags, int nty, int ntx, int ntz,
                                int tx, int zc, int* __restrict__ mark) {
@@ -541,9 +569,8 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
     throw std::invalid_argument("SMOOTH_SUM / RESID_SUM: level 1 only, with its outputs");
   if (mode == AP_AX_P && (!co || e != 0 || !co->zf || !co->pn || !co->x || !co->sc || co->pn == (const void*)in))
     throw std::invalid_argument("AX_P: level 1 only, with z, x, the scalars and a separate new p");
-#define GMGS_ZM(MODE, E, TXV)                                                                        \
-  k_apply_zm<T, TR, MODE, E, TXV><<<(unsigned)ntiles, 16 * TXV, 0, s>>>(op, tiles, zc, in, omega, in2, \
-                                                                        out, partial, done, cov)
+#define GMGS_ZM(MODE, E, TXV) \
+  launch(KApplyZm<T, TR, MODE, E, TXV>{op, tiles, zc, in, omega, in2, out, partial, done, cov}, ntiles, s)
 #define GMGS_ZM_TX(MODE, E)               \
   do {                                    \
     if (tx == 4) GMGS_ZM(MODE, E, 4);     \
