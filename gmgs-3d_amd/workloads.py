@@ -185,8 +185,12 @@ def ellipsoids_c3(seed=0) -> Problem:
     return ellipsoid_problem(337, 294, 512, 6, seed=seed)
 
 
-def batch_c4(seed=0, count=64):
-    """64 uneven problems; problem i uses default_rng([seed, i])."""
+def batch_c4(seed=0, count=64, coarsest_control=1.0e3):
+    """64 uneven problems; problem i uses default_rng([seed, i]).  The level
+    count follows the level rule with coarsestResolutionControl_ = 1000 (as
+    the 40x40x64 box): the solver's coarsest solve applies an explicit dense
+    inverse, whose bytes grow with the square of the coarsest level's DOFs
+    (with 10^4 the 64 problems' inverses held 23.6 GB)."""
     out = []
     for i in range(count):
         rng = np.random.default_rng([seed, i])
@@ -194,7 +198,7 @@ def batch_c4(seed=0, count=64):
         n_ell = int(rng.integers(4, 9))
         n_loads = int(rng.integers(50, 201))
         vox = ellipsoid_union(nx, ny, nz, n_ell, rng)
-        L = level_rule(nx, ny, nz, 1.0e4)
+        L = level_rule(nx, ny, nz, coarsest_control)
         fixed, loads = slab_fixity_and_surface_loads(vox, L, n_loads, rng)
         bbox = np.array([[0.0, 0.0, 0.0], [nx, ny, nz]], dtype=np.float64)
         out.append(Problem(f"batch{seed}_{i}", vox, bbox, fixed, loads,
