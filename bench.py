@@ -362,17 +362,17 @@ def main():
                    **(cmp or {"comparisons": "skipped (--headline-only)"}),
                    "smoother_lmax": [round(s.smoother(li)[0], 5) for li in range(1, s.num_levels)],
                    "setup_s": round(setup_s, 3)},
-        "roofline": {"kernel": ("k_apply_zm<f64,AX_P> (level-1 direction update and matvec: "
+        "roofline": {"kernel": ("KApplyZm<double, double, 5, 0, TX> (level-1 direction update and matvec: "
                                 "p = z + beta p_old, q = A p, x += alpha p_old)"), "bound": "hbm",
                      "achieved": round(achieved_p, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_p / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "tile_tx_zc": [tx64, zc64], "alg_bytes_per_launch": alg_bytes_p,
                      "avg_launch_ms": round(mvp_ms, 5), "launches": args.matvec_reps},
-        "roofline_matvec64": {"kernel": "k_apply_zm<f64,AX> (level-1 matvec q = A p)",
+        "roofline_matvec64": {"kernel": "KApplyZm<double, double, 0, 0, TX> (level-1 matvec q = A p)",
                               "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
                               "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(mv_ms, 5)},
-        "roofline_jacobi64": {"kernel": "k_apply_zm<f64,SMOOTH> (level-1 Jacobi step)",
+        "roofline_jacobi64": {"kernel": "KApplyZm<double, double, 1, 0, TX> (level-1 Jacobi step)",
                               "achieved": round(jac_bytes / (jac_ms * 1e-3) / 1e9, 1),
                               "frac": round(jac_bytes / (jac_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                               "alg_bytes_per_launch": jac_bytes, "avg_launch_ms": round(jac_ms, 5)},
