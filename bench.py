@@ -361,6 +361,8 @@ def main():
                    "true_relres_mixed": info.true_relres,
                    **(cmp or {"comparisons": "skipped (--headline-only)"}),
                    "smoother_lmax": [round(s.smoother(li)[0], 5) for li in range(1, s.num_levels)],
+                   "level1_prolongation": ("formed by the residual kernel while staging"
+                                           if int(tcfg[0]) == 4 else "kernel of its own"),
                    "setup_s": round(setup_s, 3)},
         "roofline": {"kernel": ("KApplyZm<double, double, 5, 0, TX> (level-1 direction update and matvec: "
                                 "p = z + beta p_old, q = A p, x += alpha p_old)"), "bound": "hbm",

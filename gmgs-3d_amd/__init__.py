@@ -10,4 +10,5 @@ from .gmgs import (GMGS3D, GMGSError, NotConverged, Solver, default_params,  # n
                    GMGS_PREC_FP64, GMGS_PREC_MIXED, FLAG_EXACT_GALERKIN, FLAG_NO_INDEX_TABLES,
                    FLAG_FR_BETA, FLAG_PR_BETA, FLAG_SIMPLE_KERNELS,
                    FLAG_AUTOTUNE, FLAG_NO_GRAPHS, FLAG_SYM_VCYCLE, FLAG_NO_JACOBI_STEP,
-                   FLAG_CHEBYSHEV, FLAG_ASSEMBLED_COARSE, FLAG_SPLIT_COARSE)
+                   FLAG_CHEBYSHEV, FLAG_ASSEMBLED_COARSE, FLAG_SPLIT_COARSE,
+                   FLAG_UNFUSED_PROLONG)

@@ -68,6 +68,9 @@ struct Level {
   long long nrbox = 0;
   bool split_ok = false, split = false;
   int col_tx = 8, col_zc = 16;
+  // level 1 of the symmetric V(1, 1) cycle: the residual kernel forms
+  // e = P x_c while staging it (else a prolongation kernel stores e first)
+  bool fuse_prolong = true;
   // level 1 of a mixed context: the FP64 matvec's own column tiles (the
   // cycle's FP32 sweeps are tuned on coltiles)
   DevBuf coltiles64;
@@ -105,6 +108,7 @@ struct gmgs_ctx {
   bool exact = false;
   bool tables = true;
   bool simple_kernels = false;
+  bool unfused_prolong = false;   // V(1, 1) level 1: prolongation as its own kernel
   bool autotune = false;
   bool sym = false;              // symmetric V(nu, nu) cycle
   int sweeps = 2, sweeps_coarse = 2;   // nu of level 1, of the coarser smoothed levels
@@ -155,6 +159,13 @@ Level& L1(gmgs_ctx* c) { return c->lev[0]; }
 // the mixed solver without the Jacobi step folds p = z + beta p and the
 // deferred x update into its level-1 FP64 matvec when level 1 runs the
 // z-marching operator (p then alternates between p and p2)
+// the symmetric V(1, 1) cycle's level-1 residual kernel forms e = P x_c
+// while it stages it (vcycle_sym)
+bool v11_fused(const gmgs_ctx* c) {
+  const Level& l = c->lev[0];
+  return c->sym && c->L > 1 && l.nu == 1 && l.zm && l.fuse_prolong;
+}
+
 bool fused_direction(gmgs_ctx* c) { return c->mixed && c->no_jacobi && !c->p2.empty() && L1(c).zm; }
 
 hipEvent_t kt_event(gmgs_ctx* c) {
@@ -568,6 +579,7 @@ void build_operators(gmgs_ctx* c) {
     }
     // z-marching operator where a universal element matrix exists
     l.zm = !c->simple_kernels && l.classes_ok && (li == 0 ? !c->simp : l.state == 2);
+    l.fuse_prolong = !c->unfused_prolong;
     if (l.zm && l.coltiles.empty()) {
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
       build_colrange(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.colz, s);
@@ -795,6 +807,32 @@ long long partial_capacity(gmgs_ctx* c) {
 // so repeated set-ups of one problem pick the same shapes
 inline bool clearly_faster(float ms, float best_ms) { return ms < 0.98f * best_ms; }
 
+// the level-1 prolongation e = P x_c of the mixed cycle, ms per launch (the
+// better of two batches of 5 after 2 untimed launches)
+float time_prolong(gmgs_ctx* c) {
+  Level& l = c->lev[0];
+  Level& cr = c->lev[1];
+  hipStream_t s = c->stream;
+  auto go = [&]() {
+    launch_prolong<float>(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, cr.mesh.g,
+                          cr.x.as<float>(), l.e.as<float>(), true, nullptr, s);
+  };
+  HipEvent a, b;
+  for (int w = 0; w < 2; ++w) go();
+  const int reps = 5;
+  float best = 1e30f;
+  for (int batch = 0; batch < 2; ++batch) {
+    GMGS_HIP(hipEventRecord(a, s));
+    for (int w = 0; w < reps; ++w) go();
+    GMGS_HIP(hipEventRecord(b, s));
+    GMGS_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    GMGS_HIP(hipEventElapsedTime(&ms, a, b));
+    best = std::min(best, ms / reps);
+  }
+  return best;
+}
+
 void autotune_levels(gmgs_ctx* c) {
   static const int cand[][2] = {{8, 16}, {4, 16}, {16, 16}, {8, 8}, {4, 8}, {16, 8}, {8, 4},
                                 {4, 4}, {16, 4}, {8, 32}, {4, 32}};
@@ -814,10 +852,24 @@ void autotune_levels(gmgs_ctx* c) {
       auto it = tune_cache().find(key);
       if (it != tune_cache().end()) { best = it->second; cached = true; }
     }
+    // level 1 of the mixed V(1, 1) cycle: whether its residual kernel forms
+    // e = P x_c itself, cached under its own key
+    const bool v11 = li == 0 && c->mixed && c->sym && l.nu == 1 && !l.e.empty() && c->L > 1 &&
+                     !c->lev[1].x.empty();
+    const TuneKey fkey{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, c->unfused_prolong ? 5 : 4};
+    bool best_fuse = l.fuse_prolong;
+    if (cached && v11) {
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      auto it = tune_cache().find(fkey);
+      if (it != tune_cache().end()) best_fuse = it->second.first != 0;
+      else cached = false;
+    }
     if (!cached) {
       float best_ms = 1e30f;
+      const float prolong_ms = v11 ? time_prolong(c) : 0.f;
       if (l.zm) {
         for (const auto& cd : cand) {
+          bool fuse = false;
           DevBuf tl;
           const long long nt = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
           float ms;
@@ -826,13 +878,24 @@ void autotune_levels(gmgs_ctx* c) {
               ms = time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
                                           l.rp.as<float>());
               // the symmetric V(1, 1) cycle runs its residual-and-sum kernel on
-              // the same tiles: time both (scratch: e, R_1 = rp, r0 = tmp, z = z[1])
-              if (c->sym && l.nu == 1 && !l.e.empty()) {
+              // the same tiles: time it too (scratch: e or level 2's x, R_1 =
+              // rp, r0 = tmp, z = z[1]), stand-alone after the prolongation
+              // and forming e itself, and keep the faster of the two
+              if (v11) {
                 CycleOut<float> co{};
                 co.r0 = c->tmp.as<double>();
                 co.z = c->z[1].as<float>();
-                ms += time_zm<float, float>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], l.e.as<float>(),
-                                            nullptr, AP_RESID_SUM, &co, l.rp.as<float>());
+                co.gc = c->lev[1].mesh.g;
+                const float unf = time_zm<float, float>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1],
+                                                        l.e.as<float>(), nullptr, AP_RESID_SUM, &co,
+                                                        l.rp.as<float>()) + prolong_ms;
+                const float fus = c->unfused_prolong
+                                      ? 1e30f
+                                      : time_zm<float, float>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1],
+                                                              c->lev[1].x.as<float>(), nullptr, AP_RESID_SUM_P,
+                                                              &co, l.rp.as<float>());
+                fuse = fus <= unf;
+                ms += fuse ? fus : unf;
               }
             }
             else ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(), l.rp.as<double>());
@@ -840,7 +903,11 @@ void autotune_levels(gmgs_ctx* c) {
             if (c->mixed) ms = time_zm<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(), l.rp.as<float>());
             else ms = time_zm<double, double>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<double>(), l.rp.as<double>());
           }
-          if (clearly_faster(ms, best_ms)) { best_ms = ms; best = {cd[0], cd[1]}; }
+          if (clearly_faster(ms, best_ms)) {
+            best_ms = ms;
+            best = {cd[0], cd[1]};
+            best_fuse = fuse;
+          }
           if (li > 0 && l.split_ok) {
             ms = c->mixed ? time_split<float, float>(c, li, tl.as<ColTile>(), nt, cd[0], cd[1], l.r.as<float>(),
                                                      l.rp.as<float>())
@@ -861,7 +928,9 @@ void autotune_levels(gmgs_ctx* c) {
       }
       std::lock_guard<std::mutex> lk(tune_mutex());
       tune_cache()[key] = best;
+      if (v11) tune_cache()[fkey] = {best_fuse ? 1 : 0, 0};
     }
+    if (v11) l.fuse_prolong = best_fuse && !c->unfused_prolong;
     if (best.first == 0) {
       l.zm = l.split = false;
       l.as = best.second == 1 && l.as_built;
@@ -1084,16 +1153,21 @@ int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* par
     const uint32_t* fl = l.mesh.flags.as<uint32_t>();
     const TileRec* tl = l.tiles.as<TileRec>();
     const T* invd = make_op<T>(c, li, true).invd;
-    launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
     if (li == 0 && nu == 1 && l.zm) {
       // V(1, 1): the residual kernel writes the cycle's sum itself (no Q_0,
-      // no separate sum)
-      const CycleOut<T> co{rin, nullptr, nullptr, zprev, z, (T)0};
+      // no separate sum) and, unless asked not to, forms e = P x_c itself
+      // while it stages it (e is then never stored)
+      CycleOut<T> co{rin, nullptr, nullptr, zprev, z, (T)0};
+      co.gc = cr.mesh.g;
+      if (!l.fuse_prolong)
+        launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
       nz = launch_apply_zm<T, T>(make_op<T>(c, 0, true), l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc,
-                                 AP_RESID_SUM, vec<T>(l.e), (T)l.om[0], Rbuf<T>(l, 1), nullptr, partial, done,
-                                 s, &co);
+                                 l.fuse_prolong ? AP_RESID_SUM_P : AP_RESID_SUM,
+                                 l.fuse_prolong ? vec<T>(cr.x) : vec<T>(l.e), (T)l.om[0], Rbuf<T>(l, 1),
+                                 nullptr, partial, done, s, &co);
       continue;
     }
+    launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
     apply_level<T, T>(c, li, AP_RESID, vec<T>(l.e), (T)0, Qbuf<T>(l, 0), nullptr, nullptr, done, Rbuf<T>(l, nu),
                       true);
     if (li == 0 && nu == 2 && l.zm) {
@@ -1803,6 +1877,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->exact = (p->flags & GMGS_FLAG_EXACT_GALERKIN) != 0;
     c->tables = (p->flags & GMGS_FLAG_NO_INDEX_TABLES) == 0;
     c->simple_kernels = (p->flags & GMGS_FLAG_SIMPLE_KERNELS) != 0;
+    c->unfused_prolong = (p->flags & GMGS_FLAG_UNFUSED_PROLONG) != 0;
     c->autotune = (p->flags & GMGS_FLAG_AUTOTUNE) != 0;
     c->use_graphs = (p->flags & GMGS_FLAG_NO_GRAPHS) == 0;
     c->sym = (p->flags & GMGS_FLAG_SYM_VCYCLE) != 0;
@@ -2492,7 +2567,10 @@ int gmgs_query(gmgs_ctx* c, int key, int level, void* out, int64_t count) {
       case GMGS_Q_NUM_FREE_COARSEST: i64[0] = c->nfree; break;
       case GMGS_Q_MAX_IT: i64[0] = c->prm.max_it; break;
       case GMGS_Q_TILE_CONFIG:
-        i64[0] = l->split ? 3 : (l->zm ? 1 : (l->as ? 2 : 0)); i64[1] = l->col_tx; i64[2] = l->col_zc;
+        // (4: level 1 z-marching, its V(1, 1) residual kernel forming e = P x_c)
+        i64[0] = l->split ? 3 : (l->zm ? (level == 1 && v11_fused(c) ? 4 : 1) : (l->as ? 2 : 0));
+        i64[1] = l->col_tx;
+        i64[2] = l->col_zc;
         // the FP64 matvec of the PCG runs on column tiles of its own when tuned
         i64[3] = l->coltiles64.empty() ? l->col_tx : l->col_tx64;
         i64[4] = l->coltiles64.empty() ? l->col_zc : l->col_zc64;

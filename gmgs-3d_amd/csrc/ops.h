@@ -56,8 +56,13 @@ struct PcgScalars {
 // z = omega (r0 + Q_0) / d + e instead of Q_0 (CycleOut without R1, e, w1).
 // AX_P: the PCG's direction update fused into its matvec (level 1, FP64, FP32
 // z): p = z + beta p_old from in = p_old, out = A p, p written to CycleOut::pn
-// (a buffer other than in), x += alpha p_old, and the p.q partials
-enum ApplyMode { AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3, AP_RESID_SUM = 4, AP_AX_P = 5 };
+// (a buffer other than in), x += alpha p_old, and the p.q partials.
+// RESID_SUM_P: RESID_SUM with in = x_c, the coarse level's correction on the
+// grid CycleOut::gc: every staged value of e = P x_c (fixed components 0) is
+// formed from the coarse values while it is staged, so e is never stored
+enum ApplyMode {
+  AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3, AP_RESID_SUM = 4, AP_AX_P = 5, AP_RESID_SUM_P = 6
+};
 
 // outputs of AP_SMOOTH_SUM: z = (omega (r0 + Q_0) + w1 (R1 + Q_1)) / d + e,
 // fixed components 0, and partials of r0.z and r0.zprev (zprev may be null)
@@ -74,6 +79,8 @@ struct CycleOut {
   double* pn;
   double* x;
   const PcgScalars* sc;
+  // RESID_SUM_P: the coarse grid of in = x_c
+  Grid gc;
 };
 
 // ---- launches (all on stream s; 'done' may be null) ----

@@ -92,12 +92,18 @@ struct KApplyZm {
   constexpr int TY = 16, NT = TY * TX, SY = TY + 2, SX = TX + 2;
   constexpr int NSTAGE = 3 * SX * SY;
   constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
-  constexpr bool SUM = MODE == AP_SMOOTH_SUM, RSUM = MODE == AP_RESID_SUM;
+  constexpr bool PRO = MODE == AP_RESID_SUM_P;       // e = P x_c formed while staged
+  constexpr bool SUM = MODE == AP_SMOOTH_SUM, RSUM = MODE == AP_RESID_SUM || PRO;
   constexpr bool SM = MODE == AP_SMOOTH || SUM;
   constexpr bool AXP = MODE == AP_AX_P;              // p = z + beta p_old staged, x updated
   constexpr bool TWO = SUM || RSUM;                 // two dot partials (r0.z, r0.zprev)
   constexpr int ND = (SM || AXP) ? PER : 1;          // second staged operand: 1/d or z
   using TD = typename std::conditional<AXP, float, T>::type;
+  // PRO: the coarse patch a staged plane draws on, planes (z - 1) >> 1 and
+  // z >> 1 of x_c (CY x CX nodes, three components each), and its share per
+  // thread; the staged positions' flags replace the input's loads
+  constexpr int CY = TY / 2 + 2, CX = TX / 2 + 2, NC = 2 * 3 * CY * CX;
+  constexpr int PC = PRO ? (NC + NT - 1) / NT : 1, PF = PRO ? PER : 1;
   constexpr bool HAS_EOP = E == 1, FAST_ONLY = E == 2;
   // output nodes of this kernel
   constexpr uint32_t OUT_MASK = FAST_ONLY ? F_FAST : F_ELEM_MASK;
@@ -107,6 +113,7 @@ struct KApplyZm {
   // output completion, element corrections)
   __shared__ T sU[4][3][SX][SY];
   __shared__ T sR[(SM || AXP) ? 2 : 1][3][TX][TY];   // SMOOTH: centre values of r; AX_P: of p_old
+  __shared__ T sC[PRO ? NC : 1];                     // PRO: the plane's coarse patch
   // per wave: the element corrections' task list and results
   __shared__ int sTask[FAST_ONLY ? 1 : NT / 64][64];
   __shared__ T sRes[FAST_ONLY ? 1 : NT / 64][3][64];
@@ -153,24 +160,58 @@ struct KApplyZm {
     const int c = e / (SX * SY);
     const int lo = zr[i] & 0xffff, hi = zr[i] >> 16;
     const bool solid = hi >= lo;
-    soff[i] = solid ? scol[i] + c * (int)N : 0;
+    soff[i] = solid ? scol[i] + (PRO ? 0 : c * (int)N) : 0;
     zr[i] = solid ? lo | ((hi - lo) << 16) : 0xffff;
   }
+
+  // PRO: per staged position its coarse-patch offset at (ya, xa), its
+  // component and the parities of its y and x (bits 0-11, 14-15, 12, 13);
+  // the patch origin is (cy0, cx0) in coarse node coordinates (y0, x0 even)
+  const int cy0 = (t.y0 >> 1) - 1, cx0 = (t.x0 >> 1) - 1;
+  int ppos[PF];
+#pragma unroll
+  for (int i = 0; i < PF; ++i) {
+    const int e = tid + i * NT;
+    const int c = e / (SX * SY), rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
+    const int yf = t.y0 - 2 + yy, xf = t.x0 - 2 + xx;   // fine node coordinates
+    const int base = ((yf >> 1) - cy0) + CY * (((xf >> 1) - cx0) + CX * c);
+    ppos[i] = e < NSTAGE ? base | ((yf & 1) << 12) | ((xf & 1) << 13) | (c << 14) : 0;
+  }
+  static_assert(3 * CY * CX <= 4096, "patch offset in 12 bits");
 
   // raw loads only (no arithmetic on them until they are staged, two planes
   // later): the input and, for the smoother, the inverse diagonal (AX_P: z).  Index 0 of
   // every padded array is a corner halo entry and holds 0, so a skipped
   // position loads from there: the loads need no branches.
   // The index is formed in 32 bits (3 N < 2^31) and selected, not branched on.
-  auto fetch = [&](int zz, TR* vr, TD* vd) {
+  auto fetch = [&](int zz, TR* vr, TD* vd, uint32_t* vf, TR* vc) {
     const int szz = (int)sz * zz;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const bool ok = (unsigned)(zz - (zr[i] & 0xffff)) <= (unsigned)(zr[i] >> 16);
       const unsigned m = ok ? (unsigned)(soff[i] + szz) : 0u;
-      vr[i] = in[m];
-      if (SM) vd[i] = (TD)op.invd[m];
-      if (AXP) vd[i] = (TD)co.zf[m];
+      if constexpr (PRO) {
+        vf[i] = op.flags[m];
+      } else {
+        vr[i] = in[m];
+        if (SM) vd[i] = (TD)op.invd[m];
+        if (AXP) vd[i] = (TD)co.zf[m];
+      }
+    }
+    if constexpr (PRO) {
+      // coarse planes (zz - 2) >> 1 and (zz - 1) >> 1 (node coordinates) of
+      // the fine node plane zz - 1; outside the padded coarse grid: index 0
+      const int zf = zz - 1;
+#pragma unroll
+      for (int j = 0; j < PC; ++j) {
+        const int k = tid + j * NT;
+        const int yi = k % CY, xi = (k / CY) % CX, c = (k / (CY * CX)) % 3, zi = k / (3 * CY * CX);
+        const int gy = cy0 + yi + 1, gx = cx0 + xi + 1, gz = (zi ? (zf + 1) >> 1 : zf >> 1) + 1;
+        const bool okc = k < NC && gy >= 0 && gy < co.gc.py && gx >= 0 && gx < co.gc.px && gz >= 0 &&
+                         gz < co.gc.pz;
+        const unsigned m = okc ? (unsigned)(gy + co.gc.py * (gx + co.gc.px * gz) + c * (int)co.gc.N) : 0u;
+        vc[j] = in[m];
+      }
     }
   };
 
@@ -180,15 +221,17 @@ struct KApplyZm {
   int kb = 0;                                       // buffer of the current plane
   // two planes of loads in flight: plane zz is staged from one register set
   // while the other holds plane zz + 1; z0 < z1 for every tile
-  TR rA[PER], rB[PER];
+  TR rA[PRO ? 1 : PER], rB[PRO ? 1 : PER];
   TD dA[ND], dB[ND];
+  uint32_t gA[PF], gB[PF];                          // PRO: flags of the staged positions
+  TR cA[PC], cB[PC];                                // PRO: the coarse patch
   T beta = 0, alpha = 0;
   if (AXP) {
     beta = (T)co.sc->beta;
     alpha = (T)co.sc->alpha;
   }
-  fetch(z0 - 1, rA, dA);
-  fetch(z0, rB, dB);
+  fetch(z0 - 1, rA, dA, gA, cA);
+  fetch(z0, rB, dB, gB, cB);
   // flags of the outputs this lane owns, loaded one plane before they are used
   auto flags_at = [&](const int z) -> uint32_t {
     const bool ok = lane_ok && z >= z0 && z < z1 && z >= own_lo && z <= own_hi;
@@ -196,16 +239,56 @@ struct KApplyZm {
   };
   uint32_t fN = flags_at(z0);
 
-  auto plane = [&](const int zz, TR (&sr)[PER], TD (&sd)[ND]) {
+  auto plane = [&](const int zz, TR (&sr)[PRO ? 1 : PER], TD (&sd)[ND], uint32_t (&sf)[PF], TR (&sc)[PC]) {
     T (*const sB)[SX][SY] = sU[kb];
+    if constexpr (PRO) {
+      // e = P x_c at the staged positions (kernel/InterpolationDeviation.m:3-17):
+      // the weighted sum over the eight coarse corners in the order of the
+      // stand-alone prolongation, 0 at void nodes and fixed components.  sC was
+      // last read before the previous plane's barrier.
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * NT;
-      if (e < NSTAGE) {
-        T v = (T)sr[i];
-        if (SM) v = omega * (T)sd[i] * v;
-        if (AXP) v = (T)sd[i] + beta * v;              // p = z + beta p_old (CG_solver.m:28)
-        (&sB[0][0][0])[e] = v;
+      for (int j = 0; j < PC; ++j) {
+        const int k = tid + j * NT;
+        if (k < NC) sC[k] = (T)sc[j];
+      }
+      __syncthreads();
+      // (the position words pass through an empty asm each plane, so that
+      // what is derived from them is not kept in registers across planes)
+      const int zo = (zz - 1) & 1;
+      const T wz0 = zo ? (T)0.5 : (T)1, wz1 = zo ? (T)0.5 : (T)0;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * NT;
+        if (e < NSTAGE) {
+          int w = ppos[i];
+          asm volatile("" : "+v"(w));
+          const int c = (w >> 14) & 3, yo = (w >> 12) & 1, xo = (w >> 13) & 1, base = w & 0xfff;
+          const uint32_t f = sf[i];
+          T v = 0;
+          if ((f & F_ELEM_MASK) && !(f & (1u << (F_FIX_SHIFT + c)))) {
+            const T wy0 = yo ? (T)0.5 : (T)1, wy1 = yo ? (T)0.5 : (T)0;
+            const T wx0 = xo ? (T)0.5 : (T)1, wx1 = xo ? (T)0.5 : (T)0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
+              const T wt = ((ky ? wy1 : wy0) * (kx ? wx1 : wx0)) * (kz ? wz1 : wz0);
+              const int o = base + (ky & yo) + CY * (kx & xo) + 3 * CY * CX * (kz & zo);
+              v += wt * sC[o];
+            }
+          }
+          (&sB[0][0][0])[e] = v;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * NT;
+        if (e < NSTAGE) {
+          T v = (T)sr[i];
+          if (SM) v = omega * (T)sd[i] * v;
+          if (AXP) v = (T)sd[i] + beta * v;              // p = z + beta p_old (CG_solver.m:28)
+          (&sB[0][0][0])[e] = v;
+        }
       }
     }
     // one barrier per plane: buffer kb was last read while computing planes
@@ -333,7 +416,7 @@ struct KApplyZm {
       aM[1] -= D1;
       aM[2] -= D2;
     }
-    if (zz + 2 <= z1) fetch(zz + 2, sr, sd);
+    if (zz + 2 <= z1) fetch(zz + 2, sr, sd, sf, sc);
     // flags of the output at zz+1 (only outputs in [z0, z1) are owned)
     fP = fN;
     fN = flags_at(zz + 2);
@@ -459,8 +542,8 @@ struct KApplyZm {
     kb = (kb + 1) & 3;
   };
   for (int zz = z0 - 1; zz <= z1; zz += 2) {
-    plane(zz, rA, dA);
-    if (zz + 1 <= z1) plane(zz + 1, rB, dB);
+    plane(zz, rA, dA, gA, cA);
+    if (zz + 1 <= z1) plane(zz + 1, rB, dB, gB, cB);
   }
   if (partial) {
     // block reduction (fixed order)
@@ -565,8 +648,11 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   if (3 * op.g.N >= (1LL << 31)) throw std::invalid_argument("grid too large for the z-marching operator");
   const int e = fast_only ? 2 : (op.eop != nullptr ? 1 : 0);
   const CycleOut<T> cov = co ? *co : CycleOut<T>{};
-  if ((mode == AP_SMOOTH_SUM || mode == AP_RESID_SUM) && (!co || e != 0 || !op.invd))
+  if ((mode == AP_SMOOTH_SUM || mode == AP_RESID_SUM || mode == AP_RESID_SUM_P) && (!co || e != 0 || !op.invd))
     throw std::invalid_argument("SMOOTH_SUM / RESID_SUM: level 1 only, with its outputs");
+  if (mode == AP_RESID_SUM_P &&
+      (co->gc.ey * 2 != op.g.ey || co->gc.ex * 2 != op.g.ex || co->gc.ez * 2 != op.g.ez || 3 * co->gc.N >= (1LL << 31)))
+    throw std::invalid_argument("RESID_SUM_P: the coarse grid is not this level's next coarser grid");
   if (mode == AP_AX_P && (!co || e != 0 || !co->zf || !co->pn || !co->x || !co->sc || co->pn == (const void*)in))
     throw std::invalid_argument("AX_P: level 1 only, with z, x, the scalars and a separate new p");
 #define GMGS_ZM(MODE, E, TXV) \
@@ -593,6 +679,9 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   } else if (mode == AP_RESID_SUM) {
     if constexpr (std::is_same<T, TR>::value) GMGS_ZM_TX(AP_RESID_SUM, 0);
     else throw std::invalid_argument("RESID_SUM: input in the cycle precision");
+  } else if (mode == AP_RESID_SUM_P) {
+    if constexpr (std::is_same<T, TR>::value) GMGS_ZM_TX(AP_RESID_SUM_P, 0);
+    else throw std::invalid_argument("RESID_SUM_P: input in the cycle precision");
   } else if (mode == AP_AX_P) {
     if constexpr (std::is_same<T, double>::value && std::is_same<TR, double>::value) GMGS_ZM_TX(AP_AX_P, 0);
     else throw std::invalid_argument("AX_P: FP64 operator and input");

@@ -76,11 +76,16 @@ enum {
   GMGS_FLAG_ASSEMBLED_COARSE = 1024, /* coarse smoothed levels apply an assembled 27-point
                                      stencil (243 coefficients per node, set-up time) wherever it
                                      fits, instead of the z-marching or element-sum kernels */
-  GMGS_FLAG_SPLIT_COARSE = 2048   /* coarse smoothed levels with a universal element: the
+  GMGS_FLAG_SPLIT_COARSE = 2048,  /* coarse smoothed levels with a universal element: the
                                      z-marching kernel on the nodes whose eight elements are all
                                      present and universal, the assembled rows on the others (one
                                      lane per row); elsewhere as GMGS_FLAG_ASSEMBLED_COARSE.
                                      GMGS_FLAG_AUTOTUNE also times this operator per level */
+  GMGS_FLAG_UNFUSED_PROLONG = 4096 /* symmetric V(1, 1) cycle on a z-marching level 1: store
+                                     e = P x_c with a prolongation kernel of its own before the
+                                     residual kernel reads it, instead of forming each value of e
+                                     while the residual kernel stages it (same results; for
+                                     cross-checks and timing) */
 };
 
 typedef struct gmgs_params {
@@ -145,7 +150,9 @@ enum {
   GMGS_Q_NUM_FREE_COARSEST = 27,    /* int64 */
   GMGS_Q_TILE_CONFIG = 28,          /* int64[5]: level kernel (0 tile kernel with element sums,
                                        1 z-marching, 2 tile kernel with the assembled stencil,
-                                       3 split: z-marching on F_FAST nodes + assembled rows),
+                                       3 split: z-marching on F_FAST nodes + assembled rows,
+                                       4 level 1 z-marching whose symmetric V(1, 1) residual
+                                       kernel forms e = P x_c while staging it),
                                        z-marching tile TX, z-chunk, then TX and z-chunk of the
                                        FP64 matvec of the PCG (level 1 of a mixed context tunes
                                        its own; the level's otherwise) */

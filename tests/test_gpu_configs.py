@@ -95,6 +95,26 @@ def test_c2_mixed_tight_tolerance_matches_oracle(pkg, c2, c2_gold):
     s.close()
 
 
+def test_c2_fused_prolongation_bitwise(pkg, c2):
+    """The V(1, 1) level-1 residual kernel that forms e = P x_c while staging
+    it gives the same solve, bit for bit, as the stand-alone prolongation
+    followed by the residual kernel (same column tiles: no autotuning)."""
+    out = []
+    for extra in (0, pkg.FLAG_UNFUSED_PROLONG):
+        s = pkg.Solver(c2.vox, c2.bbox, num_levels=c2.num_levels, jacobi_weight=c2.omega, tol=1e-8,
+                       max_it=3000, precision=pkg.GMGS_PREC_MIXED, flags=headline_flags(pkg, False) | extra,
+                       **HEADLINE)
+        s.set_bc(c2.fixed, c2.loads)
+        U, info, h = s.solve()
+        s.close()
+        out.append((U, info, h))
+        log(f"  {'unfused' if extra else 'fused'}: its {info.its}, true {info.true_relres:.2e}")
+    (Uf, jf, hf), (Uu, ju, hu) = out
+    assert jf.converged and jf.its == ju.its
+    assert np.array_equal(hf[:, 2], hu[:, 2])
+    assert np.array_equal(Uf, Uu)
+
+
 def test_c3_mixed_vs_fp64_reference_tight(pkg):
     t0 = time.time()
     p = pkg.workloads.ellipsoids_c3(0)
