@@ -83,6 +83,25 @@ struct CycleOut {
   Grid gc;
 };
 
+// The prolongation's arithmetic at one fine node, shared by the stand-alone
+// prolongation and the residual kernel that forms it while staging, so both
+// give the same bits: the two coarse planes are combined first (weights wz0,
+// wz1), then the two coarse columns in x, then the two rows in y.  A weight
+// is 1 or 0.5 on an odd fine coordinate's two neighbours (0.5 each) and 1, 0
+// on an even one's single neighbour, so the unused corner adds an exact 0.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <typename T>
+__device__ __forceinline__ T prolong_zplanes(T wz0, T wz1, T v0, T v1) {
+  return fma_t(wz1, v1, wz0 * v0);
+}
+// z-combined corner values zab (a: y offset, b: x offset)
+template <typename T>
+__device__ __forceinline__ T prolong_xy(T wy0, T wy1, T wx0, T wx1, T z00, T z01, T z10, T z11) {
+  const T r0 = fma_t(wx1, z01, wx0 * z00), r1 = fma_t(wx1, z11, wx0 * z10);
+  return fma_t(wy1, r1, wy0 * r0);
+}
+
 // ---- launches (all on stream s; 'done' may be null) ----
 // assembled stencil rows of a coarse level (FP64): coefficient (o, c, d) of
 // row k at asc[(9 o + 3 c + d) * stride + k], o = neighbour (dy + 1) +

@@ -815,7 +815,8 @@ struct KProlongSmooth {
 };
 
 // e = P xc on the fine level's solid nodes (the prolongation of
-// kernel/InterpolationDeviation.m:3-17 on the padded grids)
+// kernel/InterpolationDeviation.m:3-17 on the padded grids), in the order of
+// prolong_zplanes / prolong_xy
 template <typename T>
 struct KProlong {
   static constexpr int kThreads = 256;
@@ -847,23 +848,22 @@ struct KProlong {
     const int yf = ln_.yp - 1, xf = ln_.xp - 1, zf = ln_.zp - 1;
     const int ya = yf >> 1, yb = (yf + 1) >> 1, xa = xf >> 1, xb = (xf + 1) >> 1, za = zf >> 1,
               zb = (zf + 1) >> 1;
-    const T wy = (yf & 1) ? (T)0.5 : (T)1, wx = (xf & 1) ? (T)0.5 : (T)1,
-            wz = (zf & 1) ? (T)0.5 : (T)1;
+    const T wy0 = (yf & 1) ? (T)0.5 : (T)1, wy1 = (yf & 1) ? (T)0.5 : (T)0;
+    const T wx0 = (xf & 1) ? (T)0.5 : (T)1, wx1 = (xf & 1) ? (T)0.5 : (T)0;
+    const T wz0 = (zf & 1) ? (T)0.5 : (T)1, wz1 = (zf & 1) ? (T)0.5 : (T)0;
     const long long Nc = gc.N, N = gf.N, n = ln_.n;
-    T p[3] = {0, 0, 0};
-    const int ys[2] = {ya, yb}, xs[2] = {xa, xb}, zs[2] = {za, zb};
+    const long long m00 = gc.pidx(ya, xa, za), m01 = gc.pidx(ya, xb, za), m10 = gc.pidx(yb, xa, za),
+                    m11 = gc.pidx(yb, xb, za), dz = gc.pidx(ya, xa, zb) - m00;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int cy = ys[k & 1], cx = xs[(k >> 1) & 1], cz = zs[k >> 2];
-      const T w = (((k & 1) && !(yf & 1)) ? (T)0 : wy) * ((((k >> 1) & 1) && !(xf & 1)) ? (T)0 : wx)
-                  * (((k >> 2) && !(zf & 1)) ? (T)0 : wz);
-      const long long m = gc.pidx(cy, cx, cz);
-      p[0] += w * xc[m];
-      p[1] += w * xc[Nc + m];
-      p[2] += w * xc[2 * Nc + m];
+    for (int c = 0; c < 3; ++c) {
+      const T* __restrict__ v = xc + c * Nc;
+      const T z00 = prolong_zplanes(wz0, wz1, v[m00], v[m00 + dz]);
+      const T z01 = prolong_zplanes(wz0, wz1, v[m01], v[m01 + dz]);
+      const T z10 = prolong_zplanes(wz0, wz1, v[m10], v[m10 + dz]);
+      const T z11 = prolong_zplanes(wz0, wz1, v[m11], v[m11 + dz]);
+      const T p = prolong_xy(wy0, wy1, wx0, wx1, z00, z01, z10, z11);
+      e[c * N + n] = (mask && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) ? (T)0 : p;
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) e[c * N + n] = (mask && (ln_.fl & (1u << (F_FIX_SHIFT + c)))) ? (T)0 : p[c];
   }
   }
 };

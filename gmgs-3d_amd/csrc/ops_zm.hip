@@ -99,11 +99,12 @@ struct KApplyZm {
   constexpr bool TWO = SUM || RSUM;                 // two dot partials (r0.z, r0.zprev)
   constexpr int ND = (SM || AXP) ? PER : 1;          // second staged operand: 1/d or z
   using TD = typename std::conditional<AXP, float, T>::type;
-  // PRO: the coarse patch a staged plane draws on, planes (z - 1) >> 1 and
-  // z >> 1 of x_c (CY x CX nodes, three components each), and its share per
-  // thread; the staged positions' flags replace the input's loads
-  constexpr int CY = TY / 2 + 2, CX = TX / 2 + 2, NC = 2 * 3 * CY * CX;
-  constexpr int PC = PRO ? (NC + NT - 1) / NT : 1, PF = PRO ? PER : 1;
+  // PRO: the coarse patch a staged plane draws on (CY x CX coarse nodes,
+  // three components each) in planes (z - 1) >> 1 and z >> 1 of x_c; a thread
+  // loads both planes of its share of the patch, PQ positions; the staged
+  // positions' flags replace the input's loads
+  constexpr int CY = TY / 2 + 2, CX = TX / 2 + 2, NQ = 3 * CY * CX;
+  constexpr int PQ = PRO ? (NQ + NT - 1) / NT : 1, PC = 2 * PQ, PF = PRO ? PER : 1;
   constexpr bool HAS_EOP = E == 1, FAST_ONLY = E == 2;
   // output nodes of this kernel
   constexpr uint32_t OUT_MASK = FAST_ONLY ? F_FAST : F_ELEM_MASK;
@@ -113,7 +114,7 @@ struct KApplyZm {
   // output completion, element corrections)
   __shared__ T sU[4][3][SX][SY];
   __shared__ T sR[(SM || AXP) ? 2 : 1][3][TX][TY];   // SMOOTH: centre values of r; AX_P: of p_old
-  __shared__ T sC[PRO ? NC : 1];                     // PRO: the plane's coarse patch
+  __shared__ T sC[PRO ? NQ : 1];                     // PRO: the patch, its two planes combined
   // per wave: the element corrections' task list and results
   __shared__ int sTask[FAST_ONLY ? 1 : NT / 64][64];
   __shared__ T sRes[FAST_ONLY ? 1 : NT / 64][3][64];
@@ -177,7 +178,7 @@ struct KApplyZm {
     const int base = ((yf >> 1) - cy0) + CY * (((xf >> 1) - cx0) + CX * c);
     ppos[i] = e < NSTAGE ? base | ((yf & 1) << 12) | ((xf & 1) << 13) | (c << 14) : 0;
   }
-  static_assert(3 * CY * CX <= 4096, "patch offset in 12 bits");
+  static_assert(NQ <= 4096, "patch offset in 12 bits");
 
   // raw loads only (no arithmetic on them until they are staged, two planes
   // later): the input and, for the smoother, the inverse diagonal (AX_P: z).  Index 0 of
@@ -203,14 +204,15 @@ struct KApplyZm {
       // the fine node plane zz - 1; outside the padded coarse grid: index 0
       const int zf = zz - 1;
 #pragma unroll
-      for (int j = 0; j < PC; ++j) {
-        const int k = tid + j * NT;
-        const int yi = k % CY, xi = (k / CY) % CX, c = (k / (CY * CX)) % 3, zi = k / (3 * CY * CX);
-        const int gy = cy0 + yi + 1, gx = cx0 + xi + 1, gz = (zi ? (zf + 1) >> 1 : zf >> 1) + 1;
-        const bool okc = k < NC && gy >= 0 && gy < co.gc.py && gx >= 0 && gx < co.gc.px && gz >= 0 &&
-                         gz < co.gc.pz;
-        const unsigned m = okc ? (unsigned)(gy + co.gc.py * (gx + co.gc.px * gz) + c * (int)co.gc.N) : 0u;
-        vc[j] = in[m];
+      for (int j = 0; j < PQ; ++j) {
+        const int q = tid + j * NT;
+        const int yi = q % CY, xi = (q / CY) % CX, c = q / (CY * CX);
+        const int gy = cy0 + yi + 1, gx = cx0 + xi + 1, ga = (zf >> 1) + 1, gb = ((zf + 1) >> 1) + 1;
+        const bool okc = q < NQ && gy >= 0 && gy < co.gc.py && gx >= 0 && gx < co.gc.px;
+        const int col = gy + co.gc.py * gx + c * (int)co.gc.N, sp = co.gc.py * co.gc.px;
+        const bool oka = okc && ga >= 0 && ga < co.gc.pz, okb = okc && gb >= 0 && gb < co.gc.pz;
+        vc[2 * j] = in[oka ? (unsigned)(col + sp * ga) : 0u];
+        vc[2 * j + 1] = in[okb ? (unsigned)(col + sp * gb) : 0u];
       }
     }
   };
@@ -242,39 +244,35 @@ struct KApplyZm {
   auto plane = [&](const int zz, TR (&sr)[PRO ? 1 : PER], TD (&sd)[ND], uint32_t (&sf)[PF], TR (&sc)[PC]) {
     T (*const sB)[SX][SY] = sU[kb];
     if constexpr (PRO) {
-      // e = P x_c at the staged positions (kernel/InterpolationDeviation.m:3-17):
-      // the weighted sum over the eight coarse corners in the order of the
-      // stand-alone prolongation, 0 at void nodes and fixed components.  sC was
-      // last read before the previous plane's barrier.
+      // e = P x_c at the staged positions (kernel/InterpolationDeviation.m:3-17)
+      // in the order of prolong_zplanes / prolong_xy, as the stand-alone
+      // prolongation: the patch's two planes are combined into sC, then every
+      // staged value reads its four corners there; 0 at void nodes and fixed
+      // components.  sC was last read before the previous plane's barrier.
+      const int zo = (zz - 1) & 1;
+      const T wz0 = zo ? (T)0.5 : (T)1, wz1 = zo ? (T)0.5 : (T)0;
 #pragma unroll
-      for (int j = 0; j < PC; ++j) {
-        const int k = tid + j * NT;
-        if (k < NC) sC[k] = (T)sc[j];
+      for (int j = 0; j < PQ; ++j) {
+        const int q = tid + j * NT;
+        if (q < NQ) sC[q] = prolong_zplanes(wz0, wz1, (T)sc[2 * j], (T)sc[2 * j + 1]);
       }
       __syncthreads();
       // (the position words pass through an empty asm each plane, so that
       // what is derived from them is not kept in registers across planes)
-      const int zo = (zz - 1) & 1;
-      const T wz0 = zo ? (T)0.5 : (T)1, wz1 = zo ? (T)0.5 : (T)0;
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
         const int e = tid + i * NT;
         if (e < NSTAGE) {
           int w = ppos[i];
           asm volatile("" : "+v"(w));
-          const int c = (w >> 14) & 3, yo = (w >> 12) & 1, xo = (w >> 13) & 1, base = w & 0xfff;
+          const int c = (w >> 14) & 3, yo = (w >> 12) & 1, xo = (w >> 13) & 1, o = w & 0xfff;
           const uint32_t f = sf[i];
           T v = 0;
           if ((f & F_ELEM_MASK) && !(f & (1u << (F_FIX_SHIFT + c)))) {
             const T wy0 = yo ? (T)0.5 : (T)1, wy1 = yo ? (T)0.5 : (T)0;
             const T wx0 = xo ? (T)0.5 : (T)1, wx1 = xo ? (T)0.5 : (T)0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-              const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
-              const T wt = ((ky ? wy1 : wy0) * (kx ? wx1 : wx0)) * (kz ? wz1 : wz0);
-              const int o = base + (ky & yo) + CY * (kx & xo) + 3 * CY * CX * (kz & zo);
-              v += wt * sC[o];
-            }
+            const int ox = CY * xo;
+            v = prolong_xy(wy0, wy1, wx0, wx1, sC[o], sC[o + ox], sC[o + yo], sC[o + yo + ox]);
           }
           (&sB[0][0][0])[e] = v;
         }
