@@ -972,9 +972,13 @@ void autotune_levels(gmgs_ctx* c) {
       for (const auto& cd : cand) {
         DevBuf tl;
         const long long nt = build_coltiles(l1.mesh.g, l1.mesh.flags.as<uint32_t>(), cd[0], cd[1], tl, s);
-        const float ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
-                                                 c->q.as<double>(), fused ? AP_AX_P : AP_AX,
-                                                 fused ? &co : nullptr);
+        // (fused: the iterations alternate between the kernel with the x
+        // updates and the one without; both are timed)
+        float ms = time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
+                                           c->q.as<double>(), fused ? AP_AX_P : AP_AX, fused ? &co : nullptr);
+        if (fused)
+          ms += time_zm<double, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
+                                        c->q.as<double>(), AP_AX_P_NOX, &co);
         if (clearly_faster(ms, best_ms)) { best_ms = ms; best = {cd[0], cd[1]}; }
       }
       std::lock_guard<std::mutex> lk(tune_mutex());
@@ -1209,7 +1213,7 @@ int run_vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* par
 // p = z + beta p_old, q = A p (into c->q), x += alpha p_old and the p'q
 // partials in one level-1 z-march; returns the number of partials
 int launch_matvec_p(gmgs_ctx* c, const double* p_old, double* p, double* x, const float* z, double* part,
-                    const int* done) {
+                    const int* done, bool with_x) {
   Level& l = L1(c);
   CycleOut<double> co{};
   co.zf = z;
@@ -1220,7 +1224,8 @@ int launch_matvec_p(gmgs_ctx* c, const double* p_old, double* p, double* x, cons
   const bool own = !l.coltiles64.empty();
   return launch_apply_zm<double, double>(op, own ? l.coltiles64.as<ColTile>() : l.coltiles.as<ColTile>(),
                                          own ? l.ncol64 : l.ncol, own ? l.col_tx64 : l.col_tx,
-                                         own ? l.col_zc64 : l.col_zc, AP_AX_P, p_old, 0.0, nullptr,
+                                         own ? l.col_zc64 : l.col_zc, with_x ? AP_AX_P : AP_AX_P_NOX, p_old,
+                                         0.0, nullptr,
                                          c->q.as<double>(), part, done, c->stream, &co);
 }
 
@@ -1256,14 +1261,17 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   double* p = c->p.as<double>();
   if constexpr (std::is_same<T, float>::value) {
     if (fused_direction(c)) {
-      // p = z + beta p_old, q = A p, p'q and the previous iteration's
-      // x += alpha p_old in one z-march (CG_solver.m:23-31); p alternates
-      // between two buffers since tiles read their neighbours' p_old
+      // p = z + beta p_old, q = A p and p'q in one z-march (CG_solver.m:23-31);
+      // p alternates between two buffers since tiles read their neighbours'
+      // p_old.  The x updates of the last two iterations, x += alpha_prev p_2
+      // and x += alpha p_old (p_2 is still in the buffer the new p goes to),
+      // ride in the even iterations' z-march, so x is read and written every
+      // other iteration
       p = (k & 1) ? c->p2.as<double>() : c->p.as<double>();
       const double* p_old = (k & 1) ? c->p.as<double>() : c->p2.as<double>();
       {
         KTime t(c, GMGS_K_MATVEC64_P);
-        npq = launch_matvec_p(c, p_old, p, c->x.as<double>(), z, part, done);
+        npq = launch_matvec_p(c, p_old, p, c->x.as<double>(), z, part, done, (k & 1) == 0);
       }
       launch_finalize(2, part, npq, sc, nullptr, nullptr, done, s);
       launch_update_xr(g, fl, tl, l.ntiles, nullptr, p, ra, c->q.as<double>(), c->r.as<double>(), nullptr,
@@ -1334,9 +1342,9 @@ struct LoopResult {
 void finish_deferred_x(gmgs_ctx* c, hipStream_t s) {
   if (!c->no_jacobi) return;
   Level& l = L1(c);
+  const bool fused = fused_direction(c);
   launch_add_alpha_p(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.tiles.as<TileRec>(), l.ntiles, c->x.as<double>(),
-                     c->p.as<double>(), fused_direction(c) ? c->p2.as<double>() : nullptr,
-                     c->sc.as<PcgScalars>(), s);
+                     c->p.as<double>(), fused ? c->p2.as<double>() : nullptr, c->sc.as<PcgScalars>(), s, fused);
 }
 
 // Batch graphs (gmgs_batch_solve): one iteration pair of each context of a
@@ -2694,18 +2702,22 @@ int gmgs_time_operator(gmgs_ctx* c, int kernel, int reps, double* avg_ms) {
     return fail(c, GMGS_ERR_STATE, "this solver does not fuse the direction update into its matvec");
   return guarded(c, [&]() -> int {
     GMGS_HIP(hipSetDevice(c->device));
+    int nl = 0;
     auto launch = [&]() {
       if (kernel == GMGS_K_MATVEC64)
         apply_level<double, double>(c, 0, AP_AX, c->p.as<double>(), 0.0, c->q.as<double>(), nullptr,
                                     c->part.as<double>(), nullptr);
       else if (kernel == GMGS_K_MATVEC64_P)
+        // alternating as in the iteration: with the x updates, then without
         launch_matvec_p(c, c->tmp.as<double>(), c->tmp2.as<double>(), c->ra.as<double>(),
-                        c->z[0].as<float>(), c->part.as<double>(), nullptr);
+                        c->z[0].as<float>(), c->part.as<double>(), nullptr, (nl++ & 1) == 0);
       else
         apply_level<double, double>(c, 0, AP_SMOOTH, c->r.as<double>(), c->prm.jacobi_weight,
                                     c->tmp.as<double>(), nullptr, nullptr, nullptr);
     };
     launch();
+    launch();
+    nl = 0;
     HipEvent e0, e1;
     GMGS_HIP(hipEventRecord(e0, c->stream));
     for (int i = 0; i < reps; ++i) launch();

@@ -37,6 +37,7 @@ struct PcgScalars {
   double pq;
   double rr;
   double alpha, beta;
+  double alpha_prev;  // alpha of the iteration before (the two-step x update of AX_P)
   double relres;
   double tol;
   int its;
@@ -56,12 +57,17 @@ struct PcgScalars {
 // z = omega (r0 + Q_0) / d + e instead of Q_0 (CycleOut without R1, e, w1).
 // AX_P: the PCG's direction update fused into its matvec (level 1, FP64, FP32
 // z): p = z + beta p_old from in = p_old, out = A p, p written to CycleOut::pn
-// (a buffer other than in), x += alpha p_old, and the p.q partials.
+// (a buffer other than in), and the p.q partials; x takes the two deferred
+// updates x = (x + alpha_prev p_2) + alpha p_old, p_2 being the direction
+// before p_old, read from CycleOut::pn before it is overwritten (none in the
+// first iteration).  AX_P_NOX: the same without the x update (the iterations
+// alternate between the two, so x is read and written every other one).
 // RESID_SUM_P: RESID_SUM with in = x_c, the coarse level's correction on the
 // grid CycleOut::gc: every staged value of e = P x_c (fixed components 0) is
 // formed from the coarse values while it is staged, so e is never stored
 enum ApplyMode {
-  AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3, AP_RESID_SUM = 4, AP_AX_P = 5, AP_RESID_SUM_P = 6
+  AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3, AP_RESID_SUM = 4, AP_AX_P = 5, AP_RESID_SUM_P = 6,
+  AP_AX_P_NOX = 7
 };
 
 // outputs of AP_SMOOTH_SUM: z = (omega (r0 + Q_0) + w1 (R1 + Q_1)) / d + e,
@@ -192,9 +198,12 @@ void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles,
 
 // x += alpha p, unconditionally (the last deferred x update, after the PCG
 // loop); with p_odd, the directions alternate between p (even iterations)
-// and p_odd, and the last iteration's one is taken
+// and p_odd, and the last iteration's one is taken; with 'two' as well (the
+// AX_P / AX_P_NOX alternation), after an odd last iteration the one before it
+// is still pending too: x = (x + alpha_prev p) + alpha p_odd
 void launch_add_alpha_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, double* x,
-                        const double* p, const double* p_odd, const PcgScalars* sc, hipStream_t s);
+                        const double* p, const double* p_odd, const PcgScalars* sc, hipStream_t s,
+                        bool two = false);
 
 // x = (x + omega r / d) + alpha p (the Jacobi increment of the iteration, from
 // r before it is replaced; none when invd is null; no x update when x is

@@ -1091,6 +1091,7 @@ struct KAddAlphaP {
   const double* p;
   const double* p_odd;
   const PcgScalars* sc;
+  int two;
   __device__ __forceinline__ void run(int block_, int nblocks_) const {
     Grid g = this->g;
     const uint32_t* __restrict__ flags = this->flags;
@@ -1103,22 +1104,29 @@ struct KAddAlphaP {
     (void)block_;
     (void)nblocks_;
 
-  const double alpha = sc->alpha;
+  const double alpha = sc->alpha, alpha2 = sc->alpha_prev;
   // directions alternating between two buffers: iteration k (its = k + 1
-  // after it) wrote p[k & 1]
-  if (p_odd && ((sc->its - 1) & 1)) p = p_odd;
+  // after it) wrote p[k & 1]; with 'two', an odd last iteration leaves the
+  // even one before it pending as well
+  const bool odd = ((sc->its - 1) & 1) != 0;
+  const double* __restrict__ p2 = (two && odd) ? p : nullptr;
+  if (p_odd && odd) p = p_odd;
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
     if (!ln_.active) continue;
     const long long n = ln_.n;
-    double xo[3], po[3];
+    double xo[3], po[3], qo[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       xo[c] = x[c * N + n];
       po[c] = p[c * N + n];
+      qo[c] = p2 ? p2[c * N + n] : 0.0;
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) x[c * N + n] = xo[c] + alpha * po[c];
+    for (int c = 0; c < 3; ++c) {
+      const double x1 = p2 ? fma_t(alpha2, qo[c], xo[c]) : xo[c];
+      x[c * N + n] = fma_t(alpha, po[c], x1);
+    }
   }
   }
 };
@@ -1579,6 +1587,7 @@ struct KFinalize {
     const double pq = reduce_parts(part, nparts);
     if (threadIdx.x == 0) {
       sc->pq = pq;
+      sc->alpha_prev = sc->alpha;
       sc->alpha = sc->rz / pq;                                      // CG_solver.m:31
     }
   } else {
@@ -1941,9 +1950,10 @@ void launch_update_p(const Grid& g, const uint32_t* flags, const TileRec* tiles,
 }
 
 void launch_add_alpha_p(const Grid& g, const uint32_t* flags, const TileRec* tiles, long long nt, double* x,
-                        const double* p, const double* p_odd, const PcgScalars* sc, hipStream_t s) {
+                        const double* p, const double* p_odd, const PcgScalars* sc, hipStream_t s, bool two) {
   if (nt <= 0) return;
-  launch(KAddAlphaP{g, flags, tiles, nt, x, p, p_odd, sc}, apply_blocks(nt), s);
+  if (two && !p_odd) throw std::invalid_argument("add_alpha_p: the two-step update needs both direction buffers");
+  launch(KAddAlphaP{g, flags, tiles, nt, x, p, p_odd, sc, two ? 1 : 0}, apply_blocks(nt), s);
   GMGS_LAUNCH_CHECK();
 }
 

@@ -95,7 +95,8 @@ struct KApplyZm {
   constexpr bool PRO = MODE == AP_RESID_SUM_P;       // e = P x_c formed while staged
   constexpr bool SUM = MODE == AP_SMOOTH_SUM, RSUM = MODE == AP_RESID_SUM || PRO;
   constexpr bool SM = MODE == AP_SMOOTH || SUM;
-  constexpr bool AXP = MODE == AP_AX_P;              // p = z + beta p_old staged, x updated
+  constexpr bool AXP = MODE == AP_AX_P || MODE == AP_AX_P_NOX;   // p = z + beta p_old staged
+  constexpr bool XUP = MODE == AP_AX_P;              // AX_P: x takes the two deferred updates
   constexpr bool TWO = SUM || RSUM;                 // two dot partials (r0.z, r0.zprev)
   constexpr int ND = (SM || AXP) ? PER : 1;          // second staged operand: 1/d or z
   using TD = typename std::conditional<AXP, float, T>::type;
@@ -113,7 +114,7 @@ struct KApplyZm {
   // buffer k % 4 and read while computing planes k, k + 1 and k + 2 (stencil,
   // output completion, element corrections)
   __shared__ T sU[4][3][SX][SY];
-  __shared__ T sR[(SM || AXP) ? 2 : 1][3][TX][TY];   // SMOOTH: centre values of r; AX_P: of p_old
+  __shared__ T sR[(SM || XUP) ? 2 : 1][3][TX][TY];   // SMOOTH: centre values of r; AX_P: of p_old
   __shared__ T sC[PRO ? NQ : 1];                     // PRO: the patch, its two planes combined
   // per wave: the element corrections' task list and results
   __shared__ int sTask[FAST_ONLY ? 1 : NT / 64][64];
@@ -227,10 +228,14 @@ struct KApplyZm {
   TD dA[ND], dB[ND];
   uint32_t gA[PF], gB[PF];                          // PRO: flags of the staged positions
   TR cA[PC], cB[PC];                                // PRO: the coarse patch
-  T beta = 0, alpha = 0;
+  T beta = 0, alpha = 0, alpha2 = 0;
+  bool xup = false;
   if (AXP) {
     beta = (T)co.sc->beta;
     alpha = (T)co.sc->alpha;
+    alpha2 = (T)co.sc->alpha_prev;
+    // nothing is pending in the first iteration (its = 1 after its beta step)
+    xup = XUP && co.sc->its != 1;
   }
   fetch(z0 - 1, rA, dA, gA, cA);
   fetch(z0, rB, dB, gB, cB);
@@ -293,7 +298,7 @@ struct KApplyZm {
     // zz - 4 .. zz - 2, which every thread finished before the barrier of
     // plane zz - 1
     __syncthreads();
-    if (SM || AXP) {
+    if (SM || XUP) {
       // centre values of r (p_old) for the output completed at plane zz + 1; buffer
       // zz % 2 was last read at plane zz - 1, before this plane's barrier
 #pragma unroll
@@ -310,7 +315,7 @@ struct KApplyZm {
     // RESID: centre values of in2 at the output completed in this plane
     // (AX_P: of x), loaded now and used after the stencil
     T cen[3] = {0, 0, 0};
-    if ((MODE == AP_RESID || RSUM || AXP) && lane_ok && (fM & OUT_MASK)) {
+    if ((MODE == AP_RESID || RSUM || (AXP && xup)) && lane_ok && (fM & OUT_MASK)) {
       const long long n = colidx + sz * (zz - 1);
 #pragma unroll
       for (int c = 0; c < 3; ++c) cen[c] = AXP ? (T)co.x[c * N + n] : in2[c * N + n];
@@ -483,12 +488,17 @@ struct KApplyZm {
           out[c * N + n] = a;
           if (partial) dsum += (double)sU[b1][c][lx + 1][ly + 1] * (double)a;
         } else if (AXP) {
-          // q = A p, the new p, and the previous iteration's x += alpha p_old
+          // q = A p, the new p, and (AX_P) the two deferred x updates:
+          // x = (x + alpha_prev p_2) + alpha p_old, p_2 read from the new p's
+          // buffer before it is overwritten
           const long long m = c * N + n;
           const T pn = sU[b1][c][lx + 1][ly + 1];
           out[m] = a;
+          if (XUP && xup) {
+            const T p2 = co.pn[m];
+            co.x[m] = fma_t(alpha, sR[zo_ & 1][c][lx][ly], fma_t(alpha2, p2, cen[c]));
+          }
           co.pn[m] = pn;
-          co.x[m] = cen[c] + alpha * sR[zo_ & 1][c][lx][ly];
           if (partial) dsum += (double)pn * (double)a;
         } else if (MODE == AP_RESID) {
           const T v = cen[c] - a;
@@ -651,7 +661,8 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   if (mode == AP_RESID_SUM_P &&
       (co->gc.ey * 2 != op.g.ey || co->gc.ex * 2 != op.g.ex || co->gc.ez * 2 != op.g.ez || 3 * co->gc.N >= (1LL << 31)))
     throw std::invalid_argument("RESID_SUM_P: the coarse grid is not this level's next coarser grid");
-  if (mode == AP_AX_P && (!co || e != 0 || !co->zf || !co->pn || !co->x || !co->sc || co->pn == (const void*)in))
+  if ((mode == AP_AX_P || mode == AP_AX_P_NOX) &&
+      (!co || e != 0 || !co->zf || !co->pn || (mode == AP_AX_P && !co->x) || !co->sc || co->pn == (const void*)in))
     throw std::invalid_argument("AX_P: level 1 only, with z, x, the scalars and a separate new p");
 #define GMGS_ZM(MODE, E, TXV) \
   launch(KApplyZm<T, TR, MODE, E, TXV>{op, tiles, zc, in, omega, in2, out, partial, done, cov}, ntiles, s)
@@ -682,6 +693,9 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
     else throw std::invalid_argument("RESID_SUM_P: input in the cycle precision");
   } else if (mode == AP_AX_P) {
     if constexpr (std::is_same<T, double>::value && std::is_same<TR, double>::value) GMGS_ZM_TX(AP_AX_P, 0);
+    else throw std::invalid_argument("AX_P: FP64 operator and input");
+  } else if (mode == AP_AX_P_NOX) {
+    if constexpr (std::is_same<T, double>::value && std::is_same<TR, double>::value) GMGS_ZM_TX(AP_AX_P_NOX, 0);
     else throw std::invalid_argument("AX_P: FP64 operator and input");
   } else {
     GMGS_ZM_E(AP_SMOOTH);

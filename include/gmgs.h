@@ -218,8 +218,9 @@ enum {
   GMGS_K_PROLONG1 = 4,    /* level 2 -> 1 prolongation + post-smoothing (Vcycle.m:29-32) */
   GMGS_K_ITERATION = 5,   /* one whole PCG iteration */
   GMGS_K_MATVEC64_P = 6   /* q = A p with the direction update fused into it: p = z + beta p_old,
-                             x += alpha p_old (CG_solver.m:23-31; mixed precision without the
-                             Jacobi step, z-marching level 1) */
+                             and in every other iteration the two deferred x += alpha p updates
+                             (CG_solver.m:23-31; mixed precision without the Jacobi step,
+                             z-marching level 1) */
 };
 int gmgs_kernel_timing(gmgs_ctx* c, int enable);
 /* Plain device copy of nbytes (read + write), reps times; mean duration.  Used
@@ -232,7 +233,9 @@ int gmgs_kernel_timing_read(gmgs_ctx* c, int kernel, double* avg_ms, int64_t* co
  * kernel = GMGS_K_MATVEC64 (q = A p with its p'q partials),
  * GMGS_K_JACOBI64 (the Jacobi step into scratch vectors) or
  * GMGS_K_MATVEC64_P (the fused direction update and matvec, on scratch
- * vectors; GMGS_ERR_STATE where the solver does not use it).  Needs
+ * vectors, alternating as in the iteration between the launch that applies
+ * the deferred x updates and the one that does not; GMGS_ERR_STATE where the
+ * solver does not use it).  Needs
  * gmgs_set_bc; leaves the solution untouched. */
 int gmgs_time_operator(gmgs_ctx* c, int kernel, int reps, double* avg_ms);
 
