@@ -1315,13 +1315,15 @@ void pcg_init(gmgs_ctx* c, double tol) {
   const uint32_t* fl = l.mesh.flags.as<uint32_t>();
   const TileRec* tl = l.tiles.as<TileRec>();
   // With the direction update fused into the matvec, iteration 0 reads
-  // only p2 (zeroed: x += alpha p2 there) and writes p and z0 before any
-  // read; z1 enters only r'z_prev, which the first iteration's beta = 0
-  // does not use.
+  // neither p2 nor x (its p_old is not read, and x is first read by the third
+  // iteration's update, which takes it as 0), so neither is written here; p
+  // and z0 are written before any read; z1 enters only r'z_prev, which the
+  // first iteration's beta = 0 does not use.
   const bool fused = fused_direction(c);
   if (c->mixed)
-    launch_pcg_start<float>(g, fl, tl, l.ntiles, c->b.as<double>(), c->x.as<double>(),
-                            fused ? nullptr : c->p.as<double>(), c->p2.empty() ? nullptr : c->p2.as<double>(),
+    launch_pcg_start<float>(g, fl, tl, l.ntiles, c->b.as<double>(), fused ? nullptr : c->x.as<double>(),
+                            fused ? nullptr : c->p.as<double>(),
+                            fused || c->p2.empty() ? nullptr : c->p2.as<double>(),
                             c->r.as<double>(), fused ? nullptr : c->z[0].as<float>(),
                             fused ? nullptr : c->z[1].as<float>(), c->part.as<double>(), s);
   else

@@ -1110,6 +1110,9 @@ struct KAddAlphaP {
   // even one before it pending as well
   const bool odd = ((sc->its - 1) & 1) != 0;
   const double* __restrict__ p2 = (two && odd) ? p : nullptr;
+  // with 'two', x is first written by the third iteration's update: after a
+  // one- or two-iteration solve it still counts as 0 (the start leaves it)
+  const bool x0 = two && sc->its <= 2;
   if (p_odd && odd) p = p_odd;
   const long long N = g.N;
   GMGS_TILE_LOOP(g, flags, tiles, nt, ln_) {
@@ -1118,7 +1121,7 @@ struct KAddAlphaP {
     double xo[3], po[3], qo[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      xo[c] = x[c * N + n];
+      xo[c] = x0 ? 0.0 : x[c * N + n];
       po[c] = p[c * N + n];
       qo[c] = p2 ? p2[c * N + n] : 0.0;
     }
@@ -1465,7 +1468,7 @@ struct KPcgStart {
     for (int c = 0; c < 3; ++c) {
       const long long m = c * N + n;
       const double v = b[m];
-      x[m] = 0.0;
+      if (x) x[m] = 0.0;
       if (p) p[m] = 0.0;
       if (p2) p2[m] = 0.0;
       r[m] = v;

@@ -186,6 +186,8 @@ struct KApplyZm {
   // every padded array is a corner halo entry and holds 0, so a skipped
   // position loads from there: the loads need no branches.
   // The index is formed in 32 bits (3 N < 2^31) and selected, not branched on.
+  // AX_P: the first iteration of a solve (its = 1 after its beta step)
+  const bool pfirst = AXP && co.sc->its == 1;
   auto fetch = [&](int zz, TR* vr, TD* vd, uint32_t* vf, TR* vc) {
     const int szz = (int)sz * zz;
 #pragma unroll
@@ -195,7 +197,9 @@ struct KApplyZm {
       if constexpr (PRO) {
         vf[i] = op.flags[m];
       } else {
-        vr[i] = in[m];
+        // AX_P, first iteration: beta = 0 and p_old is not read (index 0, a
+        // halo entry, holds 0)
+        vr[i] = in[pfirst ? 0u : m];
         if (SM) vd[i] = (TD)op.invd[m];
         if (AXP) vd[i] = (TD)co.zf[m];
       }
@@ -229,13 +233,16 @@ struct KApplyZm {
   uint32_t gA[PF], gB[PF];                          // PRO: flags of the staged positions
   TR cA[PC], cB[PC];                                // PRO: the coarse patch
   T beta = 0, alpha = 0, alpha2 = 0;
-  bool xup = false;
+  bool xup = false, xzero = false;
   if (AXP) {
     beta = (T)co.sc->beta;
     alpha = (T)co.sc->alpha;
     alpha2 = (T)co.sc->alpha_prev;
-    // nothing is pending in the first iteration (its = 1 after its beta step)
+    // nothing is pending in the first iteration (its = 1 after its beta
+    // step), and x is still 0 in the third (its = 3), the first to update it:
+    // the start of the solve writes neither
     xup = XUP && co.sc->its != 1;
+    xzero = co.sc->its == 3;
   }
   fetch(z0 - 1, rA, dA, gA, cA);
   fetch(z0, rB, dB, gB, cB);
@@ -315,7 +322,7 @@ struct KApplyZm {
     // RESID: centre values of in2 at the output completed in this plane
     // (AX_P: of x), loaded now and used after the stencil
     T cen[3] = {0, 0, 0};
-    if ((MODE == AP_RESID || RSUM || (AXP && xup)) && lane_ok && (fM & OUT_MASK)) {
+    if ((MODE == AP_RESID || RSUM || (AXP && xup && !xzero)) && lane_ok && (fM & OUT_MASK)) {
       const long long n = colidx + sz * (zz - 1);
 #pragma unroll
       for (int c = 0; c < 3; ++c) cen[c] = AXP ? (T)co.x[c * N + n] : in2[c * N + n];

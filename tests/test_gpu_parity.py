@@ -612,14 +612,17 @@ def test_chebyshev_cycle_solves(case, oracle, pkg, nu, nu_coarse, nu2):
     s3.close()
 
 
-@pytest.mark.parametrize("max_it", [2, 3])
+@pytest.mark.parametrize("max_it", [0, 1, 2, 3])
 def test_mixed_direction_update_fixed_iterations(case, oracle, pkg, max_it):
     """The mixed solver without the Jacobi step fuses p = z + beta p and the
     deferred x += alpha p into its level-1 matvec, with p alternating between
     two buffers; stopped after an odd and an even number of iterations
     (max_it + 1), x must match the FP64 flexible PCG of the oracle with the
     same sweep weights (a wrong buffer for the last x update, or a stale p,
-    is an O(1) error here, FP32 cycle rounding ~1e-6)."""
+    is an O(1) error here, FP32 cycle rounding ~1e-6).  x is updated every
+    other iteration and is not cleared at the start of a solve, so 1 and 2
+    iterations (nothing written to x before the final update) and repeated
+    solves (the previous solution left in x) are covered too."""
     m = case["m"]
     s3 = cheb_solver(case, pkg, pkg.GMGS_PREC_MIXED, 1, 4, 2, tol=1e-30, max_it=max_it)
     w = {ii: s3.smoother(ii)[1] for ii in range(1, m.numLevels)}
