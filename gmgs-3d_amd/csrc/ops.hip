@@ -1518,34 +1518,37 @@ struct KResidSq {
   }
 };
 
-// one block of 256 threads; fixed-order sums (thread t adds part[t],
-// part[t + 256], ... in turn; the loads are issued eight at a time so the
-// chain of adds does not wait for each load in turn)
+// one block of kFinalizeThreads threads; fixed-order sums (thread t adds
+// part[t], part[t + T], ... in turn; the loads are issued eight at a time so
+// the chain of adds does not wait for each load in turn)
+constexpr int kFinalizeThreads = 1024;
 __device__ inline double reduce_parts(const double* __restrict__ part, int n) {
+  constexpr int T = kFinalizeThreads;
   double v = 0.0;
   int i = threadIdx.x;
-  for (; i + 7 * 256 < n; i += 8 * 256) {
+  for (; i + 7 * T < n; i += 8 * T) {
     double a[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = part[i + u * 256];
+    for (int u = 0; u < 8; ++u) a[u] = part[i + u * T];
 #pragma unroll
     for (int u = 0; u < 8; ++u) v += a[u];
   }
-  for (; i < n; i += 256) v += part[i];
-  __shared__ double sh[256];
-  sh[threadIdx.x] = v;
+  for (; i < n; i += T) v += part[i];
+  // the wave's sum by shuffles, then the block's 16 wave sums in order
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __shared__ double sh[T / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
-  }
-  double r = sh[0];
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < T / 64; ++w) r += sh[w];
   __syncthreads();
   return r;
 }
 
 struct KFinalize {
-  static constexpr int kThreads = 256;
+  static constexpr int kThreads = kFinalizeThreads;
   int which;
   const double* part;
   int nparts;
