@@ -71,6 +71,9 @@ struct Level {
   // level 1 of the symmetric V(1, 1) cycle: the residual kernel forms
   // e = P x_c while staging it (else a prolongation kernel stores e first)
   bool fuse_prolong = true;
+  // the FP32 inverse diagonal every F_FAST node of the level holds, if one
+  bool ufast_invd = false;
+  float ucinvd[3] = {0.f, 0.f, 0.f};
   // level 1 of a mixed context: the FP64 matvec's own column tiles (the
   // cycle's FP32 sweeps are tuned on coltiles)
   DevBuf coltiles64;
@@ -576,6 +579,9 @@ void build_operators(gmgs_ctx* c) {
     if (c->mixed) {
       if (!l.Ks64.empty()) to_f32(l.Ks64, l.Ks32, (long long)(l.Ks64.bytes() / sizeof(double)), s);
       if (!l.invd64.empty()) to_f32(l.invd64, l.invd32, 3 * l.mesh.g.N, s);
+      l.ufast_invd = li == 0 && !l.invd32.empty() &&
+                     fast_inverse_diagonal(l.mesh.g, l.tiles.as<TileRec>(), l.ntiles, l.mesh.flags.as<uint32_t>(),
+                                           l.invd32.as<float>(), l.ucinvd, s);
     }
     // z-marching operator where a universal element matrix exists
     l.zm = !c->simple_kernels && l.classes_ok && (li == 0 ? !c->simp : l.state == 2);
@@ -1163,6 +1169,10 @@ int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* par
       // while it stages it (e is then never stored)
       CycleOut<T> co{rin, nullptr, nullptr, zprev, z, (T)0};
       co.gc = cr.mesh.g;
+      if constexpr (std::is_same<T, float>::value) {
+        co.ufast = l.ufast_invd && !c->simple_kernels ? 1 : 0;
+        for (int k = 0; k < 3; ++k) co.ucinvd[k] = l.ucinvd[k];
+      }
       if (!l.fuse_prolong)
         launch_prolong<T>(g, fl, tl, l.ntiles, cr.mesh.g, vec<T>(cr.x), vec<T>(l.e), true, done, s);
       nz = launch_apply_zm<T, T>(make_op<T>(c, 0, true), l.coltiles.as<ColTile>(), l.ncol, l.col_tx, l.col_zc,

@@ -87,6 +87,11 @@ struct CycleOut {
   const PcgScalars* sc;
   // RESID_SUM_P: the coarse grid of in = x_c
   Grid gc;
+  // RESID_SUM / RESID_SUM_P: with ufast, an F_FAST output node takes its
+  // inverse diagonal from ucinvd (the value every F_FAST node holds) instead
+  // of loading it
+  int ufast;
+  T ucinvd[3];
 };
 
 // The prolongation's arithmetic at one fine node, shared by the stand-alone

@@ -517,7 +517,9 @@ struct KApplyZm {
           const long long m = c * N + n;
           const double r0 = co.r0[m];
           const bool fx = (fM & (1u << (F_FIX_SHIFT + c))) != 0;
-          T v = op.invd[m] * (omega * ((T)r0 + (cen[c] - a))) + sU[b1][c][lx + 1][ly + 1];
+          // (fixed components are set to 0 below, whatever their diagonal)
+          const T di = (co.ufast && (fM & F_FAST)) ? co.ucinvd[c] : op.invd[m];
+          T v = di * (omega * ((T)r0 + (cen[c] - a))) + sU[b1][c][lx + 1][ly + 1];
           if (fx) v = 0;
           co.z[m] = v;
           if (partial) {

@@ -47,6 +47,12 @@ void scatter_eop(const long long* elem_pidx, long long ne, const int* value, int
                  hipStream_t s);
 
 // inverse Jacobi diagonal (SetupSolver.m:204-244), FP64, SoA 3N, 0 off solid nodes
+// the inverse diagonal shared by every F_FAST node of a level (all eight
+// elements present and universal), per component: true and out[] when every
+// F_FAST node holds the same float bits, false otherwise (or with none)
+bool fast_inverse_diagonal(const Grid& g, const TileRec* tiles, long long nt, const uint32_t* flags,
+                           const float* invd, float out[3], hipStream_t s);
+
 void inverse_diagonal(const Grid& g, const TileRec* tiles, long long nt, const uint32_t* flags,
                       const int* eop, const double* Ke, const double* Ks, const double* emod,
                       double* invd, hipStream_t s);

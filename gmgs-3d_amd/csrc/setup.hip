@@ -1,5 +1,7 @@
 // Solver set-up kernels (kernel/SetupSolver.m).
 #include "setup.h"
+
+#include <cstring>
 #include "mesh.h"
 
 namespace gmgs {
@@ -63,6 +65,23 @@ __device__ inline bool lane_pos(const Grid& g, const TileRec* __restrict__ tiles
   if (xp >= g.px) return false;
   n = yp + (long long)g.py * (xp + (long long)g.px * zp);
   return true;
+}
+
+// per component the smallest and largest bit pattern of a positive float
+// inverse diagonal over the F_FAST nodes (mm: min[3], max[3])
+__global__ void k_fast_invd_range(Grid g, const TileRec* __restrict__ tiles, long long nt,
+                                  const uint32_t* __restrict__ flags, const float* __restrict__ invd,
+                                  unsigned* __restrict__ mm) {
+  long long n;
+  int yp, xp, zp;
+  if (!lane_pos(g, tiles, nt, n, yp, xp, zp)) return;
+  if (!(flags[n] & F_FAST)) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const unsigned b = __float_as_uint(invd[c * g.N + n]);
+    atomicMin(&mm[c], b);
+    atomicMax(&mm[3 + c], b);
+  }
 }
 
 __global__ void k_restrict_fixed(Grid gc, const TileRec* __restrict__ tiles, long long nt,
@@ -335,6 +354,27 @@ void scatter_eop(const long long* elem_pidx, long long ne, const int* value, int
   if (ne <= 0) return;
   k_scatter_eop<<<grid1d(ne, BS), BS, 0, s>>>(elem_pidx, ne, value, eop);
   GMGS_LAUNCH_CHECK();
+}
+
+bool fast_inverse_diagonal(const Grid& g, const TileRec* tiles, long long nt, const uint32_t* flags,
+                           const float* invd, float out[3], hipStream_t s) {
+  if (nt <= 0) return false;
+  const unsigned init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  unsigned h[6];
+  DevBuf mm;
+  mm.alloc(sizeof(init));
+  GMGS_HIP(hipMemcpyAsync(mm.as<void>(), init, sizeof(init), hipMemcpyHostToDevice, s));
+  k_fast_invd_range<<<apply_blocks(nt), 256, 0, s>>>(g, tiles, nt, flags, invd, mm.as<unsigned>());
+  GMGS_LAUNCH_CHECK();
+  GMGS_HIP(hipMemcpyAsync(h, mm.as<void>(), sizeof(h), hipMemcpyDeviceToHost, s));
+  GMGS_HIP(hipStreamSynchronize(s));
+  for (int c = 0; c < 3; ++c) {
+    if (h[c] != h[3 + c] || h[c] == 0u || h[c] == 0xffffffffu) return false;
+    unsigned b = h[c];
+    std::memcpy(&out[c], &b, sizeof(float));
+    if (!(out[c] > 0.0f)) return false;
+  }
+  return true;
 }
 
 void inverse_diagonal(const Grid& g, const TileRec* tiles, long long nt, const uint32_t* flags,
