@@ -210,8 +210,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU slab decomposition is "
-              "designed (see DESIGN.md) but not implemented. Use --gpus 1.", file=sys.stderr)
+        print("bench.py: the solver runs on one GPU; a multi-GPU slab decomposition is not "
+              "implemented. Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     pkg = load_package()

@@ -379,7 +379,7 @@ def c1_oracle(oracle, pkg):
 
 
 def test_c1_box_reference_config(c1_oracle, pkg):
-    """40x40x64 box, 4 levels, omega 0.6, rtol 1e-3, FP64."""
+    """The 40x40x64 solid box: 4 levels, omega 0.6, rtol 1e-3, FP64."""
     p, m, res, _ = c1_oracle
     s = pkg.Solver(p.vox, p.bbox, coarsest_resolution_control=p.coarsest_control,
                    jacobi_weight=p.omega, tol=p.tol, max_it=p.max_it)
