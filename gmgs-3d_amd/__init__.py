@@ -11,4 +11,4 @@ from .gmgs import (GMGS3D, GMGSError, NotConverged, Solver, default_params,  # n
                    FLAG_FR_BETA, FLAG_PR_BETA, FLAG_SIMPLE_KERNELS,
                    FLAG_AUTOTUNE, FLAG_NO_GRAPHS, FLAG_SYM_VCYCLE, FLAG_NO_JACOBI_STEP,
                    FLAG_CHEBYSHEV, FLAG_ASSEMBLED_COARSE, FLAG_SPLIT_COARSE,
-                   FLAG_UNFUSED_PROLONG)
+                   FLAG_UNFUSED_PROLONG, FLAG_UNFUSED_RESIDUAL)

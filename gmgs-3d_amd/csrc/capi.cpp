@@ -112,6 +112,7 @@ struct gmgs_ctx {
   bool tables = true;
   bool simple_kernels = false;
   bool unfused_prolong = false;   // V(1, 1) level 1: prolongation as its own kernel
+  bool unfused_resid = false;     // PCG residual update as its own kernel (see fused_residual)
   bool autotune = false;
   bool sym = false;              // symmetric V(nu, nu) cycle
   int sweeps = 2, sweeps_coarse = 2;   // nu of level 1, of the coarser smoothed levels
@@ -170,6 +171,16 @@ bool v11_fused(const gmgs_ctx* c) {
 }
 
 bool fused_direction(gmgs_ctx* c) { return c->mixed && c->no_jacobi && !c->p2.empty() && L1(c).zm; }
+
+// With the direction update fused, the residual update r = r - alpha q of
+// iteration k (and the test on |r|) moves into the first level-1 sweep of
+// iteration k + 1, which stages r - alpha q instead of r (AP_SMOOTH_R); r
+// alternates between two buffers, since tiles read their neighbours' old r
+bool fused_residual(gmgs_ctx* c) { return fused_direction(c) && c->sym && !c->unfused_resid; }
+
+// iterations a solve may launch: the test on the last one's residual runs in
+// the iteration after it when the residual update is fused
+int iteration_cap(gmgs_ctx* c) { return c->prm.max_it + 1 + (fused_residual(c) ? 1 : 0); }
 
 hipEvent_t kt_event(gmgs_ctx* c) {
   if (c->kpool_used == c->kpool.size()) {
@@ -743,20 +754,21 @@ float time_v1(gmgs_ctx* c, int li, const TR* in, T* out, bool as = false) {
 
 template <typename T, typename TR>
 float time_zm(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx, int zc, const TR* in,
-              T* out, int mode = AP_SMOOTH, const CycleOut<T>* co = nullptr, const T* in2 = nullptr) {
+              T* out, int mode = AP_SMOOTH, const CycleOut<T>* co = nullptr, const T* in2 = nullptr,
+              double* partial = nullptr) {
   Level& l = c->lev[li];
   hipStream_t s = c->stream;
   LevelOp<T> op = make_op<T>(c, li);
   HipEvent a, b;
   for (int w = 0; w < 2; ++w)
-    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, in2, out, nullptr, nullptr, s, co);
+    launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, in2, out, partial, nullptr, s, co);
   // the better of two timed batches: one disturbed batch does not decide
   const int reps = 5;
   float best = 1e30f;
   for (int batch = 0; batch < 2; ++batch) {
     GMGS_HIP(hipEventRecord(a, s));
     for (int w = 0; w < reps; ++w)
-      launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, in2, out, nullptr, nullptr, s, co);
+      launch_apply_zm<T, TR>(op, tiles, nt, tx, zc, mode, in, (T)0.5, in2, out, partial, nullptr, s, co);
     GMGS_HIP(hipEventRecord(b, s));
     GMGS_HIP(hipEventSynchronize(b));
     float ms = 0;
@@ -848,7 +860,10 @@ void autotune_levels(gmgs_ctx* c) {
     if (!l.zm && li == 0) continue;
     // the assembled stencil or the split operator was asked for
     if (li > 0 && (c->prefer_as || c->prefer_split) && l.as_built) continue;
-    const TuneKey key{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, c->mixed ? 1 : 0};
+    // (level 1 of a mixed solver whose first sweep carries the PCG's residual
+    // update is timed on that sweep and cached under its own key)
+    const bool rup = li == 0 && c->mixed && c->no_jacobi && c->sym && !c->unfused_resid;
+    const TuneKey key{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, c->mixed ? (rup ? 6 : 1) : 0};
     // (tx, zc) of the z-marching kernel; (-tx, zc) the split operator on those
     // tiles; (0, 0) the tile kernel, (0, 1) the assembled stencil
     std::pair<int, int> best{l.zm ? l.col_tx : 0, l.zm ? l.col_zc : (l.as ? 1 : 0)};
@@ -873,6 +888,21 @@ void autotune_levels(gmgs_ctx* c) {
     if (!cached) {
       float best_ms = 1e30f;
       const float prolong_ms = v11 ? time_prolong(c) : 0.f;
+      // rup: the sweep is timed with the scalars of a later iteration and
+      // alpha = 0, so the update is loaded but changes nothing (r_old = tmp,
+      // the new r into r_a, unused until a solve starts)
+      DevBuf tsc;
+      CycleOut<float> rco{};
+      if (rup) {
+        PcgScalars h{};
+        h.its = 1;
+        tsc.alloc(sizeof(PcgScalars));
+        GMGS_HIP(hipMemcpyAsync(tsc.as<void>(), &h, sizeof(h), hipMemcpyHostToDevice, s));
+        GMGS_HIP(hipStreamSynchronize(s));
+        rco.sc = tsc.as<PcgScalars>();
+        rco.qv = c->q.as<double>();
+        rco.rn = c->ra.as<double>();
+      }
       if (l.zm) {
         for (const auto& cd : cand) {
           bool fuse = false;
@@ -881,8 +911,13 @@ void autotune_levels(gmgs_ctx* c) {
           float ms;
           if (li == 0) {
             if (c->mixed) {
-              ms = time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
-                                          l.rp.as<float>());
+              // (the sweep's |r| partials, one per tile of this shape)
+              DevBuf tpart;
+              if (rup) tpart.alloc(sizeof(double) * (size_t)std::max(nt, 1LL));
+              ms = rup ? time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
+                                                l.rp.as<float>(), AP_SMOOTH_R, &rco, nullptr, tpart.as<double>())
+                       : time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
+                                                l.rp.as<float>());
               // the symmetric V(1, 1) cycle runs its residual-and-sum kernel on
               // the same tiles: time it too (scratch: e or level 2's x, R_1 =
               // rp, r0 = tmp, z = z[1]), stand-alone after the prolongation
@@ -1115,6 +1150,13 @@ T* Qbuf(Level& l, int s) {
   return vec<T>(l.wk[l.nu + s - 2]);
 }
 
+// the PCG's residual update carried by the cycle's first level-1 sweep:
+// r = r_old - alpha q
+struct ResidUpdate {
+  const double* r_old = nullptr;
+  const double* q = nullptr;
+};
+
 // Symmetric V(nu, nu) cycle (no reference counterpart): nu pre-smoothing
 // sweeps R_s = R_{s-1} - A(w_{s-1} R_{s-1}/d) from the level's input R_0, the
 // coarse correction e = P x_c, the residual recomputed as Q_0 = R_nu - A e
@@ -1124,9 +1166,12 @@ T* Qbuf(Level& l, int s) {
 // the sweeps of a level are polynomials in D^-1 A, which commute, so the
 // post-smoother is the A-adjoint of the pre-smoother for any weights.  The
 // weights are the Jacobi weight, or the reciprocal roots of the degree-nu
-// Chebyshev polynomial on the level's eigenvalue bounds.
+// Chebyshev polynomial on the level's eigenvalue bounds.  With ru, the first
+// level-1 sweep also forms rin = ru->r_old - alpha ru->q (the PCG's residual
+// update) and the test on |rin| follows it.
 template <typename T>
-int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
+int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done,
+               const ResidUpdate* ru) {
   hipStream_t s = c->stream;
   const int L = c->L;
   int nz = (int)pointwise_blocks(L1(c).ntiles);
@@ -1135,8 +1180,22 @@ int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* par
     const int nu = l.nu;
     {
       KTime t(c, li == 0 ? 2 : -1);
-      if (li == 0) apply_level<T, double>(c, 0, AP_SMOOTH, rin, (T)l.om[0], Rbuf<T>(l, 1), nullptr, nullptr,
-                                        done, nullptr, true);
+      if (li == 0 && ru) {
+        CycleOut<T> co{};
+        co.sc = c->sc.as<PcgScalars>();
+        co.qv = ru->q;
+        co.rn = const_cast<double*>(rin);
+        const int nr = launch_apply_zm<T, double>(make_op<T>(c, 0, true), l.coltiles.as<ColTile>(), l.ncol,
+                                                  l.col_tx, l.col_zc, AP_SMOOTH_R, ru->r_old, (T)l.om[0],
+                                                  nullptr, Rbuf<T>(l, 1), partial, done, s, &co);
+        // |r| and the convergence test (CG_solver.m:35-39); the rest of the
+        // iteration returns at once once it is met
+        launch_finalize(3, partial, nr, c->sc.as<PcgScalars>(), c->hist_rel.as<double>(),
+                        c->hist_t.as<unsigned long long>(), done, s);
+      } else if (li == 0) {
+        apply_level<T, double>(c, 0, AP_SMOOTH, rin, (T)l.om[0], Rbuf<T>(l, 1), nullptr, nullptr, done,
+                               nullptr, true);
+      }
       else apply_level<T, T>(c, li, AP_SMOOTH, vec<T>(l.r), (T)l.om[0], Rbuf<T>(l, 1), nullptr, nullptr, done,
                              nullptr, true);
       for (int k = 1; k < nu; ++k)
@@ -1214,8 +1273,10 @@ int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* par
 
 // z = M^-1 rin; returns the number n of r'z partials (r'zprev at [n, 2 n))
 template <typename T>
-int run_vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done) {
-  if (c->sym) return vcycle_sym<T>(c, rin, z, zprev, partial, done);
+int run_vcycle(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* partial, const int* done,
+               const ResidUpdate* ru = nullptr) {
+  if (c->sym) return vcycle_sym<T>(c, rin, z, zprev, partial, done, ru);
+  if (ru) throw std::logic_error("residual update in the cycle: symmetric cycle only");
   return vcycle<T>(c, rin, z, zprev, partial, done);
 }
 
@@ -1265,7 +1326,16 @@ void pcg_iteration(gmgs_ctx* c, int k) {
   // z = V(r) with r'z and r'z_prev partials (CG_solver.m:21)
   T* z = c->z[k & 1].as<T>();
   const T* zprev = c->flexible ? c->z[(k + 1) & 1].as<T>() : nullptr;
-  const int nz = run_vcycle<T>(c, ra, z, zprev, part, done);
+  const bool fres = fused_residual(c);
+  ResidUpdate ru{};
+  if (fres) {
+    // r of the previous iteration in one buffer, this iteration's in the
+    // other (r_a is not used without the Jacobi step)
+    ru.r_old = (k & 1) ? c->ra.as<double>() : c->r.as<double>();
+    ru.q = c->q.as<double>();
+    ra = (k & 1) ? c->r.as<double>() : c->ra.as<double>();
+  }
+  const int nz = run_vcycle<T>(c, ra, z, zprev, part, done, fres ? &ru : nullptr);
   launch_finalize(1, part, nz, sc, nullptr, nullptr, done, s);
   int npq;
   double* p = c->p.as<double>();
@@ -1284,6 +1354,7 @@ void pcg_iteration(gmgs_ctx* c, int k) {
         npq = launch_matvec_p(c, p_old, p, c->x.as<double>(), z, part, done, (k & 1) == 0);
       }
       launch_finalize(2, part, npq, sc, nullptr, nullptr, done, s);
+      if (fres) return;   // r = r - alpha q: in the next iteration's first sweep
       launch_update_xr(g, fl, tl, l.ntiles, nullptr, p, ra, c->q.as<double>(), c->r.as<double>(), nullptr,
                        c->prm.jacobi_weight, sc, part, done, s);
       launch_finalize(3, part, np, sc, c->hist_rel.as<double>(), c->hist_t.as<unsigned long long>(),
@@ -1483,7 +1554,10 @@ LoopResult pcg_run(gmgs_ctx* c) {
   if (graph && !c->gexec) capture_iteration_pair(c);
   HipEvent e0, e1;
   GMGS_HIP(hipEventRecord(e0, s));
-  const int maxk = c->prm.max_it + 1;
+  const int maxk = iteration_cap(c);
+  // the scalars' relres is that of iteration its, or of its - 1 when the
+  // residual is tested in the iteration after
+  const int rel_lag = fused_residual(c) ? 1 : 0;
   int k = 0;   // iterations launched
   PcgScalars h{};
   // queue n iterations (even with graphs: one replay is an iteration pair) and
@@ -1519,8 +1593,8 @@ LoopResult pcg_run(gmgs_ctx* c) {
     int next = prev_rel > 0.0 ? 16 : 4;   // no rate yet: a short chunk; no progress: a long one
     if (prev_rel > 0.0 && ha.its > prev_its && ha.relres > 0.0 && ha.relres < prev_rel) {
       const double rate = std::pow(ha.relres / prev_rel, 1.0 / (ha.its - prev_its));
-      const double need = std::log(ha.tol / ha.relres) / std::log(rate);   // after iteration ha.its
-      const int queued = k - ha.its;                                        // launched, not yet read
+      const double need = std::log(ha.tol / ha.relres) / std::log(rate);   // after iteration ha.its - lag
+      const int queued = k - (ha.its - rel_lag);                            // launched, not yet read
       const double want = std::ceil(need) - queued;
       next = want < 2.0 ? 2 : (want > 16.0 ? 16 : (int)want);
       next += next & 1;
@@ -1898,6 +1972,7 @@ int gmgs_create(const gmgs_params* p, const int32_t* vox, int64_t ny, int64_t nx
     c->tables = (p->flags & GMGS_FLAG_NO_INDEX_TABLES) == 0;
     c->simple_kernels = (p->flags & GMGS_FLAG_SIMPLE_KERNELS) != 0;
     c->unfused_prolong = (p->flags & GMGS_FLAG_UNFUSED_PROLONG) != 0;
+    c->unfused_resid = (p->flags & GMGS_FLAG_UNFUSED_RESIDUAL) != 0;
     c->autotune = (p->flags & GMGS_FLAG_AUTOTUNE) != 0;
     c->use_graphs = (p->flags & GMGS_FLAG_NO_GRAPHS) == 0;
     c->sym = (p->flags & GMGS_FLAG_SYM_VCYCLE) != 0;
@@ -2124,7 +2199,7 @@ int gmgs_batch_solve(gmgs_ctx** cs, int n, double** U, gmgs_solve_info* infos) {
       pcg_init(cs[i], cs[i]->prm.tol);
       GMGS_HIP(hipEventRecord(ev[i], cs[i]->stream));
       GMGS_HIP(hipStreamWaitEvent(s0, ev[i], 0));
-      maxk = std::max(maxk, cs[i]->prm.max_it + 1);
+      maxk = std::max(maxk, iteration_cap(cs[i]));
     }
     // the graph of one iteration pair of every context of 'act' (merged
     // launches per group, fork/join across groups), from the cache or built now

@@ -64,10 +64,15 @@ struct PcgScalars {
 // alternate between the two, so x is read and written every other one).
 // RESID_SUM_P: RESID_SUM with in = x_c, the coarse level's correction on the
 // grid CycleOut::gc: every staged value of e = P x_c (fixed components 0) is
-// formed from the coarse values while it is staged, so e is never stored
+// formed from the coarse values while it is staged, so e is never stored.
+// SMOOTH_R: SMOOTH from the PCG residual of the previous iteration: every
+// staged value is r = in - alpha q (FP64; q = CycleOut::qv, alpha from the
+// scalars; in as it is in the first iteration, its = 0), the sweep runs on
+// that r, and the owned nodes write r to CycleOut::rn (a buffer other than in)
+// with the partials of r.r
 enum ApplyMode {
   AP_AX = 0, AP_SMOOTH = 1, AP_RESID = 2, AP_SMOOTH_SUM = 3, AP_RESID_SUM = 4, AP_AX_P = 5, AP_RESID_SUM_P = 6,
-  AP_AX_P_NOX = 7
+  AP_AX_P_NOX = 7, AP_SMOOTH_R = 8
 };
 
 // outputs of AP_SMOOTH_SUM: z = (omega (r0 + Q_0) + w1 (R1 + Q_1)) / d + e,
@@ -92,6 +97,9 @@ struct CycleOut {
   // of loading it
   int ufast;
   T ucinvd[3];
+  // SMOOTH_R: q of the previous iteration and the new residual
+  const double* qv;
+  double* rn;
 };
 
 // The prolongation's arithmetic at one fine node, shared by the stand-alone

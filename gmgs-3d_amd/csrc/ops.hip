@@ -1598,7 +1598,9 @@ struct KFinalize {
     }
   } else {
     const double rr = reduce_parts(part, nparts);
-    if (threadIdx.x == 0) {
+    // (its = 0: the residual update fused into the next iteration's first
+    // sweep has nothing to test in the first iteration, r = b)
+    if (threadIdx.x == 0 && sc->its > 0) {
       sc->rr = rr;
       const double rel = sqrt(rr) / sc->normB;                      // CG_solver.m:35
       sc->relres = rel;

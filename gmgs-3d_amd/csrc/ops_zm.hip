@@ -94,7 +94,8 @@ struct KApplyZm {
   constexpr int PER = (NSTAGE + NT - 1) / NT;      // staged values per thread
   constexpr bool PRO = MODE == AP_RESID_SUM_P;       // e = P x_c formed while staged
   constexpr bool SUM = MODE == AP_SMOOTH_SUM, RSUM = MODE == AP_RESID_SUM || PRO;
-  constexpr bool SM = MODE == AP_SMOOTH || SUM;
+  constexpr bool RUP = MODE == AP_SMOOTH_R;         // r = in - alpha q staged, written and summed
+  constexpr bool SM = MODE == AP_SMOOTH || SUM || RUP;
   constexpr bool AXP = MODE == AP_AX_P || MODE == AP_AX_P_NOX;   // p = z + beta p_old staged
   constexpr bool XUP = MODE == AP_AX_P;              // AX_P: x takes the two deferred updates
   constexpr bool TWO = SUM || RSUM;                 // two dot partials (r0.z, r0.zprev)
@@ -114,7 +115,9 @@ struct KApplyZm {
   // buffer k % 4 and read while computing planes k, k + 1 and k + 2 (stencil,
   // output completion, element corrections)
   __shared__ T sU[4][3][SX][SY];
-  __shared__ T sR[(SM || XUP) ? 2 : 1][3][TX][TY];   // SMOOTH: centre values of r; AX_P: of p_old
+  // SMOOTH: centre values of r (SMOOTH_R: the FP64 r); AX_P: of p_old
+  using TC = typename std::conditional<RUP, TR, T>::type;
+  __shared__ TC sR[(SM || XUP) ? 2 : 1][3][TX][TY];
   __shared__ T sC[PRO ? NQ : 1];                     // PRO: the patch, its two planes combined
   // per wave: the element corrections' task list and results
   __shared__ int sTask[FAST_ONLY ? 1 : NT / 64][64];
@@ -188,7 +191,11 @@ struct KApplyZm {
   // The index is formed in 32 bits (3 N < 2^31) and selected, not branched on.
   // AX_P: the first iteration of a solve (its = 1 after its beta step)
   const bool pfirst = AXP && co.sc->its == 1;
-  auto fetch = [&](int zz, TR* vr, TD* vd, uint32_t* vf, TR* vc) {
+  // SMOOTH_R: no update in the first iteration of a solve (its = 0): q is
+  // not read (index 0, a halo entry, holds 0), so r = in exactly
+  const bool rfirst = RUP && co.sc->its == 0;
+  const TR ralpha = RUP ? (TR)co.sc->alpha : (TR)0;
+  auto fetch = [&](int zz, TR* vr, TD* vd, uint32_t* vf, TR* vc, TR* vq) {
     const int szz = (int)sz * zz;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -200,6 +207,7 @@ struct KApplyZm {
         // AX_P, first iteration: beta = 0 and p_old is not read (index 0, a
         // halo entry, holds 0)
         vr[i] = in[pfirst ? 0u : m];
+        if (RUP) vq[i] = co.qv[rfirst ? 0u : m];
         if (SM) vd[i] = (TD)op.invd[m];
         if (AXP) vd[i] = (TD)co.zf[m];
       }
@@ -232,6 +240,7 @@ struct KApplyZm {
   TD dA[ND], dB[ND];
   uint32_t gA[PF], gB[PF];                          // PRO: flags of the staged positions
   TR cA[PC], cB[PC];                                // PRO: the coarse patch
+  TR qA[RUP ? PER : 1], qB[RUP ? PER : 1];          // SMOOTH_R: q
   T beta = 0, alpha = 0, alpha2 = 0;
   bool xup = false, xzero = false;
   if (AXP) {
@@ -244,8 +253,8 @@ struct KApplyZm {
     xup = XUP && co.sc->its != 1;
     xzero = co.sc->its == 3;
   }
-  fetch(z0 - 1, rA, dA, gA, cA);
-  fetch(z0, rB, dB, gB, cB);
+  fetch(z0 - 1, rA, dA, gA, cA, qA);
+  fetch(z0, rB, dB, gB, cB, qB);
   // flags of the outputs this lane owns, loaded one plane before they are used
   auto flags_at = [&](const int z) -> uint32_t {
     const bool ok = lane_ok && z >= z0 && z < z1 && z >= own_lo && z <= own_hi;
@@ -253,7 +262,8 @@ struct KApplyZm {
   };
   uint32_t fN = flags_at(z0);
 
-  auto plane = [&](const int zz, TR (&sr)[PRO ? 1 : PER], TD (&sd)[ND], uint32_t (&sf)[PF], TR (&sc)[PC]) {
+  auto plane = [&](const int zz, TR (&sr)[PRO ? 1 : PER], TD (&sd)[ND], uint32_t (&sf)[PF], TR (&sc)[PC],
+                   TR (&sq)[RUP ? PER : 1]) {
     T (*const sB)[SX][SY] = sU[kb];
     if constexpr (PRO) {
       // e = P x_c at the staged positions (kernel/InterpolationDeviation.m:3-17)
@@ -294,6 +304,8 @@ struct KApplyZm {
       for (int i = 0; i < PER; ++i) {
         const int e = tid + i * NT;
         if (e < NSTAGE) {
+          // (SMOOTH_R: r = in - alpha q, kept in sr for the centre values)
+          if (RUP) sr[i] = fma_t(-ralpha, sq[i], sr[i]);
           T v = (T)sr[i];
           if (SM) v = omega * (T)sd[i] * v;
           if (AXP) v = (T)sd[i] + beta * v;              // p = z + beta p_old (CG_solver.m:28)
@@ -313,7 +325,7 @@ struct KApplyZm {
         const int e = tid + i * NT;
         if (e < NSTAGE) {
           const int c = e / (SX * SY), rem = e % (SX * SY), xx = rem / SY, yy = rem % SY;
-          if (xx >= 1 && xx <= TX && yy >= 1 && yy <= TY) sR[zz & 1][c][xx - 1][yy - 1] = (T)sr[i];
+          if (xx >= 1 && xx <= TX && yy >= 1 && yy <= TY) sR[zz & 1][c][xx - 1][yy - 1] = (TC)sr[i];
         }
       }
     }
@@ -426,7 +438,7 @@ struct KApplyZm {
       aM[1] -= D1;
       aM[2] -= D2;
     }
-    if (zz + 2 <= z1) fetch(zz + 2, sr, sd, sf, sc);
+    if (zz + 2 <= z1) fetch(zz + 2, sr, sd, sf, sc, sq);
     // flags of the output at zz+1 (only outputs in [z0, z1) are owned)
     fP = fN;
     fN = flags_at(zz + 2);
@@ -503,7 +515,7 @@ struct KApplyZm {
           out[m] = a;
           if (XUP && xup) {
             const T p2 = co.pn[m];
-            co.x[m] = fma_t(alpha, sR[zo_ & 1][c][lx][ly], fma_t(alpha2, p2, cen[c]));
+            co.x[m] = fma_t(alpha, (T)sR[zo_ & 1][c][lx][ly], fma_t(alpha2, p2, cen[c]));
           }
           co.pn[m] = pn;
           if (partial) dsum += (double)pn * (double)a;
@@ -532,7 +544,7 @@ struct KApplyZm {
           const long long m = c * N + n;
           const double r0 = co.r0[m];
           const bool fx = (fM & (1u << (F_FIX_SHIFT + c))) != 0;
-          const T q0 = sR[zo_ & 1][c][lx][ly];
+          const T q0 = (T)sR[zo_ & 1][c][lx][ly];
           const T q1 = q0 - a;
           T v = op.invd[m] * (omega * ((T)r0 + q0) + co.w1 * (co.R1[m] + q1)) + co.e[m];
           if (fx) v = 0;
@@ -543,7 +555,13 @@ struct KApplyZm {
           }
         } else {
           const bool fx = (fM & (1u << (F_FIX_SHIFT + c))) != 0;
-          out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : sR[zo_ & 1][c][lx][ly] - a;
+          out[c * N + n] = (fx && op.zero_fixed) ? (T)0 : (T)sR[zo_ & 1][c][lx][ly] - a;
+          if (RUP) {
+            // the new residual of the PCG and its square (CG_solver.m:33-35)
+            const TR rv = sR[zo_ & 1][c][lx][ly];
+            co.rn[c * N + n] = rv;
+            dsum += (double)rv * (double)rv;
+          }
         }
       }
     }
@@ -559,8 +577,8 @@ struct KApplyZm {
     kb = (kb + 1) & 3;
   };
   for (int zz = z0 - 1; zz <= z1; zz += 2) {
-    plane(zz, rA, dA, gA, cA);
-    if (zz + 1 <= z1) plane(zz + 1, rB, dB, gB, cB);
+    plane(zz, rA, dA, gA, cA, qA);
+    if (zz + 1 <= z1) plane(zz + 1, rB, dB, gB, cB, qB);
   }
   if (partial) {
     // block reduction (fixed order)
@@ -673,6 +691,10 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   if ((mode == AP_AX_P || mode == AP_AX_P_NOX) &&
       (!co || e != 0 || !co->zf || !co->pn || (mode == AP_AX_P && !co->x) || !co->sc || co->pn == (const void*)in))
     throw std::invalid_argument("AX_P: level 1 only, with z, x, the scalars and a separate new p");
+  if (mode == AP_SMOOTH_R &&
+      (!co || e != 0 || !co->qv || !co->rn || !co->sc || !partial || co->rn == (const void*)in ||
+       co->rn == co->qv))
+    throw std::invalid_argument("SMOOTH_R: level 1 only, with q, the scalars, the partials and a separate new r");
 #define GMGS_ZM(MODE, E, TXV) \
   launch(KApplyZm<T, TR, MODE, E, TXV>{op, tiles, zc, in, omega, in2, out, partial, done, cov}, ntiles, s)
 #define GMGS_ZM_TX(MODE, E)               \
@@ -706,6 +728,9 @@ int launch_apply_zm(const LevelOp<T>& op, const ColTile* tiles, long long ntiles
   } else if (mode == AP_AX_P_NOX) {
     if constexpr (std::is_same<T, double>::value && std::is_same<TR, double>::value) GMGS_ZM_TX(AP_AX_P_NOX, 0);
     else throw std::invalid_argument("AX_P: FP64 operator and input");
+  } else if (mode == AP_SMOOTH_R) {
+    if constexpr (std::is_same<T, float>::value && std::is_same<TR, double>::value) GMGS_ZM_TX(AP_SMOOTH_R, 0);
+    else throw std::invalid_argument("SMOOTH_R: FP32 operator, FP64 residual");
   } else {
     GMGS_ZM_E(AP_SMOOTH);
   }

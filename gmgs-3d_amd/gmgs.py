@@ -38,6 +38,7 @@ FLAG_CHEBYSHEV = 512
 FLAG_ASSEMBLED_COARSE = 1024
 FLAG_SPLIT_COARSE = 2048
 FLAG_UNFUSED_PROLONG = 4096
+FLAG_UNFUSED_RESIDUAL = 8192
 
 Q = dict(NUM_LEVELS=0, RES=1, NUM_ELEMENTS=2, NUM_NODES=3, ELE_MAP_BACK=4, ELE_MAP_FORWARD=5,
          NOD_MAP_BACK=6, NOD_MAP_FORWARD=7, ENODMAT=8, NUM_NOD2ELES=9, NODES_ON_BOUNDARY=10,

@@ -81,11 +81,18 @@ enum {
                                      present and universal, the assembled rows on the others (one
                                      lane per row); elsewhere as GMGS_FLAG_ASSEMBLED_COARSE.
                                      GMGS_FLAG_AUTOTUNE also times this operator per level */
-  GMGS_FLAG_UNFUSED_PROLONG = 4096 /* symmetric V(1, 1) cycle on a z-marching level 1: store
+  GMGS_FLAG_UNFUSED_PROLONG = 4096, /* symmetric V(1, 1) cycle on a z-marching level 1: store
                                      e = P x_c with a prolongation kernel of its own before the
                                      residual kernel reads it, instead of forming each value of e
                                      while the residual kernel stages it (same results; for
                                      cross-checks and timing) */
+  GMGS_FLAG_UNFUSED_RESIDUAL = 8192 /* mixed solver without the Jacobi step, symmetric cycle on
+                                     a z-marching level 1: update the PCG residual
+                                     r = r - alpha q (and its norm) in a pointwise kernel of its
+                                     own after the matvec, instead of in the next iteration's
+                                     first level-1 sweep, which stages r - alpha q (same
+                                     iterates; the norm's partial sums are grouped
+                                     differently) */
 };
 
 typedef struct gmgs_params {

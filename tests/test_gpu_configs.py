@@ -115,6 +115,53 @@ def test_c2_fused_prolongation_bitwise(pkg, c2):
     assert np.array_equal(Uf, Uu)
 
 
+def test_c2_fused_residual_update_same_iterates(pkg, c2):
+    """The PCG residual update r = r - alpha q carried by the next iteration's
+    first level-1 sweep (which stages r - alpha q and writes r and the |r|
+    partials) gives the same iterates as the pointwise update after the
+    matvec: same r bits (the same FMA), so the same solution bit for bit and
+    the same iteration count; only |r|, a sum grouped by column tile instead
+    of by pointwise block, may differ in its last bits."""
+    out = []
+    for extra in (0, pkg.FLAG_UNFUSED_RESIDUAL):
+        s = pkg.Solver(c2.vox, c2.bbox, num_levels=c2.num_levels, jacobi_weight=c2.omega, tol=1e-8,
+                       max_it=3000, precision=pkg.GMGS_PREC_MIXED, flags=headline_flags(pkg, False) | extra,
+                       **HEADLINE)
+        s.set_bc(c2.fixed, c2.loads)
+        U, info, h = s.solve()
+        s.close()
+        out.append((U, info, h))
+        log(f"  {'unfused' if extra else 'fused'}: its {info.its}, true {info.true_relres:.2e}")
+    (Uf, jf, hf), (Uu, ju, hu) = out
+    assert jf.converged and ju.converged and jf.its == ju.its
+    assert hf.shape == hu.shape
+    np.testing.assert_allclose(hf[:, 2], hu[:, 2], rtol=1e-12, atol=0)
+    assert np.array_equal(Uf, Uu)
+    assert jf.true_relres < 1e-8 * 1.5
+
+
+def test_max_it_cap_with_fused_residual_update(pkg, c2):
+    """A solve stopped by max_it runs as many iterations, records the same
+    history rows and reports the last residual with the residual update fused
+    into the next iteration's sweep as without it."""
+    out = []
+    for extra in (0, pkg.FLAG_UNFUSED_RESIDUAL):
+        s = pkg.Solver(c2.vox, c2.bbox, num_levels=c2.num_levels, jacobi_weight=c2.omega, tol=1e-14,
+                       max_it=4, precision=pkg.GMGS_PREC_MIXED, flags=headline_flags(pkg, False) | extra,
+                       **HEADLINE)
+        s.set_bc(c2.fixed, c2.loads)
+        U, info, h = s.solve(raise_on_nonconvergence=False)
+        s.close()
+        out.append((U, info, h))
+        log(f"  {'unfused' if extra else 'fused'}: its {info.its}, relres {info.relres:.3e}, rows {len(h)}")
+    (Uf, jf, hf), (Uu, ju, hu) = out
+    assert not jf.converged and not ju.converged
+    assert jf.its == ju.its >= 4 and len(hf) == len(hu) == jf.its
+    np.testing.assert_allclose(hf[:, 2], hu[:, 2], rtol=1e-12, atol=0)
+    assert abs(jf.relres - hf[-1, 2]) <= 1e-12 * hf[-1, 2]
+    assert np.array_equal(Uf, Uu)
+
+
 def test_c3_mixed_vs_fp64_reference_tight(pkg):
     t0 = time.time()
     p = pkg.workloads.ellipsoids_c3(0)
