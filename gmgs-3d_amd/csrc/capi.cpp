@@ -79,6 +79,11 @@ struct Level {
   DevBuf coltiles64;
   long long ncol64 = 0;
   int col_tx64 = 0, col_zc64 = 0;
+  // level 1 of a mixed solver: column tiles of the first sweep when it carries
+  // the PCG's residual update (AP_SMOOTH_R), when tuned apart (else coltiles)
+  DevBuf coltilesR;
+  long long ncolR = 0;
+  int col_txR = 0, col_zcR = 0;
   // symmetric cycle: nu sweeps with weights om, and the estimate of
   // lambda_max(D^-1 A) on the free DOFs they came from (0 = plain Jacobi)
   int nu = 2;
@@ -814,7 +819,7 @@ float time_split(gmgs_ctx* c, int li, const ColTile* tiles, long long nt, int tx
 long long partial_capacity(gmgs_ctx* c) {
   long long n = c->nparts;
   for (const Level& l : c->lev) {
-    n = std::max<long long>(n, std::max<long long>(apply_blocks(l.ntiles), std::max(l.ncol, l.ncol64)));
+    n = std::max<long long>(n, std::max<long long>(apply_blocks(l.ntiles), std::max({l.ncol, l.ncol64, l.ncolR})));
     if (l.split_ok) n = std::max<long long>(n, l.ncol + (long long)apply_blocks(l.nrbox));
   }
   return n;
@@ -864,6 +869,8 @@ void autotune_levels(gmgs_ctx* c) {
     // update is timed on that sweep and cached under its own key)
     const bool rup = li == 0 && c->mixed && c->no_jacobi && c->sym && !c->unfused_resid;
     const TuneKey key{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, c->mixed ? (rup ? 6 : 1) : 0};
+    struct SweepTime { int tx, zc; float ms; };
+    std::vector<SweepTime> sweep_times;   // rup, V(1, 1): the first sweep per shape
     // (tx, zc) of the z-marching kernel; (-tx, zc) the split operator on those
     // tiles; (0, 0) the tile kernel, (0, 1) the assembled stencil
     std::pair<int, int> best{l.zm ? l.col_tx : 0, l.zm ? l.col_zc : (l.as ? 1 : 0)};
@@ -914,10 +921,15 @@ void autotune_levels(gmgs_ctx* c) {
               // (the sweep's |r| partials, one per tile of this shape)
               DevBuf tpart;
               if (rup) tpart.alloc(sizeof(double) * (size_t)std::max(nt, 1LL));
-              ms = rup ? time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
-                                                l.rp.as<float>(), AP_SMOOTH_R, &rco, nullptr, tpart.as<double>())
-                       : time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
-                                                l.rp.as<float>());
+              const float sweep_ms =
+                  rup ? time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
+                                               l.rp.as<float>(), AP_SMOOTH_R, &rco, nullptr, tpart.as<double>())
+                      : time_zm<float, double>(c, 0, tl.as<ColTile>(), nt, cd[0], cd[1], c->tmp.as<double>(),
+                                               l.rp.as<float>());
+              // V(1, 1) with the residual update in the sweep: the sweep gets
+              // tiles of its own (below), these serve the residual kernel
+              if (rup && v11) sweep_times.push_back({cd[0], cd[1], sweep_ms});
+              ms = rup && v11 ? 0.f : sweep_ms;
               // the symmetric V(1, 1) cycle runs its residual-and-sum kernel on
               // the same tiles: time it too (scratch: e or level 2's x, R_1 =
               // rp, r0 = tmp, z = z[1]), stand-alone after the prolongation
@@ -987,6 +999,33 @@ void autotune_levels(gmgs_ctx* c) {
       l.col_tx = tx;
       l.col_zc = best.second;
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
+    }
+    if (li == 0 && rup && v11 && l.zm) {
+      // the first sweep, which carries the PCG's residual update, on tiles of
+      // its own when another shape suits it better (cached under its own key)
+      const TuneKey rkey{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, 7};
+      std::pair<int, int> rbest{l.col_tx, l.col_zc};
+      bool have = false;
+      {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        if (!sweep_times.empty()) {
+          float bms = 1e30f;
+          for (const auto& st : sweep_times)
+            if (clearly_faster(st.ms, bms)) { bms = st.ms; rbest = {st.tx, st.zc}; }
+          tune_cache()[rkey] = rbest;
+          have = true;
+        } else {
+          auto it = tune_cache().find(rkey);
+          if (it != tune_cache().end()) { rbest = it->second; have = true; }
+        }
+      }
+      l.coltilesR = DevBuf();
+      l.ncolR = 0;
+      if (have && (rbest.first != l.col_tx || rbest.second != l.col_zc)) {
+        l.col_txR = rbest.first;
+        l.col_zcR = rbest.second;
+        l.ncolR = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_txR, l.col_zcR, l.coltilesR, s);
+      }
     }
   }
   // level 1 of a mixed context: the FP64 matvec of the PCG on tiles of its own
@@ -1185,9 +1224,12 @@ int vcycle_sym(gmgs_ctx* c, const double* rin, T* z, const T* zprev, double* par
         co.sc = c->sc.as<PcgScalars>();
         co.qv = ru->q;
         co.rn = const_cast<double*>(rin);
-        const int nr = launch_apply_zm<T, double>(make_op<T>(c, 0, true), l.coltiles.as<ColTile>(), l.ncol,
-                                                  l.col_tx, l.col_zc, AP_SMOOTH_R, ru->r_old, (T)l.om[0],
-                                                  nullptr, Rbuf<T>(l, 1), partial, done, s, &co);
+        const bool own = !l.coltilesR.empty();
+        const int nr = launch_apply_zm<T, double>(make_op<T>(c, 0, true),
+                                                  own ? l.coltilesR.as<ColTile>() : l.coltiles.as<ColTile>(),
+                                                  own ? l.ncolR : l.ncol, own ? l.col_txR : l.col_tx,
+                                                  own ? l.col_zcR : l.col_zc, AP_SMOOTH_R, ru->r_old,
+                                                  (T)l.om[0], nullptr, Rbuf<T>(l, 1), partial, done, s, &co);
         // |r| and the convergence test (CG_solver.m:35-39); the rest of the
         // iteration returns at once once it is met
         launch_finalize(3, partial, nr, c->sc.as<PcgScalars>(), c->hist_rel.as<double>(),
