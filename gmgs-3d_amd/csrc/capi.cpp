@@ -1000,9 +1000,12 @@ void autotune_levels(gmgs_ctx* c) {
       l.col_zc = best.second;
       l.ncol = build_coltiles(l.mesh.g, l.mesh.flags.as<uint32_t>(), l.col_tx, l.col_zc, l.coltiles, s);
     }
-    if (li == 0 && rup && v11 && l.zm) {
+    if (li == 0 && rup && v11 && l.zm && l.mesh.g.N >= (1LL << 22)) {
       // the first sweep, which carries the PCG's residual update, on tiles of
-      // its own when another shape suits it better (cached under its own key)
+      // its own when another shape suits it better (cached under its own key).
+      // Only on large grids: a batch of small problems merges the launches of
+      // problems whose kernels have equal shapes, and one more tuned shape
+      // per problem splits it into more groups for little gain.
       const TuneKey rkey{l.mesh.g.ey, l.mesh.g.ex, l.mesh.g.ez, 7};
       std::pair<int, int> rbest{l.col_tx, l.col_zc};
       bool have = false;
