@@ -6,7 +6,9 @@
   tests/golden/make_c2_fixture.py): iterations within 1, the iterationHist_
   relres column, and U on 10^4 sampled DOFs within 1e-6; the mixed-precision
   solver at rtol 1e-10 against the oracle's rtol-1e-10 solution, U and the
-  nodal stress on sampled nodes within 1e-6.
+  nodal stress on sampled nodes within 1e-6; the two fused forms of the
+  headline solver's level-1 work (prolongation formed by the residual kernel,
+  residual update carried by the first sweep) against their unfused forms.
 * 337x294x512 (seed 0, 6 levels, 56,994,411 DOFs): the headline mixed solver
   against the device's FP64 reference algorithm, both at rtol 1e-10;
   displacement and stress fields within 1e-6 relative L2, both true
